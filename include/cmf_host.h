@@ -1,0 +1,84 @@
+/*
+ * cmf_host.h -- host-side (CPU) map initialisation for the MI355X CaMa-Flood physics.
+ *
+ * These routines produce the static tables that the device physics consumes. In a
+ * Fortran-hosted run the reference's own CMF_TOPO_INIT / CMF_LEVEE_INIT compute the
+ * same tables and pass them through cmf_hip.h; these C entry points exist so that a
+ * host without the Fortran layer (the Python host mirror, tests, benchmarks) builds
+ * bit-identical tables. Each function cites the reference routine it restates.
+ *
+ * Conventions: all per-cell arrays are in the reference's 1-D vector (NSEQ) order,
+ * 0-based. Floodplain tables are layer-planar: tab[l*n + i], l = 0..nlfp-1.
+ * 2-D maps are stored with x fastest (Fortran (NX,NY) column order): map[iy*nx+ix].
+ * Every function returns 0 on success and a negative code on bad arguments.
+ */
+#ifndef CMF_HOST_H
+#define CMF_HOST_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* River-mouth / inland-end codes of the reference next-xy map (nextxy.bin). */
+#define CMF_NEXT_MOUTH   (-9)
+#define CMF_NEXT_INLAND  (-10)
+#define CMF_NEXT_NONE    (-9999)
+
+/* Topological 1-D ordering of the river network.
+ * Restates CALC_1D_SEQ, src/cmf_ctrl_maps_mod.F90:403-525 (single region).
+ * nextx/nexty: 1-based downstream cell indices (or negative codes), x fastest.
+ * Outputs (capacity nx*ny): seqx/seqy 1-based grid indices of each cell,
+ * next[i] = 0-based downstream cell, or the negative mouth code for mouths.
+ * Returns the number of cells (NSEQALL) and writes NSEQRIV to *nseqriv. */
+int cmfh_calc_1d_seq(int nx, int ny, const int32_t* nextx, const int32_t* nexty,
+                     int32_t* seqx, int32_t* seqy, int32_t* next, int32_t* nseqriv);
+
+/* Upstream lists as CSR in ascending source order (the fill order of I1UPST,
+ * src/cmf_ctrl_maps_mod.F90:512-523). up_ptr has n+1 entries; up_idx capacity n. */
+int cmfh_build_upstream_csr(int n, int nseqriv, const int32_t* next,
+                            int32_t* up_ptr, int32_t* up_idx);
+
+/* Channel and floodplain tables: RIVSTOMAX, RIVELV (src/cmf_ctrl_maps_mod.F90:716-722)
+ * and SET_FLDSTG (src/cmf_ctrl_maps_mod.F90:969-999). lfplain=0 sets RIVSTOMAX=1e18. */
+int cmfh_set_fldstg(int n, int nlfp, int lfplain,
+                    const double* grarea, const double* rivlen, const double* rivwth,
+                    const double* rivhgt, const double* elevtn, const double* fldhgt,
+                    double* rivstomax, double* rivelv, double* fldstomax, double* fldgrd,
+                    double* dfrcinc);
+
+/* Levee tables, CMF_LEVEE_INIT section [2], src/cmf_ctrl_levee_mod.F90:148-261.
+ * levhgt/levfrc are in/out (sanitised as the reference does). fldstomax/fldgrd are
+ * recomputed in place exactly as the reference does at :169-181. */
+int cmfh_levee_init(int n, int nlfp,
+                    const double* grarea, const double* rivlen, const double* rivwth,
+                    const double* rivstomax, const double* fldhgt,
+                    double* fldstomax, double* fldgrd,
+                    double* levhgt, double* levfrc,
+                    double* bashgt, double* levdst, double* levbassto,
+                    double* levtopsto, double* levfilsto);
+
+/* Cold start: STORAGE_SEA_SURFACE, src/cmf_ctrl_vars_mod.F90:129-168.
+ * Writes rivsto (FP64) and rivdph_pre. */
+int cmfh_storage_sea_surface(int n, int nseqriv, const int32_t* next,
+                             const double* dwnelv, const double* rivelv, const double* rivhgt,
+                             const double* rivlen, const double* rivwth, const double* rivstomax,
+                             double* rivsto, double* rivdph_pre);
+
+/* ---- synthetic-network helpers (not reference restatements) ---- */
+
+/* Priority-flood D8 drainage (Barnes et al. 2014) on a lat-lon grid that wraps in x.
+ * land[k]!=0 marks land. Land cells next to sea (or beyond the y edges) seed the
+ * queue as river mouths (next = CMF_NEXT_MOUTH); every other land cell drains to the
+ * neighbour it was reached from. filled = depression-filled elevation, strictly
+ * increasing upstream by at least eps. nextx/nexty are 1-based or negative codes. */
+int cmfh_priority_flood(int nx, int ny, const float* elev, const uint8_t* land, float eps,
+                        float* filled, int32_t* nextx, int32_t* nexty);
+
+/* Upstream drainage area (sum of cell areas over the upstream tree, incl. the cell). */
+int cmfh_upstream_area(int nx, int ny, const int32_t* nextx, const int32_t* nexty,
+                       const double* cellarea, double* uparea);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+/*
+ * cmf_oracle.h -- CPU restatement of the reference's per-substep physics (FP64 mode).
+ *
+ * TEST INFRASTRUCTURE ONLY. This oracle is the checker for the HIP physics: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product (cama-flood_v4_amd/) never links, loads or calls it.
+ *
+ * It restates, function by function, the reference Fortran built in FP64 mode
+ * (no -DSinglePrec_CMF), with the default namelist options that the north-star
+ * path uses (LSPAMAT, local-inertial outflow, optional LFLDOUT/LPTHOUT/LDAMOUT/
+ * LLEVEE/LADPSTP). Every function cites the reference file:line it follows.
+ * Arithmetic keeps the reference's operation order and constant kinds: default-kind
+ * literals such as 1.E-5 are the float value widened to double; x**(-1.) is 1/x,
+ * x**0.5 is sqrt(x), x**(-0.5) is 1/sqrt(x), x**2 is x*x and x**3 is x*(x*x) (as
+ * ROCm flang lowers them); other real powers call pow().
+ *
+ * Arrays are 0-based in the reference's NSEQ vector order. Layer/level tables are
+ * planar: tab[l*n + i]. Pointers not needed by the enabled options may be NULL.
+ */
+#ifndef CMF_ORACLE_H
+#define CMF_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cmfo_model {
+    /* sizes */
+    int32_t n, nseqriv, nlfp, npth, npthlev, ndam;
+    /* options (namelist NRUNVER) */
+    int32_t ladpstp, lfldout, lpthout, ldamout, llevee;
+    /* parameters (namelist NPARAM) */
+    double pgrv, pmanfld, pdstmth, pcadp, pminslp, dfrcinc;
+    /* topology */
+    const int32_t* next;       /* downstream cell (0-based) or negative mouth code */
+    const int32_t* up_ptr;     /* upstream CSR, ascending source order */
+    const int32_t* up_idx;
+    /* static map (JPRB, FP64 here) */
+    const double *grarea, *elevtn, *nxtdst, *rivlen, *rivwth, *rivhgt, *rivman;
+    const double *rivstomax, *rivelv, *dwnelv;
+    const double *fldstomax, *fldgrd, *fldhgt;     /* [nlfp][n] */
+    const int32_t* i2mask;
+    /* bifurcation paths */
+    const int32_t *pth_upst, *pth_down;            /* 0-based cell or -1 outside */
+    const double *pth_dst, *pth_elv, *pth_wth, *pth_man;   /* elv/wth [npthlev][npth] */
+    const int32_t *pout_ptr, *pout_idx, *pinf_ptr, *pinf_idx;  /* per-cell path lists */
+    /* levee tables */
+    const double *levhgt, *levfrc, *bashgt, *levdst, *levbassto, *levtopsto, *levfilsto;
+    /* dams */
+    const int32_t *i1dam, *damseq, *damstat;
+    const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
+    /* prognostic state */
+    double *rivsto, *fldsto, *levsto, *damsto, *daminf;
+    double *rivout, *fldout, *rivout_pre, *fldout_pre, *rivdph_pre, *fldsto_pre;
+    double *pthflw, *pthflw_pre;                   /* [npthlev][npth] */
+    double *runoff;
+    /* diagnostic state */
+    double *rivdph, *flddph, *fldfrc, *fldare, *sfcelv, *storge, *outflw, *rivvel;
+    double *rivinf, *fldinf, *pthout, *levdph, *pthflwsum;
+    /* substep averages / maxima (CMF_DIAG_*_ADPSTP) */
+    double *rivout_avg, *fldout_avg, *outflw_avg, *rivvel_avg, *pthout_avg, *runoff_avg;
+    double *outflw_max, *rivdph_max, *storge_max;
+    double *pthflw_avg, *pthflwsum_avg;            /* [npthlev][npth], [npth] */
+    double nadd;
+    /* global sums of the last substep: STOPRE STONXT STONEW RIVINF RIVOUT
+     * STOPRE2 STONEW2 RIVSTO FLDSTO FLDARE LEVSTO */
+    double glb[11];
+    /* work arrays, each of length n */
+    double *w_sfcelv_pre, *w_dwnelv_pre, *w_flddph_pre, *w_stoout, *w_rate;
+    double *w_rivinf, *w_fldinf, *w_pthout;
+} cmfo_model;
+
+/* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */
+void cmfo_physics_fldstg(cmfo_model* m);
+/* CMF_PHYSICS_ADVANCE (src/cmf_ctrl_physics_mod.F90:21-155); dt is DT (in/out as
+ * in the reference, restored on return). Returns NT. nt_cap>0 limits the number of
+ * substeps run (for bounded timing samples); 0 runs all NT. */
+int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap);
+
+/* single routines, exposed for unit tests */
+void cmfo_calc_fldstg_def(cmfo_model* m);
+void cmfo_levee_fldstg(cmfo_model* m);
+int32_t cmfo_calc_adpstp(cmfo_model* m, double dt_def, double* dt);
+void cmfo_calc_outflw(cmfo_model* m, double dt);
+void cmfo_calc_pthout(cmfo_model* m, double dt);
+void cmfo_levee_opt_pthout(cmfo_model* m, double dt);
+void cmfo_damout_calc(cmfo_model* m, double dt);
+void cmfo_calc_inflow_lspamat(cmfo_model* m, double dt);
+void cmfo_damout_watbal(cmfo_model* m, double dt);
+void cmfo_calc_vars_pre(cmfo_model* m);
+void cmfo_calc_stonxt(cmfo_model* m, double dt);
+void cmfo_diag_reset_adpstp(cmfo_model* m);
+void cmfo_diag_avemax_adpstp(cmfo_model* m, double dt);
+void cmfo_diag_getave_adpstp(cmfo_model* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,139 @@
+"""Python binding of the C oracle (cmf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker of the HIP physics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_P = C.c_void_p
+_FIELDS = [
+    ("n", C.c_int32), ("nseqriv", C.c_int32), ("nlfp", C.c_int32), ("npth", C.c_int32),
+    ("npthlev", C.c_int32), ("ndam", C.c_int32),
+    ("ladpstp", C.c_int32), ("lfldout", C.c_int32), ("lpthout", C.c_int32),
+    ("ldamout", C.c_int32), ("llevee", C.c_int32),
+    ("pgrv", C.c_double), ("pmanfld", C.c_double), ("pdstmth", C.c_double),
+    ("pcadp", C.c_double), ("pminslp", C.c_double), ("dfrcinc", C.c_double),
+]
+_PTRS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax rivelv "
+         "dwnelv fldstomax fldgrd fldhgt i2mask pth_upst pth_down pth_dst pth_elv pth_wth pth_man "
+         "pout_ptr pout_idx pinf_ptr pinf_idx levhgt levfrc bashgt levdst levbassto levtopsto "
+         "levfilsto i1dam damseq damstat dam_convol dam_emevol dam_adjvol dam_qa dam_qn dam_qf "
+         "rivsto fldsto levsto damsto daminf rivout fldout rivout_pre fldout_pre rivdph_pre "
+         "fldsto_pre pthflw pthflw_pre runoff rivdph flddph fldfrc fldare sfcelv storge outflw "
+         "rivvel rivinf fldinf pthout levdph pthflwsum rivout_avg fldout_avg outflw_avg rivvel_avg "
+         "pthout_avg runoff_avg outflw_max rivdph_max storge_max pthflw_avg pthflwsum_avg").split()
+_TAIL = [("nadd", C.c_double), ("glb", C.c_double * 11)]
+_WORK = "w_sfcelv_pre w_dwnelv_pre w_flddph_pre w_stoout w_rate w_rivinf w_fldinf w_pthout".split()
+
+
+class _Model(C.Structure):
+    _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK]
+
+
+MAP_KEYS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax "
+            "rivelv dwnelv fldstomax fldgrd fldhgt i2mask").split()
+PTH_KEYS = "pth_upst pth_down pth_dst pth_elv pth_wth pth_man pout_ptr pout_idx pinf_ptr pinf_idx".split()
+LEV_KEYS = "levhgt levfrc bashgt levdst levbassto levtopsto levfilsto".split()
+DAM_KEYS = "i1dam damseq damstat dam_convol dam_emevol dam_adjvol dam_qa dam_qn dam_qf".split()
+CELL_STATE = ("rivsto fldsto rivout fldout rivout_pre fldout_pre rivdph_pre fldsto_pre runoff "
+              "rivdph flddph fldfrc fldare sfcelv storge outflw rivvel rivinf fldinf pthout "
+              "rivout_avg fldout_avg outflw_avg rivvel_avg pthout_avg runoff_avg outflw_max "
+              "rivdph_max storge_max").split()
+GLB_NAMES = ("stopre stonxt stonew rivinf rivout stopre2 stonew2 rivsto fldsto fldare levsto").split()
+
+
+def _lib():
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"{LIB} missing: run make -C oracle")
+    lib = C.CDLL(LIB)
+    lib.cmfo_physics_advance.restype = C.c_int32
+    lib.cmfo_physics_advance.argtypes = [C.POINTER(_Model), C.POINTER(C.c_double), C.c_int32]
+    lib.cmfo_calc_adpstp.restype = C.c_int32
+    lib.cmfo_calc_adpstp.argtypes = [C.POINTER(_Model), C.c_double, C.POINTER(C.c_double)]
+    for f in ("cmfo_physics_fldstg", "cmfo_calc_fldstg_def", "cmfo_levee_fldstg",
+              "cmfo_calc_vars_pre", "cmfo_diag_reset_adpstp", "cmfo_diag_getave_adpstp"):
+        getattr(lib, f).restype = None
+        getattr(lib, f).argtypes = [C.POINTER(_Model)]
+    for f in ("cmfo_calc_outflw", "cmfo_calc_pthout", "cmfo_levee_opt_pthout", "cmfo_damout_calc",
+              "cmfo_calc_inflow_lspamat", "cmfo_damout_watbal", "cmfo_calc_stonxt",
+              "cmfo_diag_avemax_adpstp"):
+        getattr(lib, f).restype = None
+        getattr(lib, f).argtypes = [C.POINTER(_Model), C.c_double]
+    return lib
+
+
+class OracleModel:
+    """FP64 CPU oracle over a cmf.maps.CMFMap and a prognostic state dict."""
+
+    def __init__(self, m, state: dict):
+        self.lib = _lib()
+        self.map = m
+        o = m.opts
+        self.s = _Model()
+        s = self.s
+        s.n, s.nseqriv, s.nlfp = m.n, m.nseqriv, m.nlfp
+        s.npth, s.npthlev, s.ndam = m.npth, m.npthlev, m.ndam
+        s.ladpstp, s.lfldout, s.lpthout = int(o.ladpstp), int(o.lfldout), int(o.lpthout)
+        s.ldamout, s.llevee = int(o.ldamout), int(o.llevee)
+        s.pgrv, s.pmanfld, s.pdstmth, s.pcadp, s.pminslp = o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp
+        s.dfrcinc = m.dfrcinc
+        self._keep = {}
+        n = m.n
+        for k in MAP_KEYS:
+            self._bind(k, getattr(m, k))
+        if o.lpthout:
+            for k in PTH_KEYS:
+                self._bind(k, getattr(m, k))
+        if o.llevee:
+            for k in LEV_KEYS:
+                self._bind(k, getattr(m, k))
+        if o.ldamout:
+            for k in DAM_KEYS:
+                self._bind(k, getattr(m, k))
+        self.st = {}
+        for k in CELL_STATE:
+            self.st[k] = np.ascontiguousarray(state.get(k, np.zeros(n)), dtype=np.float64).copy()
+        if o.llevee:
+            self.st["levsto"] = np.array(state.get("levsto", np.zeros(n)), np.float64)
+            self.st["levdph"] = np.array(state.get("levdph", np.zeros(n)), np.float64)
+        if o.ldamout:
+            self.st["damsto"] = np.array(state["damsto"], np.float64)
+            self.st["daminf"] = np.array(state.get("daminf", np.zeros(n)), np.float64)
+        if o.lpthout:
+            shp = (m.npthlev, m.npth)
+            for k in ("pthflw", "pthflw_pre", "pthflw_avg"):
+                self.st[k] = np.ascontiguousarray(state.get(k, np.zeros(shp)), np.float64).copy()
+            for k in ("pthflwsum", "pthflwsum_avg"):
+                self.st[k] = np.ascontiguousarray(state.get(k, np.zeros(m.npth)), np.float64).copy()
+        for k, a in self.st.items():
+            setattr(s, k, a.ctypes.data)
+        self.work = {w: np.zeros(n) for w in _WORK}
+        for w, a in self.work.items():
+            setattr(s, w, a.ctypes.data)
+
+    def _bind(self, k, a):
+        if a is None:
+            return
+        a = np.ascontiguousarray(a)
+        self._keep[k] = a
+        setattr(self.s, k, a.ctypes.data)
+
+    # reference entry points ------------------------------------------------
+    def physics_fldstg(self):
+        self.lib.cmfo_physics_fldstg(C.byref(self.s))
+
+    def physics_advance(self, dt: float, nt_cap: int = 0) -> int:
+        d = C.c_double(dt)
+        return int(self.lib.cmfo_physics_advance(C.byref(self.s), C.byref(d), nt_cap))
+
+    @property
+    def glb(self) -> dict:
+        return dict(zip(GLB_NAMES, list(self.s.glb)))
