@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X CaMa-Flood substep physics.
+
+Workload (one GPU): glb_06min-shaped synthetic network, 3600x1800 grid, ~1.5e6
+river cells, levees on 3% of cells (crown 2-8 m, distance fraction U(0.2,0.8)),
+2500 dams (capacity lognormal(ln 1e8 m3, 1.5), Yamazaki-Funato rule), hourly
+gamma(k=0.5, mean 1.5 mm/day) runoff, DT = 3600 s with adaptive substeps.
+
+A step is one outer step CMF_PHYSICS_ADVANCE (stage, CFL, NT substeps) over all
+cells with inputs resident in HBM. The metric is cell-substeps/s = N * sum(NT) / t.
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "cama-flood_v4_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell-substep (FP64), by kernel; see DESIGN.md "Roofline".
+BYTES_FP64 = {"outflow": 200.0, "rate": 64.0, "update": 244.0}
+BYTES_MIXED = {"outflow": 112.0, "rate": 40.0, "update": 124.0}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(nx, ny, seed, nrec):
+    from cmf import maps, synth
+    t0 = time.time()
+    net = synth.make_network(nx, ny, seed=seed)
+    synth.add_levees(net, 0.03, seed)
+    synth.add_dams(net, 2500, seed)
+    ro = synth.make_runoff(net, nrec, seed)
+    log(f"network {nx}x{ny}: {net.ncells} cells, {time.time() - t0:.1f}s")
+    opts = maps.Options(llevee=True, ldamout=True)
+    m = maps.build_map(net, opts)
+    log(f"map init {time.time() - t0:.1f}s")
+    return net, m, ro
+
+
+def device_model(net, m, precision):
+    import dataclasses
+    from cmf import maps
+    from cmf.model import CMFModel
+    st = maps.cold_start(m)
+    # the initial stage precedes CMF_DAMOUT_INIT (drv_control:243-252): no dam tables yet
+    gm0 = CMFModel(m, precision, opts=dataclasses.replace(m.opts, ldamout=False))
+    gm0.set_state(st)
+    gm0.physics_fldstg()                       # initial CMF_PHYSICS_FLDSTG on the device
+    st.update(gm0.get_state(["rivsto", "fldsto", "rivdph", "flddph", "fldfrc", "fldare",
+                             "sfcelv", "storge", "levsto", "levdph"]))
+    gm0.close()
+    maps.init_dams(m, net, st)                 # CMF_DAMOUT_INIT (host, after the stage)
+    gm = CMFModel(m, precision)
+    gm.set_state(st)
+    return gm
+
+
+def run_gpu(net, m, rvec, precision, steps, warmup, dt):
+    gm = device_model(net, m, precision)
+    rec = 0
+    for _ in range(warmup):
+        gm.set_runoff(rvec[rec % len(rvec)]); rec += 1
+        gm.physics_advance(dt)
+    gm.synchronize()
+    # timed region: runoff for each step is uploaded before the clock starts
+    sum_nt = 0
+    t_total = 0.0
+    for s in range(steps):
+        gm.set_runoff(rvec[rec % len(rvec)]); rec += 1
+        gm.synchronize()
+        t0 = time.perf_counter()
+        nt = gm.physics_advance(dt)
+        gm.synchronize()
+        t_total += time.perf_counter() - t0
+        sum_nt += nt
+    # kernel timing pass (HIP events on the model's stream)
+    gm.kernel_times(True)
+    gm.set_runoff(rvec[rec % len(rvec)])
+    gm.physics_advance(dt)
+    kt = gm.kernel_times(False)
+    gm.close()
+    return sum_nt, t_total, kt
+
+
+def cpu_baseline(net, ro, n_cells, dt, sample_steps):
+    """Reference OpenMP FP64 build timed on this host over a bounded sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "omp64", "cmf_ref")
+    if not os.path.exists(exe):
+        return None
+    from cmf import synth
+    from oracle import refdump
+    ncores = len(os.sched_getaffinity(0))
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        days = max(1, int(np.ceil(sample_steps * dt / 86400.0)))
+        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     llevee=True, ldamout=True)
+        # shorten the run to the sample: end time = start + sample_steps*dt
+        nml = open(os.path.join(d, "input_cmf.nam")).read()
+        import datetime as _dt
+        t1 = _dt.datetime(2001, 1, 1) + _dt.timedelta(seconds=sample_steps * dt)
+        nml = nml.replace(nml[nml.index("EYEAR"):nml.index("EHOUR")],
+                          f"EYEAR = {t1.year}\nEMON  = {t1.month}\nEDAY  = {t1.day}\n")
+        nml = nml.replace("EHOUR = 0", f"EHOUR = {t1.hour}")
+        open(os.path.join(d, "input_cmf.nam"), "w").write(nml)
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        r = subprocess.run([exe, "runoff.bin", "NONE", "0"], cwd=d, env=env, capture_output=True,
+                           text=True, timeout=900)
+        if r.returncode != 0:
+            log("reference run failed:", r.stderr[-500:])
+            return None
+        line = [l for l in r.stdout.splitlines() if l.startswith("REFTIME")][0].split()
+        t, nsteps, n = float(line[1]), int(line[2]), int(line[3])
+        nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+        cs = n * sum(nts)
+        return dict(value=cs / t, unit="cell-substeps/s", cores=ncores, kind="reference",
+                    sample=f"reference FP64 OpenMP (flang -O3 -fopenmp), {nsteps} outer steps of "
+                           f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
+
+
+def roofline(kt, n_cells, bytes_tab):
+    steps = max(kt[4], 1.0)
+    per = {"outflow": kt[0] / steps, "rate": kt[1] / steps, "update": kt[2] / steps}
+    dom = max(per, key=per.get)
+    ms = per[dom]
+    achieved = bytes_tab[dom] * n_cells / (ms * 1e-3) / 1e9
+    return dom, per, dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                          frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                          kernel=f"k_{dom}", avg_ms=round(ms, 4),
+                          bytes_per_cell_substep=bytes_tab[dom])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=3600)
+    ap.add_argument("--ny", type=int, default=1800)
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding is not "
+              "part of this build", file=sys.stderr)
+        sys.exit(2)
+    dt = 3600.0
+    net, m, ro = build_workload(a.nx, a.ny, a.seed, 4)
+    rvec = [m.vec(ro[r]) for r in range(ro.shape[0])]
+    n = m.n
+    log("FP64 run")
+    sum_nt, t, kt = run_gpu(net, m, rvec, "fp64", a.steps, a.warmup, dt)
+    value = n * sum_nt / t
+    dom, per, roof = roofline(kt, n, BYTES_FP64)
+    log(f"FP64: {value:.4g} cell-substeps/s, sum NT {sum_nt}, {t:.3f}s; kernels ms {per}")
+    mixed = None
+    if not a.no_mixed:
+        net2, m2, _ = net, None, None
+        from cmf import maps
+        m2 = maps.build_map(net, maps.Options(llevee=True, ldamout=True))
+        sum_nt2, t2, kt2 = run_gpu(net, m2, rvec, "mixed", a.steps, a.warmup, dt)
+        dom2, per2, roof2 = roofline(kt2, n, BYTES_MIXED)
+        mixed = dict(value=n * sum_nt2 / t2, unit="cell-substeps/s", sum_nt=sum_nt2,
+                     ms_per_step=1e3 * t2 / a.steps, roofline=roof2)
+        log(f"mixed: {mixed['value']:.4g} cell-substeps/s; kernels ms {per2}")
+    cpu = None
+    if not a.no_cpu:
+        log("CPU baseline (reference OpenMP)")
+        cpu = cpu_baseline(net, ro, n, dt, a.cpu_steps)
+    out = {
+        "metric": "cell-substeps/s (1 GPU, 1.5M cells) FP64 vs mixed+batched; spin-up speedup",
+        "value": value, "unit": "cell-substeps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * t / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"glb_06min-shaped synthetic network {a.nx}x{a.ny}, {n} cells, "
+                               f"3% levees, 2500 dams, hourly runoff, DT=3600 s adaptive",
+                   "cells": n, "sum_nt": sum_nt, "precision": "fp64"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "mixed": mixed,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

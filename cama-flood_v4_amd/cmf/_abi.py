@@ -18,7 +18,56 @@ HOST_FUNCS = {
     "cmfh_upstream_area": (I, [I, I, P, P, P, P]),
 }
 
-HIP_FUNCS: dict = {}
+
+class cmf_dims(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("nseqall", "nseqriv", "nlfp", "npthout", "npthlev", "ndam")]
+
+
+class cmf_options(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("ladpstp", "lfplain", "lfldout", "lpthout", "ldamout",
+                                           "llevee", "precision")] + \
+               [(k, C.c_double) for k in ("pgrv", "pmanfld", "pdstmth", "pcadp", "pminslp")]
+
+
+MAP_FIELDS = [("i1next", P)] + [(k, P) for k in (
+    "grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax rivelv dwnelv "
+    "fldhgt fldstomax fldgrd").split()] + [("dfrcinc", C.c_double)] + [(k, P) for k in (
+    "i2mask pth_upst pth_down pth_dst pth_elv pth_wth pth_man levhgt levfrc bashgt levdst "
+    "levbassto levtopsto levfilsto i1dam damseq damstat dam_convol dam_emevol dam_adjvol "
+    "dam_qa dam_qn dam_qf").split()]
+
+
+class cmf_map(C.Structure):
+    _fields_ = MAP_FIELDS
+
+
+STATE_FIELDS = ("rivsto fldsto levsto damsto daminf rivout fldout rivout_pre fldout_pre "
+                "rivdph_pre fldsto_pre pthflw pthflw_pre rivdph flddph fldfrc fldare sfcelv storge "
+                "outflw rivvel rivinf fldinf pthout levdph pthflwsum rivout_avg fldout_avg outflw_avg "
+                "rivvel_avg pthout_avg runoff_avg outflw_max rivdph_max storge_max daminf_avg "
+                "pthflw_avg pthflwsum_avg").split()
+
+
+class cmf_state(C.Structure):
+    _fields_ = [(k, P) for k in STATE_FIELDS]
+
+
+H = C.c_void_p
+HIP_FUNCS = {
+    "cmf_hip_create": (I, [C.POINTER(cmf_dims), C.POINTER(cmf_options), C.POINTER(H)]),
+    "cmf_hip_destroy": (None, [H]),
+    "cmf_hip_last_error": (C.c_char_p, []),
+    "cmf_hip_set_map": (I, [H, C.POINTER(cmf_map)]),
+    "cmf_hip_set_state": (I, [H, C.POINTER(cmf_state)]),
+    "cmf_hip_get_state": (I, [H, C.POINTER(cmf_state)]),
+    "cmf_hip_set_runoff": (I, [H, P]),
+    "cmf_hip_physics_fldstg": (I, [H]),
+    "cmf_hip_physics_advance": (I, [H, C.POINTER(C.c_double), C.POINTER(C.c_int32), P, C.c_int32]),
+    "cmf_hip_physics_advance_capped": (I, [H, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32),
+                                           P, C.c_int32]),
+    "cmf_hip_synchronize": (I, [H]),
+    "cmf_hip_kernel_times": (I, [H, C.c_int32, P]),
+}
 
 
 def _declare(lib, table):

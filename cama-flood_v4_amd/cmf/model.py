@@ -1,0 +1,182 @@
+"""Device-resident CaMa-Flood physics: the host-side mirror of CMF_CTRL_PHYSICS_MOD.
+
+``CMFModel`` owns one handle of the HIP library (include/cmf_hip.h) and exposes the
+reference's two physics entry points with their names and meaning:
+
+* ``physics_fldstg()``  -- CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305)
+* ``physics_advance(dt)`` -- CMF_PHYSICS_ADVANCE (:21-155); returns NT
+
+plus the state movement a device-resident model needs (``set_state``,
+``get_state``, ``set_runoff``). Failures raise ``CMFError`` with the library's
+message; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Iterable, Optional
+
+import numpy as np
+
+from . import _native
+from ._abi import STATE_FIELDS, cmf_dims, cmf_map, cmf_options, cmf_state, ptr
+from .maps import CMFMap
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+NGLB = 11
+GLB_NAMES = ("stopre", "stonxt", "stonew", "rivinf", "rivout", "stopre2", "stonew2",
+             "rivsto", "fldsto", "fldare", "levsto")
+
+
+class CMFError(RuntimeError):
+    pass
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class CMFModel:
+    def __init__(self, m: CMFMap, precision: str = "fp64", opts=None):
+        self.lib = _native.hip()
+        self.map = m
+        o = opts if opts is not None else m.opts
+        self.opts = o
+        self.prec = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[precision]
+        d = cmf_dims(m.n, m.nseqriv, m.nlfp, m.npth if o.lpthout else 0,
+                     m.npthlev if o.lpthout else 0, m.ndam if o.ldamout else 0)
+        op = cmf_options(int(o.ladpstp), int(o.lfplain), int(o.lfldout), int(o.lpthout),
+                         int(o.ldamout), int(o.llevee), self.prec,
+                         o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp)
+        h = C.c_void_p()
+        self._check(self.lib.cmf_hip_create(C.byref(d), C.byref(op), C.byref(h)))
+        self.h = h
+        self._keep = []
+        self._set_map(m)
+
+    # ------------------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.cmf_hip_last_error()
+            raise CMFError(f"cmf_hip error {rc}: {msg.decode() if msg else ''}")
+
+    def _p(self, a):
+        self._keep.append(a)
+        return ptr(a)
+
+    def _set_map(self, m: CMFMap):
+        o = self.opts
+        mp = cmf_map()
+        i1next = np.where(m.next >= 0, m.next + 1, m.next).astype(np.int32)
+        mp.i1next = self._p(i1next)
+        for k in ("grarea", "elevtn", "nxtdst", "rivlen", "rivwth", "rivhgt", "rivman",
+                  "rivstomax", "rivelv", "dwnelv", "fldhgt", "fldstomax", "fldgrd"):
+            setattr(mp, k, self._p(_f64(getattr(m, k))))
+        mp.dfrcinc = m.dfrcinc
+        mp.i2mask = self._p(_i32(m.i2mask))
+        if o.lpthout:
+            mp.pth_upst = self._p(_i32(np.where(m.pth_upst >= 0, m.pth_upst + 1, 0)))
+            mp.pth_down = self._p(_i32(np.where(m.pth_down >= 0, m.pth_down + 1, 0)))
+            for k in ("pth_dst", "pth_elv", "pth_wth", "pth_man"):
+                setattr(mp, k, self._p(_f64(getattr(m, k))))
+        if o.llevee:
+            for k in ("levhgt", "levfrc", "bashgt", "levdst", "levbassto", "levtopsto", "levfilsto"):
+                setattr(mp, k, self._p(_f64(getattr(m, k))))
+        if o.ldamout:
+            mp.i1dam = self._p(_i32(m.i1dam))
+            mp.damseq = self._p(_i32(np.where(m.damseq >= 0, m.damseq + 1, -9999)))
+            mp.damstat = self._p(_i32(m.damstat))
+            for k in ("dam_convol", "dam_emevol", "dam_adjvol", "dam_qa", "dam_qn", "dam_qf"):
+                setattr(mp, k, self._p(_f64(getattr(m, k))))
+        self._check(self.lib.cmf_hip_set_map(self.h, C.byref(mp)))
+        self._keep.clear()
+
+    # ------------------------------------------------------------------
+    def set_state(self, st: dict) -> None:
+        """Host -> device copy of the given arrays (names of cmf_state)."""
+        s = cmf_state()
+        keep = []
+        for k, v in st.items():
+            if k in STATE_FIELDS and v is not None:
+                a = _f64(v)
+                keep.append(a)
+                setattr(s, k, a.ctypes.data)
+        self._check(self.lib.cmf_hip_set_state(self.h, C.byref(s)))
+
+    def get_state(self, names: Optional[Iterable[str]] = None) -> dict:
+        m, o = self.map, self.opts
+        names = list(names) if names is not None else [
+            k for k in STATE_FIELDS if self._available(k)]
+        out, s = {}, cmf_state()
+        for k in names:
+            if not self._available(k):
+                continue
+            if k in ("pthflw", "pthflw_pre", "pthflw_avg"):
+                a = np.zeros((m.npthlev, m.npth))
+            elif k in ("pthflwsum", "pthflwsum_avg"):
+                a = np.zeros(m.npth)
+            else:
+                a = np.zeros(m.n)
+            out[k] = a
+            setattr(s, k, a.ctypes.data)
+        self._check(self.lib.cmf_hip_get_state(self.h, C.byref(s)))
+        return out
+
+    def _available(self, k: str) -> bool:
+        o = self.opts
+        if k in ("levsto", "levdph"):
+            return o.llevee
+        if k in ("damsto", "daminf", "daminf_avg"):
+            return o.ldamout
+        if k.startswith("pthflw"):
+            return o.lpthout
+        return True
+
+    def set_runoff(self, runoff: np.ndarray) -> None:
+        a = _f64(runoff)
+        self._check(self.lib.cmf_hip_set_runoff(self.h, a.ctypes.data))
+
+    # ------------------------------------------------------------------
+    def physics_fldstg(self) -> None:
+        self._check(self.lib.cmf_hip_physics_fldstg(self.h))
+
+    def physics_advance(self, dt: float, nt_cap: int = 0, budget: bool = False):
+        """One outer step of length dt [s]. Returns NT, or (NT, budget array
+        [NT, 11]) when budget=True."""
+        d = C.c_double(dt)
+        nt = C.c_int32(0)
+        if budget:
+            cap = 4096
+            g = np.zeros((cap, NGLB))
+            self._check(self.lib.cmf_hip_physics_advance_capped(
+                self.h, C.byref(d), nt_cap, C.byref(nt), g.ctypes.data, cap))
+            nrun = min(nt.value if nt_cap <= 0 else min(nt.value, nt_cap), cap)
+            return nt.value, g[:nrun].copy()
+        self._check(self.lib.cmf_hip_physics_advance_capped(
+            self.h, C.byref(d), nt_cap, C.byref(nt), None, 0))
+        return nt.value
+
+    def synchronize(self) -> None:
+        self._check(self.lib.cmf_hip_synchronize(self.h))
+
+    def kernel_times(self, enable: bool) -> np.ndarray:
+        """Accumulated kernel milliseconds since the last call: [outflow, rate,
+        update, stage, substeps]; then enable/disable event timing."""
+        out = np.zeros(5)
+        self._check(self.lib.cmf_hip_kernel_times(self.h, int(enable), out.ctypes.data))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.cmf_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,330 @@
+// cmf_device.hpp -- device-side data layout and per-cell physics of the MI355X
+// CaMa-Flood substep. Shared by the kernels (cmf_kernels.hip) and the host glue.
+//
+// Layout in HBM (one structure of arrays per handle, NSEQ order):
+//  * static map: the seven float32 source maps (every reference map is read from
+//    a float32 file, so float32 holds them exactly) and the NLFP float32 profile
+//    planes; the derived tables (RIVSTOMAX, RIVELV, FLDSTOMAX, FLDGRD) are
+//    recomputed in FP64 in registers with the reference's own statements, so the
+//    FP64 path reads 4 B instead of 8-16 B per map value and stays bit-identical.
+//  * topology: I1NEXT (0-based) and the upstream CSR (ascending sources).
+//  * state: storages FP64; flux/stage arrays of type R (double for the FP64
+//    path, float for the mixed path); inflow accumulators FP64.
+//
+// Arithmetic of the FP64 path keeps the reference's statement order (the Fortran
+// left-to-right product order, x**(-1.) = 1/x, x**0.5 = sqrt(x), default-kind
+// literals widened from float). Files are compiled with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cmf {
+
+// default-kind (float) literals of the reference, widened
+constexpr double K_1EM5F  = (double)1.e-5f;
+constexpr double K_1EM10F = (double)1.e-10f;
+constexpr double K_0P01F  = (double)0.01f;
+constexpr double K_2O3F   = (double)(2.f / 3.f);
+constexpr double K_0P1F   = (double)0.1f;
+constexpr double K_1E18F  = (double)1.e18f;
+constexpr double K_P7O3   = -7.0 / 3.0;     // -7._JPRB/3._JPRB
+constexpr double K_P4O3   = -4.0 / 3.0;     // -4._JPRB/3._JPRB
+constexpr double K_LEVSENT = 1.e18;         // 1.E18_JPRB sentinel of non-levee cells
+
+constexpr int NGLB = 11;
+enum { G_STOPRE, G_STONXT, G_STONEW, G_RIVINF, G_RIVOUT,
+       G_STOPRE2, G_STONEW2, G_RIVSTO, G_FLDSTO, G_FLDARE, G_LEVSTO };
+
+constexpr int BLOCK = 256;
+constexpr int NLFP_MAX = 16;
+
+template <class T> __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
+template <class T> __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
+__device__ __forceinline__ double vpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float vpow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float vsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double vabs(double x) { return fabs(x); }
+__device__ __forceinline__ float vabs(float x) { return fabsf(x); }
+
+struct DevMap {
+    int32_t n, nseqriv, nlfp, npth, npthlev, ndam;
+    int32_t lfplain, nlevcell;
+    double dfrcinc;
+    const int32_t* next;                       // 0-based downstream, <0 for mouths
+    const int32_t* up_ptr;                     // upstream CSR (n+1)
+    const int32_t* up_idx;
+    const float *elevtn, *rivhgt, *nxtdst, *rivwth, *rivman, *rivlen, *grarea;
+    const float* fldhgt;                       // [nlfp][n]
+    const double* dwnelv;                      // boundary elevation (mouths)
+    const int32_t* i2mask;
+    // bifurcation paths, 0-based endpoints (-1 outside domain)
+    const int32_t *pth_upst, *pth_down;
+    const double *pth_dst, *pth_elv, *pth_wth, *pth_man;   // elv/wth [npthlev][npth]
+    const int32_t *pout_ptr, *pout_idx, *pinf_ptr, *pinf_idx;
+    // levee: compact table of levee cells, lev_idx[i] = row or -1
+    const int32_t* lev_idx;
+    const double* lev_tab;                     // [7][nlevcell]
+    // dams
+    const int32_t* i1dam;                      // per cell (0,1,10,11)
+    const int32_t* dam_of_cell;                // per cell: dam row or -1
+    const int32_t* damstat;
+    const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
+};
+
+template <class R>
+struct DevState {
+    double *rivsto, *fldsto, *levsto, *damsto, *daminf;
+    R *rivout, *fldout;                        // limited outflow == *_PRE after a substep
+    const R* rivout_vel;                       // D2RIVOUT read by OUTFLW for RIVVEL
+    R *qr_u, *qf_u;                            // outflow before the inflow limiter
+    R *rivdph_pre, *fldsto_pre;
+    R *rivdph, *flddph, *fldfrc, *fldare, *sfcelv, *storge, *outflw, *rivvel, *levdph;
+    double *rivinf, *fldinf, *pthout;
+    R* rate;
+    R* runoff;
+    R *pthflw, *pthflw_u, *pthflwsum, *pthflwsum_u;   // [npthlev][npth], [npth]
+    R *rivout_avg, *fldout_avg, *outflw_avg, *rivvel_avg, *pthout_avg, *runoff_avg;
+    R *outflw_max, *rivdph_max, *storge_max, *daminf_avg;
+    R *pthflw_avg, *pthflwsum_avg;
+    double* partials;                          // [nblocks][NGLB]
+    double* cfl_part;                          // [nblocks]
+    double* glb_hist;                          // [nt_cap][NGLB]
+    double* ctl;                               // [4]: dt_min
+};
+
+struct StepArgs {
+    double dt, gdt;                            // DT, PGRV*DT
+    double pgrv, pmanfld, pdstmth, pcadp, pminslp;
+    int32_t lfldout, lpthout, ldamout, llevee;
+    int32_t first, last;                       // first / last substep of the outer step
+    double nadd;                               // sum of DT over the outer step (last)
+    int32_t it;                                // substep index (budget record)
+};
+
+// ---------------------------------------------------------------------------
+// Floodplain profile table recomputed on the fly: SET_FLDSTG
+// (src/cmf_ctrl_maps_mod.F90:984-995) and, identically, CMF_LEVEE_INIT :169-181.
+struct FldTab {
+    const float* h;          // &fldhgt[i], stride n
+    int n;
+    double rivlen, rivwth, dwthinc, stopre, hpre;
+    __device__ __forceinline__ void init(const DevMap& M, int i, double rivstomax) {
+        h = M.fldhgt + i; n = M.n;
+        rivlen = (double)M.rivlen[i];
+        rivwth = (double)M.rivwth[i];
+        dwthinc = ((double)M.grarea[i] * (1.0 / rivlen)) * M.dfrcinc;
+        stopre = rivstomax; hpre = 0.0;
+    }
+    // next layer l (1-based, consecutive calls): FLDSTOMAX(l), FLDGRD(l), FLDHGT(l)
+    __device__ __forceinline__ void layer(int l, double& smax, double& grd, double& hgt) {
+        const double hl = (double)h[(size_t)(l - 1) * n];
+        const double dstonow = (rivlen * (rivwth + dwthinc * ((double)l - 0.5))) * (hl - hpre);
+        smax = stopre + dstonow;
+        grd = (hl - hpre) * (1.0 / dwthinc);
+        hgt = hl;
+        stopre = smax; hpre = hl;
+    }
+};
+
+__device__ __forceinline__ double rivstomax_of(const DevMap& M, int i) {
+    // maps:716-721
+    return M.lfplain ? ((double)M.rivlen[i] * (double)M.rivwth[i]) * (double)M.rivhgt[i] : K_1E18F;
+}
+__device__ __forceinline__ double rivelv_of(const DevMap& M, int i) {
+    return (double)M.elevtn[i] - (double)M.rivhgt[i];            // maps:722
+}
+
+struct Stage {
+    double rivsto, fldsto, levsto;
+    double rivdph, flddph, fldfrc, fldare, levdph;
+};
+
+// CMF_CALC_FLDSTG_DEF per cell, src/cmf_calc_fldstg_mod.F90:49-95
+__device__ __forceinline__ void stage_def(const DevMap& M, int i, double rivsto, double fldsto, Stage& o) {
+    const double pstoall = rivsto + fldsto;
+    const double dstoall = pstoall;
+    const double rivstomax = rivstomax_of(M, i);
+    const double rivlen = (double)M.rivlen[i], rivwth = (double)M.rivwth[i];
+    o.levsto = 0.0; o.levdph = 0.0;
+    if (pstoall > rivstomax) {
+        const int nl = M.nlfp;
+        FldTab T; T.init(M, i, rivstomax);
+        const double dwth_inc = ((double)M.grarea[i] / rivlen) * M.dfrcinc;
+        int I = 1;
+        double dsto_fil = rivstomax, dwth_fil = rivwth, ddph_fil = 0.0;
+        double smax, grd, hgt;
+        T.layer(1, smax, grd, hgt);
+        while (pstoall > smax) {                      // :59-65
+            dsto_fil = smax;
+            dwth_fil = dwth_fil + dwth_inc;
+            ddph_fil = ddph_fil + grd * dwth_inc;
+            I = I + 1;
+            if (I > nl) break;
+            T.layer(I, smax, grd, hgt);
+        }
+        double dwth_add;
+        if (I > nl) {                                 // :66-70
+            const double dsto_add = dstoall - dsto_fil;
+            dwth_add = 0.0;
+            o.flddph = ddph_fil + (dsto_add / dwth_fil) / rivlen;
+        } else {                                      // :71-74
+            const double dsto_add = dstoall - dsto_fil;
+            dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rivlen) / grd);
+            o.flddph = ddph_fil + grd * dwth_add;
+        }
+        double rs = rivstomax + (rivlen * rivwth) * o.flddph;     // :76-77
+        rs = vmin(rs, pstoall);
+        o.rivsto = rs;
+        o.rivdph = (rs / rivlen) / rivwth;
+        const double fs = pstoall - rs;
+        o.fldsto = vmax(fs, 0.0);
+        double fr = ((-rivwth + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
+        fr = vmax(fr, 0.0);
+        fr = vmin(fr, 1.0);
+        o.fldfrc = fr;
+        o.fldare = (double)M.grarea[i] * fr;
+    } else {                                          // :87-95
+        o.rivsto = pstoall;
+        const double d = (dstoall / rivlen) / rivwth;
+        o.rivdph = vmax(d, 0.0);
+        o.fldsto = 0.0; o.flddph = 0.0; o.fldfrc = 0.0; o.fldare = 0.0;
+    }
+}
+
+// CMF_LEVEE_FLDSTG per cell, src/cmf_ctrl_levee_mod.F90:302-473
+__device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivsto, double fldsto,
+                                            double levsto, Stage& o) {
+    const int nl = M.nlfp;
+    const int li = M.lev_idx[i];
+    const int nlc = M.nlevcell;
+    double levhgt = 0.0, levfrc = 1.0, bashgt = K_LEVSENT, levdst = K_LEVSENT;
+    double levbassto = K_LEVSENT, levtopsto = K_LEVSENT, levfilsto = K_LEVSENT;
+    if (li >= 0) {
+        levhgt = M.lev_tab[li]; levfrc = M.lev_tab[nlc + li]; bashgt = M.lev_tab[2 * nlc + li];
+        levdst = M.lev_tab[3 * nlc + li]; levbassto = M.lev_tab[4 * nlc + li];
+        levtopsto = M.lev_tab[5 * nlc + li]; levfilsto = M.lev_tab[6 * nlc + li];
+    }
+    const double pstoall = (rivsto + fldsto) + levsto;
+    const double dstoall = pstoall;
+    const double rivstomax = rivstomax_of(M, i);
+    const double rl = (double)M.rivlen[i], rw = (double)M.rivwth[i], ga = (double)M.grarea[i];
+    const double dwth_inc = (ga / rl) * M.dfrcinc;
+    if (dstoall > rivstomax) {
+        if (dstoall < levbassto || dstoall >= levtopsto) {
+            // case 1 (:312-349) and case 4 (:424-461) share the layer search
+            const bool case1 = dstoall < levbassto;
+            FldTab T; T.init(M, i, rivstomax);
+            int I = 1;
+            double dsto_fil = rivstomax, dwth_fil = rw, ddph_fil = 0.0, dwth_add;
+            double smax, grd, hgt;
+            T.layer(1, smax, grd, hgt);
+            if (case1 || dstoall >= levfilsto) {
+                while (dstoall > smax) {
+                    dsto_fil = smax;
+                    dwth_fil = dwth_fil + dwth_inc;
+                    ddph_fil = ddph_fil + grd * dwth_inc;
+                    I = I + 1;
+                    if (I > nl) break;
+                    T.layer(I, smax, grd, hgt);
+                }
+                if (I <= nl) {
+                    const double dsto_add = dstoall - dsto_fil;
+                    dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rl) / grd);
+                    o.flddph = ddph_fil + grd * dwth_add;
+                } else {
+                    const double dsto_add = dstoall - dsto_fil;
+                    dwth_add = 0.0;
+                    o.flddph = ddph_fil + (dsto_add / dwth_fil) / rl;
+                }
+                if (case1) {
+                    o.rivsto = rivstomax + (rl * rw) * o.flddph;
+                    o.rivdph = (o.rivsto / rl) / rw;
+                    const double fs = pstoall - o.rivsto;
+                    o.fldsto = vmax(fs, 0.0);
+                    double fr = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
+                    fr = vmax(fr, 0.0);
+                    fr = vmin(fr, 1.0);
+                    o.fldfrc = fr;
+                    o.fldare = ga * fr;
+                    o.levsto = 0.0; o.levdph = 0.0;
+                } else {
+                    o.fldfrc = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
+                    o.fldare = ga * o.fldfrc;
+                    o.rivsto = rivstomax + (rl * rw) * o.flddph;
+                    o.rivdph = (o.rivsto / rl) / rw;
+                    const double dsto_add = ((o.flddph - levhgt) * (levdst + rw)) * rl;
+                    const double fs = (levtopsto + dsto_add) - o.rivsto;
+                    o.fldsto = vmax(fs, 0.0);
+                    const double ls = (pstoall - o.rivsto) - o.fldsto;
+                    o.levsto = vmax(ls, 0.0);
+                    o.levdph = o.flddph;
+                }
+                return;
+            }
+            // levtopsto <= dstoall < levfilsto: case 3 below
+        }
+        if (dstoall < levtopsto) {                    // case 2 (:353-368)
+            const double dsto_add = dstoall - levbassto;
+            const double dwth_add = levdst + rw;
+            o.flddph = bashgt + (dsto_add / dwth_add) / rl;
+            o.rivsto = rivstomax + (rl * rw) * o.flddph;
+            o.rivdph = (o.rivsto / rl) / rw;
+            const double fs = pstoall - o.rivsto;
+            o.fldsto = vmax(fs, 0.0);
+            o.fldfrc = levfrc;
+            o.fldare = ga * o.fldfrc;
+            o.levsto = 0.0; o.levdph = 0.0;
+            return;
+        }
+        // case 3 (:372-420): river side full, protected side filling
+        o.flddph = levhgt;
+        o.rivsto = rivstomax + (rl * rw) * o.flddph;
+        o.rivdph = (o.rivsto / rl) / rw;
+        const double fs = levtopsto - o.rivsto;
+        o.fldsto = vmax(fs, 0.0);
+        const double ls = (pstoall - o.rivsto) - o.fldsto;
+        o.levsto = vmax(ls, 0.0);
+        const int ilev = (int)(levfrc * (double)nl) + 1;
+        double dsto_fil = levtopsto, dwth_fil = 0.0, ddph_fil = 0.0;
+        FldTab T; T.init(M, i, rivstomax);
+        double smax = 0.0, grd = 0.0, hgt = 0.0;
+        for (int l = 1; l < ilev && l <= nl; ++l) T.layer(l, smax, grd, hgt);
+        int I = ilev;
+        while (I <= nl) {
+            T.layer(I, smax, grd, hgt);
+            const double dsto_add = ((levdst + rw) * (levhgt - hgt)) * rl;
+            if (dstoall < smax + dsto_add) break;
+            dsto_fil = smax + dsto_add;
+            dwth_fil = dwth_inc * (double)I - levdst;
+            ddph_fil = hgt - bashgt;
+            I = I + 1;
+        }
+        if (I <= nl) {
+            const double dsto_add = dstoall - dsto_fil;
+            const double dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rl) / grd);
+            const double ddph_add = dwth_add * grd;
+            o.levdph = (bashgt + ddph_fil) + ddph_add;
+            double fr = (dwth_fil + levdst) / (dwth_inc * (double)nl);
+            fr = vmax(fr, 0.0);
+            fr = vmin(fr, 1.0);
+            o.fldfrc = fr;
+            o.fldare = ga * fr;
+        } else {
+            const double dsto_add = dstoall - dsto_fil;
+            const double ddph_add = (dsto_add / dwth_fil) / rl;
+            o.levdph = (bashgt + ddph_fil) + ddph_add;
+            o.fldfrc = 1.0;
+            o.fldare = ga * o.fldfrc;
+        }
+        return;
+    }
+    // case 0 (:463-473)
+    o.rivsto = pstoall;
+    const double d = (dstoall / rl) / rw;
+    o.rivdph = vmax(d, 0.0);
+    o.fldsto = 0.0; o.flddph = 0.0; o.fldfrc = 0.0; o.fldare = 0.0;
+    o.levsto = 0.0; o.levdph = 0.0;
+}
+
+}  // namespace cmf

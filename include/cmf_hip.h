@@ -1,0 +1,152 @@
+/*
+ * cmf_hip.h -- C-ABI of the MI355X-native CaMa-Flood per-substep physics.
+ *
+ * Drop-in boundary: these entry points replace the two public subroutines of the
+ * reference module CMF_CTRL_PHYSICS_MOD (src/cmf_ctrl_physics_mod.F90):
+ *   CMF_PHYSICS_ADVANCE  (:21-155)   -> cmf_hip_physics_advance
+ *   CMF_PHYSICS_FLDSTG   (:288-305)  -> cmf_hip_physics_fldstg
+ * plus the data movement a device-resident implementation needs around them
+ * (maps once, prognostic state when the host changes it, forcing per input step,
+ * diagnostics when the host consumes them). The Fortran binding is
+ * cama-flood_v4_amd/fortran/cmf_ctrl_physics_mod.F90.
+ *
+ * Conventions
+ *  - Arrays are the host's contiguous Fortran allocatables in NSEQ vector order.
+ *    Index arrays keep the reference's 1-based values (I1NEXT, PTH_UPST, ...).
+ *    Real arrays are JPRB = double (the reference's FP64 build) unless noted.
+ *    2-D tables keep Fortran column order: FLDSTOMAX(NSEQMAX,1,NLFP) is NLFP
+ *    planes of NSEQALL; PTH_ELV(NPTHOUT,NPTHLEV) is NPTHLEV planes of NPTHOUT.
+ *  - Every function returns 0 on success, a negative CMF_E* code on failure;
+ *    cmf_hip_last_error() returns the message of the calling thread's last failure.
+ *  - Calls on one handle must not overlap; one HIP stream per handle, default
+ *    priority. No pointer passed in is retained after the call returns.
+ */
+#ifndef CMF_HIP_H
+#define CMF_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMF_OK          0
+#define CMF_EINVAL     -1   /* bad argument / inconsistent sizes */
+#define CMF_EHIP       -2   /* HIP runtime error */
+#define CMF_ESTATE     -3   /* call out of order (e.g. advance before set_map) */
+#define CMF_EMAP       -4   /* map tables fail validation */
+#define CMF_ENOMEM     -5
+
+/* Device arithmetic of the flux/stage variables (JPRB on the device). Storages,
+ * the storage update, the stage inversion and the budget sums are FP64 in both. */
+#define CMF_PREC_FP64   0   /* reference FP64 build semantics */
+#define CMF_PREC_MIXED  1   /* FP32 flux/stage arithmetic, FP64 storage */
+
+typedef struct cmf_hip_ctx* cmf_handle;
+
+/* Sizes: NSEQALL, NSEQRIV, NLFP, NPTHOUT, NPTHLEV, NDAM (src/yos_cmf_map.F90,
+ * src/cmf_ctrl_damout_mod.F90:41). npthout/npthlev/ndam may be 0 when unused. */
+typedef struct cmf_dims {
+    int32_t nseqall, nseqriv, nlfp, npthout, npthlev, ndam;
+} cmf_dims;
+
+/* Namelist switches and parameters read by the physics
+ * (src/cmf_ctrl_nmlist_mod.F90:77-115, :227-232). LSPAMAT is implied (gather
+ * inflow); LKINE, LSLPMIX, LSTG_ES, LGDWDLY, LROSPLIT, LWEVAP, LUPSINF, LOUTINS
+ * must be false. */
+typedef struct cmf_options {
+    int32_t ladpstp, lfplain, lfldout, lpthout, ldamout, llevee;
+    int32_t precision;                 /* CMF_PREC_* */
+    double pgrv, pmanfld, pdstmth, pcadp, pminslp;
+} cmf_options;
+
+/* Static map (YOS_CMF_MAP, levee and dam module tables). Required: i1next,
+ * grarea, elevtn, nxtdst, rivlen, rivwth, rivhgt, rivman, rivstomax, rivelv,
+ * dwnelv, fldhgt, fldstomax, fldgrd, i2mask. Paths when lpthout, levee tables
+ * when llevee, dam tables when ldamout. */
+typedef struct cmf_map {
+    const int32_t* i1next;                           /* I1NEXT, maps:420 */
+    const double *grarea, *elevtn, *nxtdst, *rivlen, *rivwth, *rivhgt, *rivman;
+    const double *rivstomax, *rivelv, *dwnelv;      /* maps:716-736 */
+    const double *fldhgt, *fldstomax, *fldgrd;       /* NLFP planes */
+    double dfrcinc;                                  /* DFRCINC */
+    const int32_t* i2mask;
+    /* bifurcation (maps:531-639) */
+    const int32_t *pth_upst, *pth_down;
+    const double *pth_dst, *pth_elv, *pth_wth, *pth_man;
+    /* levee (levee:93-264) */
+    const double *levhgt, *levfrc, *bashgt, *levdst, *levbassto, *levtopsto, *levfilsto;
+    /* dams (damout:123-308): per dam, 1-based DamSeq, DamStat */
+    const int32_t *i1dam, *damseq, *damstat;
+    const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
+} cmf_map;
+
+/* Prognostic and diagnostic arrays (YOS_CMF_PROG, YOS_CMF_DIAG). Any pointer may
+ * be NULL: that array is skipped by set/get. Storages (P2*) are FP64. */
+typedef struct cmf_state {
+    double *rivsto, *fldsto, *levsto, *damsto, *daminf;          /* P2* */
+    double *rivout, *fldout, *rivout_pre, *fldout_pre;           /* D2*OUT(_PRE) */
+    double *rivdph_pre, *fldsto_pre;
+    double *pthflw, *pthflw_pre;                                 /* NPTHLEV planes */
+    double *rivdph, *flddph, *fldfrc, *fldare, *sfcelv, *storge; /* D2 diag */
+    double *outflw, *rivvel, *rivinf, *fldinf, *pthout, *levdph;
+    double *pthflwsum;
+    /* substep averages/maxima of the last advance (CMF_DIAG_*_ADPSTP) */
+    double *rivout_avg, *fldout_avg, *outflw_avg, *rivvel_avg, *pthout_avg;
+    double *runoff_avg, *outflw_max, *rivdph_max, *storge_max, *daminf_avg;
+    double *pthflw_avg, *pthflwsum_avg;
+} cmf_state;
+
+/* Global water-budget sums of one substep, the P0GLB* of YOS_CMF_DIAG
+ * (src/yos_cmf_diag.F90:109-125) in this order. */
+#define CMF_NGLB 11
+enum { CMF_GLB_STOPRE, CMF_GLB_STONXT, CMF_GLB_STONEW, CMF_GLB_RIVINF, CMF_GLB_RIVOUT,
+       CMF_GLB_STOPRE2, CMF_GLB_STONEW2, CMF_GLB_RIVSTO, CMF_GLB_FLDSTO, CMF_GLB_FLDARE,
+       CMF_GLB_LEVSTO };
+
+/* Create a handle on the current HIP device. No reference counterpart (the
+ * reference keeps this state in module globals). */
+int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* out);
+void cmf_hip_destroy(cmf_handle h);
+const char* cmf_hip_last_error(void);
+
+/* Upload and validate the static map; builds the upstream CSR from I1NEXT in
+ * ascending source order (the I1UPST fill order, maps:518-522) and the per-cell
+ * path lists (I1P_OUT/I1P_INF, maps:607-636). Replaces the reads of the map
+ * module globals by CMF_PHYSICS_*. */
+int cmf_hip_set_map(cmf_handle h, const cmf_map* map);
+
+/* Host -> device copy of the non-NULL arrays of st (after CMF_PROG_INIT,
+ * CMF_RESTART_INIT, CMF_DAMOUT_INIT, or any host-side change of state). */
+int cmf_hip_set_state(cmf_handle h, const cmf_state* st);
+/* Device -> host copy into the non-NULL arrays of st. */
+int cmf_hip_get_state(cmf_handle h, cmf_state* st);
+
+/* Runoff forcing D2RUNOFF [m3/s] (written by CMF_FORCING_PUT). */
+int cmf_hip_set_runoff(cmf_handle h, const double* runoff);
+
+/* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
+int cmf_hip_physics_fldstg(cmf_handle h);
+
+/* CMF_PHYSICS_ADVANCE for one outer step of length *dt (DT). On return *dt is
+ * restored (physics:145), *nt_out = NT. If glb != NULL it receives the budget sums
+ * of up to glb_cap substeps (CMF_NGLB values each, substep-major). */
+int cmf_hip_physics_advance(cmf_handle h, double* dt, int32_t* nt_out,
+                            double* glb, int32_t glb_cap);
+
+/* Timing support: run the outer step with at most nt_cap substeps (0 = all).
+ * Same as cmf_hip_physics_advance otherwise. No reference counterpart. */
+int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap,
+                                   int32_t* nt_out, double* glb, int32_t glb_cap);
+
+/* Wait for all work of the handle (no reference counterpart). */
+int cmf_hip_synchronize(cmf_handle h);
+
+/* Duration in milliseconds of the dominant substep kernels over the last
+ * advance, measured with HIP events on the handle's stream (diagnostics for
+ * benchmarking; no reference counterpart). out[0]=outflow, [1]=rate,
+ * [2]=update(stage), [3]=stage+CFL, [4]=number of substeps timed. */
+int cmf_hip_kernel_times(cmf_handle h, int32_t enable, double* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
