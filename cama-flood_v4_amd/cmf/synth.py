@@ -172,10 +172,10 @@ def add_bifurcations(net: SynthNetwork, npaths: int, nlev: int, seed: int,
         k = int(land_idx[rng.integers(len(land_idx))])
         iy, ix = divmod(k, nx)
         d = int(rng.integers(8))
-        jy = iy + DY[d]
+        jy = int(iy + DY[d])
         if jy < 0 or jy >= ny:
             continue
-        jx = (ix + DX[d]) % nx
+        jx = int((ix + DX[d]) % nx)
         if not net.land[jy, jx]:
             continue
         if (net.nextx[iy, ix] == jx + 1 and net.nexty[iy, ix] == jy + 1) or \
