@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--compare-order", action="store_true",
+                    help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding is not "
@@ -166,6 +168,15 @@ def main():
     value = n * sum_nt / t
     dom, per, roof = roofline(kt, n, BYTES_FP64)
     log(f"FP64: {value:.4g} cell-substeps/s, sum NT {sum_nt}, {t:.3f}s; kernels ms {per}")
+    order_cmp = None
+    if a.compare_order:
+        from cmf import maps
+        os.environ["CMF_HIP_ORDER"] = "nseq"
+        m3 = maps.build_map(net, maps.Options(llevee=True, ldamout=True))
+        s3, t3, kt3 = run_gpu(net, m3, rvec, "fp64", a.steps, a.warmup, dt)
+        del os.environ["CMF_HIP_ORDER"]
+        order_cmp = dict(value=n * s3 / t3, kernels_ms=roofline(kt3, n, BYTES_FP64)[1])
+        log(f"FP64 NSEQ order: {order_cmp['value']:.4g} cell-substeps/s; kernels ms {order_cmp['kernels_ms']}")
     mixed = None
     if not a.no_mixed:
         net2, m2, _ = net, None, None
@@ -191,6 +202,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "fp64_nseq_order": order_cmp,
     }
     print(json.dumps(out), flush=True)
 

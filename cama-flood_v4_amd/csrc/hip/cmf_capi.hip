@@ -7,8 +7,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 using namespace cmf;
@@ -48,6 +50,10 @@ struct cmf_hip_ctx {
     double* h_ctl = nullptr;          // pinned
     int nb_cells = 0;
     int hist_cap = 0;
+    // device cell order: perm[p] = NSEQ index of device position p (identity when
+    // CMF_HIP_ORDER=nseq); the C-ABI always speaks NSEQ order
+    std::vector<int32_t> perm, iperm;
+    bool permuted = false;
     // per-kernel timing (HIP events on the handle's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -63,8 +69,11 @@ int dalloc(cmf_hip_ctx* h, T** p, size_t n) {
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, n * sizeof(T));
     if (e != hipSuccess) return fail(CMF_ENOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
-    e = hipMemset(q, 0, n * sizeof(T));
-    if (e != hipSuccess) return fail(CMF_EHIP, "hipMemset: %s", hipGetErrorString(e));
+    // clear on the handle's own (non-blocking) stream and wait, so the clear is
+    // ordered before every later copy and kernel of this handle
+    e = hipMemsetAsync(q, 0, n * sizeof(T), h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(CMF_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
     h->allocs.push_back(q);
     *p = static_cast<T*>(q);
     return 0;
@@ -116,33 +125,49 @@ int alloc_state(cmf_hip_ctx* h, DevState<R>& S) {
 bool is_f32(double x) { return (double)(float)x == x; }
 
 // host copies with element-type conversion
+// Host <-> device copies with element-type conversion. cell=true arrays are
+// per-cell (NSEQ order on the host, device order on the device).
 template <class R>
-int put(cmf_hip_ctx* h, R* dst, const double* src, size_t n) {
+int put(cmf_hip_ctx* h, R* dst, const double* src, size_t n, bool cell = true) {
     if (!src || n == 0) return 0;
-    if (sizeof(R) == sizeof(double)) {
+    const bool perm = cell && h->permuted;
+    if (sizeof(R) == sizeof(double) && !perm) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    } else {
-        std::vector<R> tmp(n);
-        for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[k];
-        HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
     }
+    std::vector<R> tmp(n);
+    if (perm) for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[h->perm[k]];
+    else for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[k];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
 template <class R>
-int get(cmf_hip_ctx* h, double* dst, const R* src, size_t n) {
+int get(cmf_hip_ctx* h, double* dst, const R* src, size_t n, bool cell = true) {
     if (!dst || n == 0) return 0;
-    if (sizeof(R) == sizeof(double)) {
+    const bool perm = cell && h->permuted;
+    if (sizeof(R) == sizeof(double) && !perm) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
-    } else {
-        std::vector<R> tmp(n);
-        HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        for (size_t k = 0; k < n; ++k) dst[k] = (double)tmp[k];
+        return 0;
     }
+    std::vector<R> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (perm) for (size_t k = 0; k < n; ++k) dst[h->perm[k]] = (double)tmp[k];
+    else for (size_t k = 0; k < n; ++k) dst[k] = (double)tmp[k];
     return 0;
+}
+
+// per-cell host array in device order (planes of n cells)
+template <class T>
+std::vector<T> to_dev(const cmf_hip_ctx* h, const T* src, size_t n, size_t planes = 1) {
+    std::vector<T> out(n * planes);
+    for (size_t l = 0; l < planes; ++l)
+        for (size_t p = 0; p < n; ++p) out[l * n + p] = src[l * n + (h->permuted ? (size_t)h->perm[p] : p)];
+    return out;
 }
 
 template <class T>
@@ -156,19 +181,46 @@ int put_raw(cmf_hip_ctx* h, T** dst, const T* src, size_t n) {
 }
 
 // float32 copy of a map array; every reference map is read from a float32 file
-int put_f32(cmf_hip_ctx* h, const float** dst, const double* src, size_t n, const char* name) {
+int put_f32(cmf_hip_ctx* h, const float** dst, const double* src, size_t n, const char* name,
+            size_t planes = 1) {
     if (!src) return fail(CMF_EINVAL, "map array %s is required", name);
-    std::vector<float> tmp(n);
-    for (size_t k = 0; k < n; ++k) {
+    for (size_t k = 0; k < n * planes; ++k)
         if (!is_f32(src[k]))
             return fail(CMF_EMAP, "map array %s[%zu]=%.17g is not a float32 value (maps are float32 files)",
                         name, k, src[k]);
-        tmp[k] = (float)src[k];
-    }
+    std::vector<double> d = to_dev(h, src, n, planes);
+    std::vector<float> tmp(n * planes);
+    for (size_t k = 0; k < n * planes; ++k) tmp[k] = (float)d[k];
     float* p = nullptr;
-    TRY(put_raw(h, &p, tmp.data(), n));
+    TRY(put_raw(h, &p, tmp.data(), n * planes));
     *dst = p;
     return 0;
+}
+
+// Device cell order: depth-first post-order of the river tree from each mouth,
+// upstream branches in ascending NSEQ order. A cell's upstream subtree precedes
+// it and a chain of cells is contiguous, so the downstream and upstream gathers of
+// a wavefront mostly touch the same or adjacent cache lines.
+bool dfs_order(int n, int nr, const std::vector<int32_t>& up_ptr, const std::vector<int32_t>& up_idx,
+               std::vector<int32_t>& perm) {
+    perm.assign((size_t)n, -1);
+    std::vector<std::pair<int32_t, int32_t>> stack;
+    int pos = 0;
+    for (int root = nr; root < n; ++root) {
+        stack.push_back({root, up_ptr[root]});
+        while (!stack.empty()) {
+            auto& top = stack.back();
+            if (top.second < up_ptr[(size_t)top.first + 1]) {
+                const int32_t c = up_idx[(size_t)top.second++];
+                stack.push_back({c, up_ptr[c]});
+            } else {
+                if (pos >= n) return false;
+                perm[(size_t)pos++] = top.first;
+                stack.pop_back();
+            }
+        }
+    }
+    return pos == n;
 }
 
 // Recompute RIVSTOMAX, RIVELV (maps:716-722) and SET_FLDSTG (maps:969-999) and check
@@ -219,12 +271,7 @@ int set_state_t(cmf_hip_ctx* h, const cmf_state* st) {
     const size_t n = (size_t)h->d.nseqall;
     const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
     const size_t npl = np * (h->o.lpthout ? (size_t)h->d.npthlev : 0);
-    auto putd = [&](double* dst, const double* src, size_t m) -> int {
-        if (!src) return 0;
-        HIPCHK(hipMemcpyAsync(dst, src, m * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        return 0;
-    };
+    auto putd = [&](double* dst, const double* src, size_t m) -> int { return put<double>(h, dst, src, m); };
     TRY(putd(S.rivsto, st->rivsto, n));
     TRY(putd(S.fldsto, st->fldsto, n));
     if (h->o.llevee) TRY(putd(S.levsto, st->levsto, n));
@@ -243,8 +290,8 @@ int set_state_t(cmf_hip_ctx* h, const cmf_state* st) {
     TRY(put<R>(h, S.fldsto_pre, st->fldsto_pre, n));
     if (h->o.lpthout) {
         const double* ppre = st->pthflw_pre ? st->pthflw_pre : st->pthflw;
-        TRY(put<R>(h, S.pthflw, ppre, npl));
-        TRY(put<R>(h, S.pthflwsum, st->pthflwsum, np));
+        TRY(put<R>(h, S.pthflw, ppre, npl, false));
+        TRY(put<R>(h, S.pthflwsum, st->pthflwsum, np, false));
     }
     TRY(put<R>(h, S.rivdph, st->rivdph, n));
     TRY(put<R>(h, S.flddph, st->flddph, n));
@@ -267,12 +314,7 @@ int get_state_t(cmf_hip_ctx* h, cmf_state* st) {
     const size_t n = (size_t)h->d.nseqall;
     const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
     const size_t npl = np * (h->o.lpthout ? (size_t)h->d.npthlev : 0);
-    auto getd = [&](double* dst, const double* src, size_t m) -> int {
-        if (!dst) return 0;
-        HIPCHK(hipMemcpyAsync(dst, src, m * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        return 0;
-    };
+    auto getd = [&](double* dst, const double* src, size_t m) -> int { return get<double>(h, dst, src, m); };
     HIPCHK(hipStreamSynchronize(h->stream));
     TRY(getd(st->rivsto, S.rivsto, n));
     TRY(getd(st->fldsto, S.fldsto, n));
@@ -285,11 +327,11 @@ int get_state_t(cmf_hip_ctx* h, cmf_state* st) {
     TRY(get<R>(h, st->rivdph_pre, S.rivdph_pre, n));
     TRY(get<R>(h, st->fldsto_pre, S.fldsto_pre, n));
     if (h->o.lpthout) {
-        TRY(get<R>(h, st->pthflw, S.pthflw, npl));
-        TRY(get<R>(h, st->pthflw_pre, S.pthflw, npl));
-        TRY(get<R>(h, st->pthflwsum, S.pthflwsum, np));
-        TRY(get<R>(h, st->pthflw_avg, S.pthflw_avg, npl));
-        TRY(get<R>(h, st->pthflwsum_avg, S.pthflwsum_avg, np));
+        TRY(get<R>(h, st->pthflw, S.pthflw, npl, false));
+        TRY(get<R>(h, st->pthflw_pre, S.pthflw, npl, false));
+        TRY(get<R>(h, st->pthflwsum, S.pthflwsum, np, false));
+        TRY(get<R>(h, st->pthflw_avg, S.pthflw_avg, npl, false));
+        TRY(get<R>(h, st->pthflwsum_avg, S.pthflwsum_avg, np, false));
     }
     TRY(get<R>(h, st->rivdph, S.rivdph, n));
     TRY(get<R>(h, st->flddph, S.flddph, n));
@@ -368,11 +410,13 @@ int advance_t(cmf_hip_ctx* h, double* dt, int32_t nt_cap, int32_t* nt_out, doubl
     A.dt = dts;
     A.gdt = A.pgrv * dts;
     A.nadd = nadd;
-    DevState<R> Sk = S;
+    A.nrun = nrun;
     for (int32_t it = 0; it < nrun; ++it) {
         A.first = (it == 0);
         A.last = (it == nrun - 1);
         A.it = it;
+        DevState<R> Sk = S;
+        Sk.rivdph_next = S.rivdph_pre;
         Sk.rivout_vel = (h->vel_pending && it == 0) ? (const R*)h->vel_buf : S.rivout;
         if (h->timing) TRY(ev_record(h, ne++));
         launch_outflow<R>(h->M, Sk, A, h->stream);
@@ -381,6 +425,7 @@ int advance_t(cmf_hip_ctx* h, double* dt, int32_t nt_cap, int32_t* nt_out, doubl
         if (h->timing) { TRY(ev_record(h, ne++)); TRY(ev_record(h, ne++)); }
         launch_update<R>(h->M, Sk, A, true, h->stream);
         if (h->timing) TRY(ev_record(h, ne++));
+        std::swap(S.rivdph, S.rivdph_pre);                            // CALC_VARS_PRE for RIVDPH
     }
     HIPCHK(hipGetLastError());
     h->vel_pending = false;
@@ -486,10 +531,31 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     std::vector<int32_t> owner(nr), item(nr), up_ptr, up_idx;
     for (int i = 0; i < nr; ++i) { owner[i] = nxt[i]; item[i] = i; }
     csr(n, owner, up_ptr, up_idx, item);
+    // device cell order
+    const char* ord = getenv("CMF_HIP_ORDER");
+    h->permuted = !(ord && std::string(ord) == "nseq");
+    if (h->permuted && !dfs_order(n, nr, up_ptr, up_idx, h->perm))
+        return fail(CMF_EMAP, "river network is not a forest draining to the mouth cells");
+    if (!h->permuted) { h->perm.resize((size_t)n); for (int i = 0; i < n; ++i) h->perm[i] = i; }
+    h->iperm.assign((size_t)n, 0);
+    for (int p = 0; p < n; ++p) h->iperm[(size_t)h->perm[p]] = p;
+    const std::vector<int32_t>& P = h->perm;
+    const std::vector<int32_t>& IP = h->iperm;
+    // topology in device positions; each upstream list keeps ascending NSEQ order
+    std::vector<int32_t> dnext((size_t)n), dptr((size_t)n + 1, 0), didx(up_idx.size());
+    for (int p = 0; p < n; ++p) {
+        const int32_t v = nxt[(size_t)P[p]];
+        dnext[(size_t)p] = v >= 0 ? IP[(size_t)v] : v;
+        dptr[(size_t)p + 1] = dptr[p] + (up_ptr[(size_t)P[p] + 1] - up_ptr[(size_t)P[p]]);
+    }
+    for (int p = 0; p < n; ++p) {
+        int32_t k = dptr[p];
+        for (int32_t q = up_ptr[(size_t)P[p]]; q < up_ptr[(size_t)P[p] + 1]; ++q) didx[(size_t)k++] = IP[(size_t)up_idx[q]];
+    }
     int32_t* p32 = nullptr;
-    TRY(put_raw(h, &p32, nxt.data(), (size_t)n)); M.next = p32;
-    TRY(put_raw(h, &p32, up_ptr.data(), up_ptr.size())); M.up_ptr = p32;
-    TRY(put_raw(h, &p32, up_idx.data(), up_idx.size())); M.up_idx = p32;
+    TRY(put_raw(h, &p32, dnext.data(), (size_t)n)); M.next = p32;
+    TRY(put_raw(h, &p32, dptr.data(), dptr.size())); M.up_ptr = p32;
+    TRY(put_raw(h, &p32, didx.data(), didx.size())); M.up_idx = p32;
     TRY(validate_tables(h, mp));
     TRY(put_f32(h, &M.grarea, mp->grarea, n, "GRAREA"));
     TRY(put_f32(h, &M.elevtn, mp->elevtn, n, "ELEVTN"));
@@ -498,10 +564,14 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     TRY(put_f32(h, &M.rivwth, mp->rivwth, n, "RIVWTH"));
     TRY(put_f32(h, &M.rivhgt, mp->rivhgt, n, "RIVHGT"));
     TRY(put_f32(h, &M.rivman, mp->rivman, n, "RIVMAN"));
-    TRY(put_f32(h, &M.fldhgt, mp->fldhgt, (size_t)nl * n, "FLDHGT"));
+    TRY(put_f32(h, &M.fldhgt, mp->fldhgt, (size_t)n, "FLDHGT", (size_t)nl));
     double* pd = nullptr;
-    TRY(put_raw(h, &pd, mp->dwnelv, (size_t)n)); M.dwnelv = pd;
-    TRY(put_raw(h, &p32, mp->i2mask, (size_t)n)); M.i2mask = p32;
+    {
+        std::vector<double> v = to_dev(h, mp->dwnelv, (size_t)n);
+        TRY(put_raw(h, &pd, v.data(), (size_t)n)); M.dwnelv = pd;
+        std::vector<int32_t> m2 = to_dev(h, mp->i2mask, (size_t)n);
+        TRY(put_raw(h, &p32, m2.data(), (size_t)n)); M.i2mask = p32;
+    }
     // bifurcation paths (maps:531-639)
     if (h->o.lpthout) {
         const int np = h->d.npthout, nlv = h->d.npthlev;
@@ -513,6 +583,8 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
             pu[p] = mp->pth_upst[p] > 0 ? mp->pth_upst[p] - 1 : -1;
             pdn[p] = mp->pth_down[p] > 0 ? mp->pth_down[p] - 1 : -1;
             if (pu[p] >= n || pdn[p] >= n) return fail(CMF_EMAP, "path %d endpoint outside NSEQALL", p + 1);
+            if (pu[p] >= 0) pu[p] = IP[(size_t)pu[p]];
+            if (pdn[p] >= 0) pdn[p] = IP[(size_t)pdn[p]];
             if (pu[p] >= 0 && pdn[p] >= 0) {
                 oo.push_back(pu[p]); oi.push_back(p);
                 io.push_back(pdn[p]); ii.push_back(p);
@@ -538,11 +610,12 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
         for (int k = 0; k < 7; ++k) if (!L[k]) return fail(CMF_EINVAL, "LLEVEE: levee tables missing");
         std::vector<int32_t> lidx(n, -1);
         std::vector<int32_t> cells;
-        for (int i = 0; i < n; ++i) {
+        for (int p = 0; p < n; ++p) {                                   // device order
+            const int i = P[(size_t)p];
             const bool plain = mp->levhgt[i] == 0.0 && mp->levfrc[i] == 1.0 && mp->bashgt[i] == K_LEVSENT &&
                                mp->levdst[i] == K_LEVSENT && mp->levbassto[i] == K_LEVSENT &&
                                mp->levtopsto[i] == K_LEVSENT && mp->levfilsto[i] == K_LEVSENT;
-            if (!plain) { lidx[i] = (int32_t)cells.size(); cells.push_back(i); }
+            if (!plain) { lidx[p] = (int32_t)cells.size(); cells.push_back(i); }
         }
         const int nc = (int)cells.size();
         std::vector<double> tab((size_t)7 * (nc ? nc : 1));
@@ -564,9 +637,10 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
             if (mp->damstat[d] == -9999) continue;
             const int s = mp->damseq[d] - 1;
             if (s < 0 || s >= n) return fail(CMF_EMAP, "DamSeq(%d)=%d outside NSEQALL", d + 1, mp->damseq[d]);
-            doc[s] = d;
+            doc[(size_t)IP[(size_t)s]] = d;
         }
-        TRY(put_raw(h, &p32, mp->i1dam, (size_t)n)); M.i1dam = p32;
+        std::vector<int32_t> i1d = to_dev(h, mp->i1dam, (size_t)n);
+        TRY(put_raw(h, &p32, i1d.data(), (size_t)n)); M.i1dam = p32;
         TRY(put_raw(h, &p32, doc.data(), (size_t)n)); M.dam_of_cell = p32;
         TRY(put_raw(h, &p32, mp->damstat, (size_t)nd)); M.damstat = p32;
         TRY(put_raw(h, &pd, mp->dam_convol, (size_t)nd)); M.dam_convol = pd;

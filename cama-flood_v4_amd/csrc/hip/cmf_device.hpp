@@ -79,6 +79,7 @@ struct DevState {
     const R* rivout_vel;                       // D2RIVOUT read by OUTFLW for RIVVEL
     R *qr_u, *qf_u;                            // outflow before the inflow limiter
     R *rivdph_pre, *fldsto_pre;
+    R* rivdph_next;                            // buffer receiving the new RIVDPH (k_update)
     R *rivdph, *flddph, *fldfrc, *fldare, *sfcelv, *storge, *outflw, *rivvel, *levdph;
     double *rivinf, *fldinf, *pthout;
     R* rate;
@@ -100,6 +101,7 @@ struct StepArgs {
     int32_t first, last;                       // first / last substep of the outer step
     double nadd;                               // sum of DT over the outer step (last)
     int32_t it;                                // substep index (budget record)
+    int32_t nrun;                              // substeps run in this outer step
 };
 
 // ---------------------------------------------------------------------------

@@ -49,10 +49,11 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     const R rivwth = R(M.rivwth[i]), rivman = R(M.rivman[i]);
     const R qr_pre = S.rivout[i];                                   // D2RIVOUT_PRE
     const R qr_old = S.rivout_vel[i];                               // D2RIVOUT (for RIVVEL)
-    const bool river = i < M.nseqriv;
+    const int nxt = M.next[i];
+    const bool river = nxt >= 0;                                     // mouths carry a negative code
     R dwn, dwn_pre, rdst;
     if (river) {                                                     // outflw:52-57
-        const int j = M.next[i];
+        const int j = nxt;
         const R rivelv_j = R(rivelv_of(M, j));
         dwn = rivelv_j + S.rivdph[j];
         dwn_pre = rivelv_j + S.rivdph_pre[j];
@@ -150,7 +151,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
             S.daminf[i] = inf;
             const double dt = A.dt;
             if (code == 10 && river) {                               // :471-493
-                const int j = M.next[i];
+                const int j = nxt;
                 double dslope = ((double)M.elevtn[i] - (double)M.elevtn[j]) * (1.0 / (double)M.nxtdst[i]);
                 dslope = vmax(dslope, A.pminslp);
                 const double rd = (double)rivdph;
@@ -198,7 +199,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     }
     S.qr_u[i] = q_r;
     S.qf_u[i] = q_f;
-    S.rivvel[i] = vel;
+    if (A.last) S.rivvel[i] = vel;                                  // output-only diagnostic
     // CMF_DIAG_AVEMAX_ADPSTP (diag:113) for RIVVEL, final after OUTFLW/DAMOUT
     R va = (A.first ? zero : S.rivvel_avg[i]) + vel * R(A.dt);
     if (A.last) va = va / R(A.nadd);
@@ -319,7 +320,8 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     // limiter, inflow_lspamat:456-473
     const R qr = S.qr_u[i], qf = S.qf_u[i];
     const R ri = S.rate[i];
-    const R r = (i < M.nseqriv && !(qr > zero)) ? S.rate[M.next[i]] : ri;
+    const int nxt = M.next[i];
+    const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[nxt] : ri;
     const R qrl = qr * r, qfl = qf * r;
     // upstream gather, inflow_lspamat:481-489
     double rivinf = 0.0, fldinf = 0.0;
@@ -338,7 +340,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
             pthout = pthout - (double)limited_path_sum<R>(M, S, M.pinf_idx[k]);
     }
     const R rivinf_r = R(rivinf), fldinf_r = R(fldinf), pthout_r = R(pthout);
-    S.rivinf[i] = rivinf; S.fldinf[i] = fldinf; S.pthout[i] = pthout;
+    if (A.last) { S.rivinf[i] = rivinf; S.fldinf[i] = fldinf; S.pthout[i] = pthout; }
     const R runoff = S.runoff[i];
     // CMF_DAMOUT_WATBAL, damout:521-536
     if (A.ldamout) {
@@ -349,12 +351,12 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
             S.damsto[i] = (S.damsto[i] + inflow * dt) - outflw * dt;
         }
     }
-    // CALC_VARS_PRE, physics:254-276 (RIVOUT_PRE/FLDOUT_PRE share the RIVOUT/FLDOUT arrays)
-    const R rivdph_old = S.rivdph[i];
+    // CALC_VARS_PRE, physics:254-276. RIVOUT_PRE/FLDOUT_PRE share the RIVOUT/FLDOUT
+    // arrays; RIVDPH/RIVDPH_PRE are a buffer pair whose roles the host swaps after
+    // this kernel, so the new depth goes to the buffer of the old RIVDPH_PRE.
     double rs = S.rivsto[i], fs = S.fldsto[i];
     S.rivout[i] = qrl;
     S.fldout[i] = qfl;
-    S.rivdph_pre[i] = rivdph_old;
     S.fldsto_pre[i] = R(fs);
     // CMF_CALC_STONXT, stonxt:54-98
     g[G_STOPRE] = rs + fs;
@@ -382,11 +384,14 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     S.rivsto[i] = o.rivsto;
     S.fldsto[i] = o.fldsto;
     const R rivdph = R(o.rivdph);
-    S.rivdph[i] = rivdph;
+    S.rivdph_next[i] = rivdph;
     S.flddph[i] = R(o.flddph);
     S.fldfrc[i] = R(o.fldfrc);
-    S.fldare[i] = R(o.fldare);
-    S.sfcelv[i] = R(rivelv_of(M, i)) + rivdph;
+    if (A.last) {                                                   // output-only diagnostics
+        S.fldare[i] = R(o.fldare);
+        S.sfcelv[i] = R(rivelv_of(M, i)) + rivdph;
+        S.outflw[i] = outflw;
+    }
     double stg = o.rivsto + o.fldsto;
     if (A.llevee) {
         S.levsto[i] = o.levsto;
@@ -395,7 +400,6 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     }
     const R storge = R(stg);
     S.storge[i] = storge;
-    S.outflw[i] = outflw;
     g[G_STOPRE2] = A.llevee ? (rs + fs) + levsto : rs + fs;
     g[G_STONEW2] = stg;
     g[G_RIVSTO] = o.rivsto;
@@ -408,14 +412,23 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     R a1 = (f0 ? zero : S.rivout_avg[i]) + qrl * rdt;
     R a2 = (f0 ? zero : S.fldout_avg[i]) + qfl * rdt;
     R a3 = (f0 ? zero : S.outflw_avg[i]) + outflw * rdt;
-    R a4 = (f0 ? zero : S.pthout_avg[i]) + pthout_r * rdt;
-    R a5 = (f0 ? zero : S.runoff_avg[i]) + runoff * rdt;
     if (A.last) {
         const R nadd = R(A.nadd);
-        a1 = a1 / nadd; a2 = a2 / nadd; a3 = a3 / nadd; a4 = a4 / nadd; a5 = a5 / nadd;
+        a1 = a1 / nadd; a2 = a2 / nadd; a3 = a3 / nadd;
+        // RUNOFF is constant over the outer step: its NT additions are replayed in
+        // registers, in the same order, instead of a read-modify-write per substep
+        R a5 = zero;
+        for (int k = 0; k < A.nrun; ++k) a5 = a5 + runoff * rdt;
+        S.runoff_avg[i] = a5 / nadd;
     }
     S.rivout_avg[i] = a1; S.fldout_avg[i] = a2; S.outflw_avg[i] = a3;
-    S.pthout_avg[i] = a4; S.runoff_avg[i] = a5;
+    if (A.lpthout) {
+        R a4 = (f0 ? zero : S.pthout_avg[i]) + pthout_r * rdt;
+        if (A.last) a4 = a4 / R(A.nadd);
+        S.pthout_avg[i] = a4;
+    } else if (f0) {
+        S.pthout_avg[i] = zero;                                     // PTHOUT = 0 without paths
+    }
     S.outflw_max[i] = vmax(f0 ? zero : S.outflw_max[i], vabs(outflw));
     S.rivdph_max[i] = vmax(f0 ? zero : S.rivdph_max[i], rivdph);
     S.storge_max[i] = vmax(f0 ? zero : S.storge_max[i], storge);
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(BLOCK) void k_stage(DevMap M, DevState<R> S, StepAr
         S.storge[i] = R(stg);
         if (with_cfl && M.i2mask[i] == 0) {                          // physics:178-197
             const double ddph = vmax((double)rivdph, 0.01);
-            const double ddst = (i < M.nseqriv) ? (double)M.nxtdst[i] : A.pdstmth;
+            const double ddst = (M.next[i] >= 0) ? (double)M.nxtdst[i] : A.pdstmth;
             c = (A.pcadp * ddst) * (1.0 / sqrt(A.pgrv * ddph));
         }
     }
