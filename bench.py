@@ -94,9 +94,10 @@ def run_gpu(net, m, rvec, precision, steps, warmup, dt):
     return sum_nt, t_total, kt
 
 
-def cpu_baseline(net, ro, n_cells, dt, sample_steps):
-    """Reference OpenMP FP64 build timed on this host over a bounded sample."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "omp64", "cmf_ref")
+def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
+    """Reference OpenMP build (omp64: FP64; omp32: the reference's default
+    single-precision mode) timed on this host over a bounded sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", variant, "cmf_ref")
     if not os.path.exists(exe):
         return None
     from cmf import synth
@@ -114,18 +115,28 @@ def cpu_baseline(net, ro, n_cells, dt, sample_steps):
                           f"EYEAR = {t1.year}\nEMON  = {t1.month}\nEDAY  = {t1.day}\n")
         nml = nml.replace("EHOUR = 0", f"EHOUR = {t1.hour}")
         open(os.path.join(d, "input_cmf.nam"), "w").write(nml)
-        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores",
+                   OMP_STACKSIZE="512M")
+        import resource
+
+        def big_stack():   # the reference keeps NX*NY work arrays on the stack (run scripts: ulimit -s unlimited)
+            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
         r = subprocess.run([exe, "runoff.bin", "NONE", "0"], cwd=d, env=env, capture_output=True,
-                           text=True, timeout=900)
+                           text=True, timeout=900, preexec_fn=big_stack)
         if r.returncode != 0:
-            log("reference run failed:", r.stderr[-500:])
+            log(f"reference run failed (rc={r.returncode}): {r.stdout[-800:]} {r.stderr[-800:]}")
+            try:
+                log(open(os.path.join(d, "log_CaMa.txt")).read()[-1500:])
+            except OSError:
+                pass
             return None
         line = [l for l in r.stdout.splitlines() if l.startswith("REFTIME")][0].split()
         t, nsteps, n = float(line[1]), int(line[2]), int(line[3])
         nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
         cs = n * sum(nts)
         return dict(value=cs / t, unit="cell-substeps/s", cores=ncores, kind="reference",
-                    sample=f"reference FP64 OpenMP (flang -O3 -fopenmp), {nsteps} outer steps of "
+                    sample=f"reference {'FP64' if variant == 'omp64' else 'single-precision'} OpenMP "
+                           f"(flang -O3 -fopenmp), {nsteps} outer steps of "
                            f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
 
 
@@ -151,7 +162,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
@@ -191,6 +202,13 @@ def main():
     if not a.no_cpu:
         log("CPU baseline (reference OpenMP)")
         cpu = cpu_baseline(net, ro, n, dt, a.cpu_steps)
+        if cpu:
+            log(f"CPU reference FP64: {cpu['value']:.4g} cell-substeps/s on {cpu['cores']} cores")
+            if mixed is not None:
+                c32 = cpu_baseline(net, ro, n, dt, a.cpu_steps, "omp32")
+                mixed["cpu_baseline"] = c32
+                if c32:
+                    log(f"CPU reference single precision: {c32['value']:.4g} cell-substeps/s")
     out = {
         "metric": "cell-substeps/s (1 GPU, 1.5M cells) FP64 vs mixed+batched; spin-up speedup",
         "value": value, "unit": "cell-substeps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -110,10 +110,10 @@ int alloc_state(cmf_hip_ctx* h, DevState<R>& S) {
     TRY(dalloc(h, &S.outflw_max, n)); TRY(dalloc(h, &S.rivdph_max, n)); TRY(dalloc(h, &S.storge_max, n));
     TRY(dalloc(h, &S.daminf_avg, h->o.ldamout ? n : 1));
     TRY(dalloc(h, &S.pthflw_avg, npl)); TRY(dalloc(h, &S.pthflwsum_avg, np));
-    TRY(dalloc(h, &S.partials, (size_t)h->nb_cells * NGLB));
+    h->hist_cap = 16;                                   // grown on demand (substeps per outer step)
+    TRY(dalloc(h, &S.partials, (size_t)h->hist_cap * h->nb_cells * NGLB));
     TRY(dalloc(h, &S.cfl_part, (size_t)h->nb_cells));
     TRY(dalloc(h, &S.ctl, 4));
-    h->hist_cap = 1024;
     TRY(dalloc(h, &S.glb_hist, (size_t)h->hist_cap * NGLB));
     S.rivout_vel = S.rivout;
     R* vb = nullptr;
@@ -399,11 +399,17 @@ int advance_t(cmf_hip_ctx* h, double* dt, int32_t nt_cap, int32_t* nt_out, doubl
         dts = dt_def * (double)(1.0f / (float)nt);                   // physics:206
     }
     const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
-    if (nrun > h->hist_cap) {
-        double* p = nullptr;
-        TRY(dalloc(h, &p, (size_t)nrun * NGLB));
+    if (nrun > h->hist_cap) {                           // grow the per-substep budget buffers
+        const int cap = nrun + nrun / 4;
+        double *p = nullptr, *q = nullptr;
+        TRY(dalloc(h, &p, (size_t)cap * NGLB));
+        TRY(dalloc(h, &q, (size_t)cap * h->nb_cells * NGLB));
+        (void)hipFree(S.glb_hist);
+        (void)hipFree(S.partials);
+        for (auto& a : h->allocs) if (a == (void*)S.glb_hist || a == (void*)S.partials) a = nullptr;
         S.glb_hist = p;
-        h->hist_cap = nrun;
+        S.partials = q;
+        h->hist_cap = cap;
     }
     double nadd = 0.0;                                               // diag:108
     for (int32_t it = 0; it < nrun; ++it) nadd = nadd + dts;
@@ -423,10 +429,11 @@ int advance_t(cmf_hip_ctx* h, double* dt, int32_t nt_cap, int32_t* nt_out, doubl
         if (h->timing) { TRY(ev_record(h, ne++)); TRY(ev_record(h, ne++)); }
         launch_rate<R>(h->M, Sk, A, h->stream);
         if (h->timing) { TRY(ev_record(h, ne++)); TRY(ev_record(h, ne++)); }
-        launch_update<R>(h->M, Sk, A, true, h->stream);
+        launch_update<R>(h->M, Sk, A, h->stream);
         if (h->timing) TRY(ev_record(h, ne++));
         std::swap(S.rivdph, S.rivdph_pre);                            // CALC_VARS_PRE for RIVDPH
     }
+    launch_budget(h->M, S.partials, nrun, S.glb_hist, h->stream);    // CALC_WATBAL sums, all substeps
     HIPCHK(hipGetLastError());
     h->vel_pending = false;
     if (glb && glb_cap > 0) {
@@ -498,7 +505,7 @@ int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* ou
 void cmf_hip_destroy(cmf_handle h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (void* p : h->allocs) (void)hipFree(p);
+    for (void* p : h->allocs) if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->h_ctl) (void)hipHostFree(h->h_ctl);
     if (h->stream) (void)hipStreamDestroy(h->stream);

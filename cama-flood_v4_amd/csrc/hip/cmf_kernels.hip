@@ -20,19 +20,27 @@ namespace cmf {
 
 // ---------------------------------------------------------------------------
 // Block reductions (fixed tree order: deterministic)
+// Sum over the block: a fixed butterfly inside each 64-lane wavefront, then the
+// BLOCK/64 wave sums in wave order. Result in lds[k * NWAVE] for thread-independent
+// reads after the final barrier.
+constexpr int NWAVE = BLOCK / 64;
 template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][BLOCK] */) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][NWAVE] */) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) lds[k * BLOCK + t] = v[k];
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k * NWAVE + wave] = v[k];
     __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if (t < s) {
-#pragma unroll
-            for (int k = 0; k < NV; ++k) lds[k * BLOCK + t] += lds[k * BLOCK + t + s];
-        }
-        __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int w = 0; w < NWAVE; ++w) s += lds[threadIdx.x * NWAVE + w];
+        lds[threadIdx.x * NWAVE] = s;
     }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -466,7 +474,7 @@ __device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& 
 
 template <class R>
 __global__ __launch_bounds__(BLOCK) void k_update(DevMap M, DevState<R> S, StepArgs A, int nb_cells) {
-    __shared__ double lds[NGLB * BLOCK];
+    __shared__ double lds[NGLB * NWAVE];
     if ((int)blockIdx.x < nb_cells) {
         const int i = blockIdx.x * BLOCK + threadIdx.x;
         double g[NGLB];
@@ -474,24 +482,28 @@ __global__ __launch_bounds__(BLOCK) void k_update(DevMap M, DevState<R> S, StepA
         for (int k = 0; k < NGLB; ++k) g[k] = 0.0;
         if (i < M.n) update_cell<R>(M, S, A, i, g);
         block_sum<NGLB>(g, lds);
-        if (threadIdx.x < NGLB) S.partials[(size_t)blockIdx.x * NGLB + threadIdx.x] = lds[threadIdx.x * BLOCK];
+        // per-substep slot: reduced once for all substeps at the end of the outer step
+        if (threadIdx.x < NGLB)
+            S.partials[((size_t)A.it * nb_cells + blockIdx.x) * NGLB + threadIdx.x] = lds[threadIdx.x * NWAVE];
     } else {
         const int p = (blockIdx.x - nb_cells) * BLOCK + threadIdx.x;
         if (p < M.npth) update_path<R>(M, S, A, p);
     }
 }
 
-// deterministic second-level reduction of the per-block budget sums
+// deterministic second-level reduction of the per-block budget sums: block b
+// reduces substep b of the outer step
 __global__ __launch_bounds__(BLOCK) void k_reduce_budget(const double* partials, int nb, double* out) {
-    __shared__ double lds[NGLB * BLOCK];
+    __shared__ double lds[NGLB * NWAVE];
+    const double* src = partials + (size_t)blockIdx.x * nb * NGLB;
     double v[NGLB];
 #pragma unroll
     for (int k = 0; k < NGLB; ++k) v[k] = 0.0;
     for (int b = threadIdx.x; b < nb; b += BLOCK)
 #pragma unroll
-        for (int k = 0; k < NGLB; ++k) v[k] += partials[(size_t)b * NGLB + k];
+        for (int k = 0; k < NGLB; ++k) v[k] += src[(size_t)b * NGLB + k];
     block_sum<NGLB>(v, lds);
-    if (threadIdx.x < NGLB) out[threadIdx.x] = lds[threadIdx.x * BLOCK];
+    if (threadIdx.x < NGLB) out[(size_t)blockIdx.x * NGLB + threadIdx.x] = lds[threadIdx.x * NWAVE];
 }
 
 // ---------------------------------------------------------------------------
@@ -574,13 +586,15 @@ void launch_rate(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipSt
 }
 
 template <class R>
-void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, bool budget, hipStream_t st) {
+void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st) {
     const int nbc = (M.n + BLOCK - 1) / BLOCK;
     const int nbp = A.lpthout ? (M.npth + BLOCK - 1) / BLOCK : 0;
     hipLaunchKernelGGL(k_update<R>, dim3(nbc + nbp), dim3(BLOCK), 0, st, M, S, A, nbc);
-    if (budget)
-        hipLaunchKernelGGL(k_reduce_budget, dim3(1), dim3(BLOCK), 0, st, S.partials, nbc,
-                           S.glb_hist + (size_t)A.it * NGLB);
+}
+
+void launch_budget(const DevMap& M, const double* partials, int nrun, double* glb_hist, hipStream_t st) {
+    const int nbc = (M.n + BLOCK - 1) / BLOCK;
+    hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(BLOCK), 0, st, partials, nbc, glb_hist);
 }
 
 #define CMF_INSTANTIATE(R)                                                                              \
@@ -588,7 +602,7 @@ void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, boo
                                   hipStream_t);                                                         \
     template void launch_outflow<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);   \
     template void launch_rate<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);      \
-    template void launch_update<R>(const DevMap&, const DevState<R>&, const StepArgs&, bool, hipStream_t);
+    template void launch_update<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);
 CMF_INSTANTIATE(double)
 CMF_INSTANTIATE(float)
 

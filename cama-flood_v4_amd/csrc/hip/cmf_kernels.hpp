@@ -12,6 +12,8 @@ void launch_outflow(const DevMap& M, const DevState<R>& S, const StepArgs& A, hi
 template <class R>
 void launch_rate(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st);
 template <class R>
-void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, bool budget, hipStream_t st);
+void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st);
+// reduce the per-block budget sums of nrun substeps into glb_hist[nrun][NGLB]
+void launch_budget(const DevMap& M, const double* partials, int nrun, double* glb_hist, hipStream_t st);
 
 }  // namespace cmf

@@ -21,7 +21,7 @@ REF=${CMF_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT=$HERE/_ref
 FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
-VARIANTS=${*:-serial64 omp64}
+VARIANTS=${*:-serial64 omp64 omp32}
 
 SRCS="parkind1 yos_cmf_input yos_cmf_time yos_cmf_map yos_cmf_prog yos_cmf_diag cmf_utils_mod
 cmf_calc_outflw_mod cmf_calc_pthout_mod cmf_calc_fldstg_mod cmf_calc_stonxt_mod cmf_calc_diag_mod cmf_opt_outflw_mod
