@@ -140,6 +140,126 @@ def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
                            f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
 
 
+ENS_DOMAINS = 16
+ENS_MEMBERS = 4           # members per domain: 64 member instances
+ENS_SIGMA = 0.3           # multiplicative lognormal runoff perturbation
+
+
+def ensemble_domains(seed, ndom=ENS_DOMAINS):
+    """Regional 0.1-degree domains of uneven size, 2e4 .. 4e5 river cells
+    (geometric-like spread, about 1.5e6 cells in total)."""
+    from cmf import maps, synth
+    sizes = 2e4 * 20.0 ** ((np.arange(ndom) / (ndom - 1)) ** 2.0)
+    nets, ms = [], []
+    for d, target in enumerate(sizes):
+        side = int(np.sqrt(target / 0.6 / 1.5))
+        nx, ny = int(side * 1.5), side
+        north = 70.0 - 8.0 * (d % 8)
+        net = synth.make_network(nx, ny, seed=seed + 101 * d, land_frac=0.6, west=-180.0 + 20.0 * d,
+                                 east=-180.0 + 20.0 * d + 0.1 * nx, north=north, south=north - 0.1 * ny,
+                                 regional=True)
+        nets.append(net)
+        ms.append(maps.build_map(net, maps.Options()))
+    return nets, ms
+
+
+def run_ensemble(nets, ms, precision, days, warmup, seed, tune=None, timing_days=2):
+    """Batched ensemble on one handle: every domain x ENS_MEMBERS members, daily
+    DT with a per-instance adaptive substep, forcing = domain runoff x
+    member factor (lognormal, sigma ENS_SIGMA, redrawn each day)."""
+    from cmf import maps, synth
+    from cmf.model import CMFModel
+    cm, dom_cells = maps.concat_maps(ms)
+    nm = ENS_MEMBERS
+    ninst = len(ms) * nm
+    st0 = [maps.cold_start(m) for m in ms]
+    init = {k: np.stack([np.concatenate([s[k] for s in st0])] * nm)
+            for k in ("rivsto", "fldsto", "rivout", "fldout", "rivdph_pre", "fldsto_pre")}
+    em = CMFModel(cm, precision, members=nm, dom_cells=dom_cells)
+    em.set_state(init)
+    em.physics_fldstg()
+    ndays = warmup + days + timing_days
+    ro = [synth.make_runoff(net, ndays, seed + 7 * d) for d, net in enumerate(nets)]
+    base = [np.concatenate([m.vec(ro[d][k]) for d, m in enumerate(ms)]) for k in range(ndays)]
+    rng = np.random.default_rng(seed)
+    fac = np.exp(rng.normal(-0.5 * ENS_SIGMA ** 2, ENS_SIGMA, size=(ndays, ninst)))
+    cells_inst = np.repeat(np.asarray(dom_cells, np.float64), nm)
+    if tune == "auto":
+        em.set_runoff_scaled(base[0], fac[0])
+        best, rep = em.autotune(86400.0, nsub=8)
+        tune_used, trials = best, rep.tolist()
+    else:
+        if tune:
+            em.set_tune(**tune)
+        tune_used, trials = em.get_tune(), []
+    day = 0
+    for _ in range(warmup):
+        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.ensemble_advance(86400.0)
+    em.synchronize()
+    work, t_total = 0.0, 0.0
+    for _ in range(days):
+        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.synchronize()
+        t0 = time.perf_counter()
+        nt, _dt = em.ensemble_advance(86400.0)
+        em.synchronize()
+        t_total += time.perf_counter() - t0
+        work += float(np.dot(cells_inst, nt))
+    # latency pass: per-instance completion time of its simulated day
+    lat = []
+    em.kernel_times(True)
+    for _ in range(timing_days):
+        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.ensemble_advance(86400.0)
+        lat.extend(em.instance_times().tolist())
+    kt = em.kernel_times(False)
+    st = em.get_state(["rivsto", "fldsto", "outflw", "rivdph"])
+    em.close()
+    return dict(work=work, t=t_total, lat=np.asarray(lat), kt=kt, tune=tune_used, trials=trials,
+                state=st, cells=int(sum(dom_cells)), ninst=ninst, dom_cells=dom_cells)
+
+
+def ensemble_section(a):
+    t0 = time.time()
+    nets, ms = ensemble_domains(a.seed)
+    log(f"ensemble: {len(ms)} domains, {sum(m.n for m in ms)} cells x {ENS_MEMBERS} members "
+        f"({time.time() - t0:.1f}s)")
+    res = {}
+    for prec in ("mixed", "fp64"):
+        r = run_ensemble(nets, ms, prec, a.ens_days, 1, a.seed, tune="auto")
+        res[prec] = r
+        log(f"ensemble {prec}: {r['work'] / r['t']:.4g} member-cell-substeps/s; tune {r['tune']}; "
+            f"latency p50 {np.percentile(r['lat'], 50):.1f} ms p99 {np.percentile(r['lat'], 99):.1f} ms")
+    # mixed vs FP64 storage error per instance after the same days
+    sm, sd = res["mixed"]["state"], res["fp64"]["state"]
+    tot_m = sm["rivsto"] + sm["fldsto"]
+    tot_d = sd["rivsto"] + sd["fldsto"]
+    offs = np.cumsum([0] + res["fp64"]["dom_cells"])
+    rel_tot, rel_max = [], []
+    for mi in range(ENS_MEMBERS):
+        for d in range(len(ms)):
+            a_ = tot_m[mi, offs[d]:offs[d + 1]]
+            b_ = tot_d[mi, offs[d]:offs[d + 1]]
+            rel_tot.append(abs(a_.sum() - b_.sum()) / b_.sum())
+            rel_max.append(float(np.max(np.abs(a_ - b_)) / np.max(np.abs(b_))))
+    r = res["mixed"]
+    rd = res["fp64"]
+    return dict(
+        metric="member-cell-substeps/s", value=r["work"] / r["t"], unit="member-cell-substeps/s",
+        precision="mixed", domains=len(ms), members_per_domain=ENS_MEMBERS, instances=r["ninst"],
+        cells=r["cells"], member_cells=r["cells"] * ENS_MEMBERS, days=a.ens_days, dt=86400.0,
+        ms_per_day=1e3 * r["t"] / a.ens_days,
+        latency_ms={"p50": float(np.percentile(r["lat"], 50)), "p99": float(np.percentile(r["lat"], 99)),
+                    "samples": int(r["lat"].size)},
+        tune=r["tune"],
+        fp64={"value": rd["work"] / rd["t"], "ms_per_day": 1e3 * rd["t"] / a.ens_days,
+              "latency_ms": {"p50": float(np.percentile(rd["lat"], 50)),
+                             "p99": float(np.percentile(rd["lat"], 99))}, "tune": rd["tune"]},
+        mixed_vs_fp64_storage={"total_rel_max": float(max(rel_tot)), "cell_rel_max": float(max(rel_max))},
+    )
+
+
 def roofline(kt, n_cells, bytes_tab):
     steps = max(kt[4], 1.0)
     per = {"outflow": kt[0] / steps, "rate": kt[1] / steps, "update": kt[2] / steps}
@@ -163,6 +283,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-days", type=int, default=3)
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
@@ -198,6 +320,11 @@ def main():
         mixed = dict(value=n * sum_nt2 / t2, unit="cell-substeps/s", sum_nt=sum_nt2,
                      ms_per_step=1e3 * t2 / a.steps, roofline=roof2)
         log(f"mixed: {mixed['value']:.4g} cell-substeps/s; kernels ms {per2}")
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_section(a)
+        ens["vs_fp64_single_gpu"] = ens["value"] / value
+        log(f"ensemble (mixed) / FP64 single model: {ens['vs_fp64_single_gpu']:.2f}x")
     cpu = None
     if not a.no_cpu:
         log("CPU baseline (reference OpenMP)")
@@ -221,6 +348,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "fp64_nseq_order": order_cmp,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

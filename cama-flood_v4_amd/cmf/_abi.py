@@ -16,6 +16,7 @@ HOST_FUNCS = {
     "cmfh_storage_sea_surface": (I, [I, I, P, P, P, P, P, P, P, P, P]),
     "cmfh_priority_flood": (I, [I, I, P, P, C.c_float, P, P, P]),
     "cmfh_upstream_area": (I, [I, I, P, P, P, P]),
+    "cmfh_accumulate": (I, [I, P, I, P, P]),
 }
 
 
@@ -52,9 +53,28 @@ class cmf_state(C.Structure):
     _fields_ = [(k, P) for k in STATE_FIELDS]
 
 
+class cmf_ensemble(C.Structure):
+    _fields_ = [("ndom", C.c_int32), ("dom_cells", P), ("members", C.c_int32)]
+
+
+TUNE_FIELDS = ("bs_outflow", "bs_rate", "bs_update", "fused", "bs_fused", "grid_fused", "xcd", "graph")
+
+
+class cmf_tune(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in TUNE_FIELDS]
+
+
 H = C.c_void_p
 HIP_FUNCS = {
     "cmf_hip_create": (I, [C.POINTER(cmf_dims), C.POINTER(cmf_options), C.POINTER(H)]),
+    "cmf_hip_create_ensemble": (I, [C.POINTER(cmf_dims), C.POINTER(cmf_options), C.POINTER(cmf_ensemble),
+                                    C.POINTER(H)]),
+    "cmf_hip_set_runoff_scaled": (I, [H, P, P]),
+    "cmf_hip_ensemble_advance": (I, [H, C.c_double, C.c_int32, P, P]),
+    "cmf_hip_instance_times": (I, [H, P]),
+    "cmf_hip_set_tune": (I, [H, C.POINTER(cmf_tune)]),
+    "cmf_hip_get_tune": (I, [H, C.POINTER(cmf_tune)]),
+    "cmf_hip_autotune": (I, [H, C.c_double, C.c_int32, C.POINTER(cmf_tune), P, C.c_int32]),
     "cmf_hip_destroy": (None, [H]),
     "cmf_hip_last_error": (C.c_char_p, []),
     "cmf_hip_set_map": (I, [H, C.POINTER(cmf_map)]),
@@ -67,6 +87,23 @@ HIP_FUNCS = {
                                            P, C.c_int32]),
     "cmf_hip_synchronize": (I, [H]),
     "cmf_hip_kernel_times": (I, [H, C.c_int32, P]),
+}
+
+
+MLP_FUNCS = {
+    "cmf_mlp_create": (I, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(H)]),
+    "cmf_mlp_destroy": (None, [H]),
+    "cmf_mlp_last_error": (C.c_char_p, []),
+    "cmf_mlp_param_count": (C.c_int64, [H]),
+    "cmf_mlp_set_params": (I, [H, P]),
+    "cmf_mlp_get_params": (I, [H, P, P]),
+    "cmf_mlp_forward": (I, [H, P, C.c_int64, P]),
+    "cmf_mlp_grad": (I, [H, P, P, C.c_int32, P]),
+    "cmf_mlp_set_data": (I, [H, P, P, C.c_int64]),
+    "cmf_mlp_train": (I, [H, P, C.c_int64, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                          C.c_int32, P]),
+    "cmf_mlp_gemm_test": (I, [C.c_int32, C.c_int32, C.c_int32, P, C.c_int32, C.c_int32, P, C.c_int32,
+                              C.c_int32, P]),
 }
 
 
@@ -83,6 +120,7 @@ def declare_host(lib):
 
 def declare_hip(lib):
     _declare(lib, HIP_FUNCS)
+    _declare(lib, MLP_FUNCS)
 
 
 def ptr(a):

@@ -302,3 +302,58 @@ def init_dams(m: CMFMap, net, st: dict) -> None:
     m.i1dam, m.damseq, m.damstat = i1dam, damseq, damstat
     m.dam_convol, m.dam_emevol, m.dam_adjvol = convol, emevol, adjvol
     m.dam_qn, m.dam_qf, m.dam_qa = qn, qf, qa
+
+
+_CELL = ("grarea", "elevtn", "nxtdst", "rivlen", "rivwth", "rivhgt", "rivman", "rivstomax", "rivelv",
+         "dwnelv", "i2mask", "seqx", "seqy")
+_LEVEE = ("levhgt", "levfrc", "bashgt", "levdst", "levbassto", "levtopsto", "levfilsto")
+_DAM = ("damstat", "dam_convol", "dam_emevol", "dam_adjvol", "dam_qa", "dam_qn", "dam_qf")
+
+
+def concat_maps(ms) -> tuple:
+    """Concatenate the maps of independent regional domains into one map for a
+    batched handle: domain d owns the next ms[d].n cells (its NSEQ order kept),
+    downstream links, upstream lists, paths and dams are shifted into the
+    concatenated numbering. Returns (map, dom_cells). All domains must share the
+    options, NLFP and NPTHLEV."""
+    ms = list(ms)
+    o = ms[0].opts
+    for m in ms:
+        if m.opts != o or m.nlfp != ms[0].nlfp:
+            raise ValueError("domains differ in options or NLFP")
+    offs = np.cumsum([0] + [m.n for m in ms])
+    n = int(offs[-1])
+    cat = lambda k: np.concatenate([getattr(m, k) for m in ms])
+    nxt = np.concatenate([np.where(m.next >= 0, m.next + off, m.next) for m, off in zip(ms, offs)]).astype(np.int32)
+    owners = np.flatnonzero(nxt >= 0)
+    up_ptr, up_idx = _csr(n, nxt[owners], owners)
+    fl = lambda k: np.ascontiguousarray(np.concatenate([getattr(m, k) for m in ms], axis=1))
+    out = CMFMap(0, 0, n, int(sum(m.nseqriv for m in ms)), ms[0].nlfp, o, cat("seqx"), cat("seqy"), nxt,
+                 up_ptr, up_idx if len(up_idx) else np.zeros(1, np.int32),
+                 *[cat(k) for k in _CELL[:10]], fl("fldhgt"), fl("fldstomax"), fl("fldgrd"), ms[0].dfrcinc,
+                 cat("i2mask").astype(np.int32))
+    if o.lpthout:
+        if len({m.npthlev for m in ms}) != 1:
+            raise ValueError("domains differ in NPTHLEV")
+        sh = lambda a, off: np.where(a >= 0, a + off, a).astype(np.int32)
+        out.npth, out.npthlev = int(sum(m.npth for m in ms)), ms[0].npthlev
+        out.pth_upst = np.concatenate([sh(m.pth_upst, off) for m, off in zip(ms, offs)])
+        out.pth_down = np.concatenate([sh(m.pth_down, off) for m, off in zip(ms, offs)])
+        out.pth_dst = cat("pth_dst")
+        out.pth_elv, out.pth_wth = fl("pth_elv"), fl("pth_wth")
+        out.pth_man = ms[0].pth_man.copy()
+        ok = (out.pth_upst >= 0) & (out.pth_down >= 0)
+        pids = np.flatnonzero(ok)
+        out.pout_ptr, out.pout_idx = _csr(n, out.pth_upst[pids], pids)
+        out.pinf_ptr, out.pinf_idx = _csr(n, out.pth_down[pids], pids)
+    if o.llevee:
+        for k in _LEVEE:
+            setattr(out, k, cat(k))
+    if o.ldamout:
+        out.ndam = int(sum(m.ndam for m in ms))
+        out.i1dam = cat("i1dam").astype(np.int32)
+        out.damseq = np.concatenate([np.where(m.damseq >= 0, m.damseq + off, m.damseq)
+                                     for m, off in zip(ms, offs)]).astype(np.int32)
+        for k in _DAM:
+            setattr(out, k, cat(k))
+    return out, [m.n for m in ms]

@@ -9,6 +9,12 @@ reference's two physics entry points with their names and meaning:
 plus the state movement a device-resident model needs (``set_state``,
 ``get_state``, ``set_runoff``). Failures raise ``CMFError`` with the library's
 message; there is no CPU fallback.
+
+Ensembles: ``CMFModel(concatenated_map, members=M, dom_cells=[...])`` runs M
+runoff realisations of every domain of a concatenated map
+(:func:`cmf.maps.concat_maps`) in one handle; per-cell arrays then have shape
+(M, NSEQALL) and path arrays (M, NPTHLEV, NPTHOUT) / (M, NPTHOUT).
+``ensemble_advance`` advances every instance with its own adaptive step.
 """
 from __future__ import annotations
 
@@ -18,7 +24,8 @@ from typing import Iterable, Optional
 import numpy as np
 
 from . import _native
-from ._abi import STATE_FIELDS, cmf_dims, cmf_map, cmf_options, cmf_state, ptr
+from ._abi import (STATE_FIELDS, TUNE_FIELDS, cmf_dims, cmf_ensemble, cmf_map, cmf_options, cmf_state,
+                   cmf_tune, ptr)
 from .maps import CMFMap
 
 PREC_FP64 = 0
@@ -41,11 +48,16 @@ def _i32(a):
 
 
 class CMFModel:
-    def __init__(self, m: CMFMap, precision: str = "fp64", opts=None):
+    def __init__(self, m: CMFMap, precision: str = "fp64", opts=None, members: int = 1,
+                 dom_cells: Optional[Iterable[int]] = None):
         self.lib = _native.hip()
         self.map = m
         o = opts if opts is not None else m.opts
         self.opts = o
+        self.members = int(members)
+        self.dom_cells = [m.n] if dom_cells is None else [int(c) for c in dom_cells]
+        self.ninst = len(self.dom_cells) * self.members
+        self.ensemble = self.ninst > 1 or dom_cells is not None
         self.prec = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[precision]
         d = cmf_dims(m.n, m.nseqriv, m.nlfp, m.npth if o.lpthout else 0,
                      m.npthlev if o.lpthout else 0, m.ndam if o.ldamout else 0)
@@ -53,7 +65,12 @@ class CMFModel:
                          int(o.ldamout), int(o.llevee), self.prec,
                          o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp)
         h = C.c_void_p()
-        self._check(self.lib.cmf_hip_create(C.byref(d), C.byref(op), C.byref(h)))
+        if self.ensemble:
+            dc = np.ascontiguousarray(self.dom_cells, np.int32)
+            ens = cmf_ensemble(len(self.dom_cells), dc.ctypes.data, self.members)
+            self._check(self.lib.cmf_hip_create_ensemble(C.byref(d), C.byref(op), C.byref(ens), C.byref(h)))
+        else:
+            self._check(self.lib.cmf_hip_create(C.byref(d), C.byref(op), C.byref(h)))
         self.h = h
         self._keep = []
         self._set_map(m)
@@ -100,9 +117,20 @@ class CMFModel:
         """Host -> device copy of the given arrays (names of cmf_state)."""
         s = cmf_state()
         keep = []
+        m = self.map
         for k, v in st.items():
             if k in STATE_FIELDS and v is not None:
+                if not self._available(k):
+                    continue
                 a = _f64(v)
+                if k in ("pthflw", "pthflw_pre", "pthflw_avg"):
+                    want = m.npthlev * m.npth
+                elif k in ("pthflwsum", "pthflwsum_avg"):
+                    want = m.npth
+                else:
+                    want = m.n
+                if a.size != want * self.members:
+                    raise CMFError(f"set_state: {k} has {a.size} values, expected {want * self.members}")
                 keep.append(a)
                 setattr(s, k, a.ctypes.data)
         self._check(self.lib.cmf_hip_set_state(self.h, C.byref(s)))
@@ -115,12 +143,13 @@ class CMFModel:
         for k in names:
             if not self._available(k):
                 continue
+            lead = (self.members,) if self.members > 1 else ()
             if k in ("pthflw", "pthflw_pre", "pthflw_avg"):
-                a = np.zeros((m.npthlev, m.npth))
+                a = np.zeros(lead + (m.npthlev, m.npth))
             elif k in ("pthflwsum", "pthflwsum_avg"):
-                a = np.zeros(m.npth)
+                a = np.zeros(lead + (m.npth,))
             else:
-                a = np.zeros(m.n)
+                a = np.zeros(lead + (m.n,))
             out[k] = a
             setattr(s, k, a.ctypes.data)
         self._check(self.lib.cmf_hip_get_state(self.h, C.byref(s)))
@@ -137,8 +166,19 @@ class CMFModel:
         return True
 
     def set_runoff(self, runoff: np.ndarray) -> None:
+        """D2RUNOFF; shape (NSEQALL,) or (members, NSEQALL) for ensembles."""
         a = _f64(runoff)
+        if a.size != self.members * self.map.n:
+            raise CMFError(f"runoff has {a.size} values, expected {self.members * self.map.n}")
         self._check(self.lib.cmf_hip_set_runoff(self.h, a.ctypes.data))
+
+    def set_runoff_scaled(self, base: np.ndarray, factor: np.ndarray) -> None:
+        """Runoff of instance k (domain d, member m; k = d*members + m) =
+        base[cell] * factor[k]."""
+        b, f = _f64(base), _f64(factor)
+        if b.size != self.map.n or f.size != self.ninst:
+            raise CMFError("set_runoff_scaled: base needs NSEQALL values, factor one per instance")
+        self._check(self.lib.cmf_hip_set_runoff_scaled(self.h, b.ctypes.data, f.ctypes.data))
 
     # ------------------------------------------------------------------
     def physics_fldstg(self) -> None:
@@ -159,6 +199,44 @@ class CMFModel:
         self._check(self.lib.cmf_hip_physics_advance_capped(
             self.h, C.byref(d), nt_cap, C.byref(nt), None, 0))
         return nt.value
+
+    def ensemble_advance(self, dt: float, nt_cap: int = 0):
+        """One outer step of length dt of every instance. Returns (NT, DT) arrays
+        of length ninst (instance k = domain*members + member)."""
+        nt = np.zeros(self.ninst, np.int32)
+        dto = np.zeros(self.ninst)
+        self._check(self.lib.cmf_hip_ensemble_advance(self.h, float(dt), int(nt_cap), nt.ctypes.data,
+                                                      dto.ctypes.data))
+        return nt, dto
+
+    def instance_times(self) -> np.ndarray:
+        """Per-instance completion time [ms] of the last timed advance."""
+        out = np.zeros(self.ninst)
+        self._check(self.lib.cmf_hip_instance_times(self.h, out.ctypes.data))
+        return out
+
+    def set_tune(self, **kw) -> dict:
+        t = self.get_tune()
+        t.update(kw)
+        ct = cmf_tune(*[int(t[k]) for k in TUNE_FIELDS])
+        self._check(self.lib.cmf_hip_set_tune(self.h, C.byref(ct)))
+        return self.get_tune()
+
+    def get_tune(self) -> dict:
+        ct = cmf_tune()
+        self._check(self.lib.cmf_hip_get_tune(self.h, C.byref(ct)))
+        return {k: getattr(ct, k) for k in TUNE_FIELDS}
+
+    def autotune(self, dt: float, nsub: int = 8):
+        """Time nsub substeps under each launch configuration, keep the fastest
+        and restore the state. Returns (best configuration, trial table [k, 8])."""
+        cap = 64
+        rep = np.zeros((cap, 8))
+        ct = cmf_tune()
+        n = self.lib.cmf_hip_autotune(self.h, float(dt), int(nsub), C.byref(ct), rep.ctypes.data, cap)
+        if n < 0:
+            self._check(n)
+        return {k: getattr(ct, k) for k in TUNE_FIELDS}, rep[:min(n, cap)].copy()
 
     def synchronize(self) -> None:
         self._check(self.lib.cmf_hip_synchronize(self.h))

@@ -105,10 +105,15 @@ def _haversine(lat1, lon1, lat2, lon2):
 
 def make_network(nx: int, ny: int, seed: int, land_frac: float = 0.2315,
                  relief: float = 3000.0, west=-180.0, east=180.0, north=90.0,
-                 south=-90.0) -> SynthNetwork:
-    """River network on an nx x ny global grid (see module docstring)."""
+                 south=-90.0, regional: bool = False) -> SynthNetwork:
+    """River network on an nx x ny grid (see module docstring). regional=True
+    makes the outer ring of the grid ocean, so that no river crosses the edges
+    of a regional extent."""
     rng = np.random.default_rng(seed)
     z = _fractal(nx, ny, 3.0, rng)
+    if regional:
+        z[0, :] = z[-1, :] = z.min()
+        z[:, 0] = z[:, -1] = z.min()
     thr = np.quantile(z, 1.0 - land_frac)
     land = z > thr
     elev = np.where(land, (z - thr) / (z.max() - thr) * relief, 0.0).astype(np.float32)

@@ -35,6 +35,12 @@ int fail(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(CMF_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
+// one registered per-slot state array (for the tuner's snapshot/restore)
+struct StateArr {
+    void** slot;           // address of the pointer member in DevState (follows swaps)
+    size_t bytes;
+};
+
 struct cmf_hip_ctx {
     cmf_dims d{};
     cmf_options o{};
@@ -47,17 +53,39 @@ struct cmf_hip_ctx {
     DevState<float> S32{};
     void* vel_buf = nullptr;          // D2RIVOUT when it differs from D2RIVOUT_PRE
     bool vel_pending = false;
-    double* h_ctl = nullptr;          // pinned
-    int nb_cells = 0;
+    double* h_ctl = nullptr;          // pinned scratch for small device->host reads
+    // ensemble: ndom domains concatenated in NSEQ, nmem = 2^lgm members each
+    int ndom = 1, nmem = 1, lgm = 0, ninst = 1;
+    std::vector<int32_t> dom_off;     // NSEQ offset of each domain (+ total)
+    InstStep* h_inst = nullptr;       // pinned [ninst]
+    unsigned long long* h_cfl = nullptr;  // pinned [ninst]
+    std::vector<int32_t> last_nt;     // NT of each instance in the last advance
+    std::vector<double> last_dt;
+    InstStep* d_inst = nullptr;
+    double *d_rbase = nullptr, *d_rfac = nullptr;   // staging of cmf_hip_set_runoff_scaled
+    size_t partial_cap = 0;           // doubles in S.partials
     int hist_cap = 0;
     // device cell order: perm[p] = NSEQ index of device position p (identity when
     // CMF_HIP_ORDER=nseq); the C-ABI always speaks NSEQ order
     std::vector<int32_t> perm, iperm;
     bool permuted = false;
+    // host index of every device slot: cells (ns), path levels (npthlev*nsp), paths (nsp)
+    std::vector<int64_t> hx_cell, hx_pthl, hx_pth;
+    // launch configuration
+    cmf_tune tune{};
+    int coop_max = 0;                 // co-resident blocks of the fused kernel at tune.bs_fused
+    // graphs of whole outer-step substep sequences, keyed by (NT, buffer parity)
+    struct GraphEntry { int32_t nt; int parity; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
+    void* dph_a = nullptr;            // the RIVDPH buffer allocated first (parity reference)
+    std::vector<StateArr> state_arrs;
+    void* snapshot = nullptr;
+    size_t snapshot_bytes = 0;
     // per-kernel timing (HIP events on the handle's stream)
     bool timing = false;
     std::vector<hipEvent_t> ev;
     double t_ms[5] = {0, 0, 0, 0, 0};
+    std::vector<double> inst_ms;      // per-instance completion of the last timed advance
 };
 
 namespace {
@@ -85,70 +113,94 @@ int dalloc(cmf_hip_ctx* h, T** p, size_t n) {
         if (r_ != 0) return r_; \
     } while (0)
 
+// state array of ns/nsp slots, registered for the tuner's snapshot
+template <class R, class T>
+int salloc(cmf_hip_ctx* h, T** p, size_t n, bool snap = true) {
+    TRY(dalloc(h, p, n));
+    if (snap) h->state_arrs.push_back({reinterpret_cast<void**>(p), (n ? n : 1) * sizeof(T)});
+    return 0;
+}
+
 template <class R>
 int alloc_state(cmf_hip_ctx* h, DevState<R>& S) {
-    const size_t n = (size_t)h->d.nseqall;
-    const size_t np = (size_t)(h->o.lpthout ? h->d.npthout : 0);
-    const size_t npl = np * (size_t)(h->o.lpthout ? h->d.npthlev : 0);
-    TRY(dalloc(h, &S.rivsto, n)); TRY(dalloc(h, &S.fldsto, n));
-    TRY(dalloc(h, &S.levsto, h->o.llevee ? n : 1));
-    TRY(dalloc(h, &S.damsto, h->o.ldamout ? n : 1));
-    TRY(dalloc(h, &S.daminf, h->o.ldamout ? n : 1));
-    TRY(dalloc(h, &S.rivout, n)); TRY(dalloc(h, &S.fldout, n));
-    TRY(dalloc(h, &S.qr_u, n)); TRY(dalloc(h, &S.qf_u, n));
-    TRY(dalloc(h, &S.rivdph_pre, n)); TRY(dalloc(h, &S.fldsto_pre, n));
-    TRY(dalloc(h, &S.rivdph, n)); TRY(dalloc(h, &S.flddph, n)); TRY(dalloc(h, &S.fldfrc, n));
-    TRY(dalloc(h, &S.fldare, n)); TRY(dalloc(h, &S.sfcelv, n)); TRY(dalloc(h, &S.storge, n));
-    TRY(dalloc(h, &S.outflw, n)); TRY(dalloc(h, &S.rivvel, n));
-    TRY(dalloc(h, &S.levdph, h->o.llevee ? n : 1));
-    TRY(dalloc(h, &S.rivinf, n)); TRY(dalloc(h, &S.fldinf, n)); TRY(dalloc(h, &S.pthout, n));
-    TRY(dalloc(h, &S.rate, n)); TRY(dalloc(h, &S.runoff, n));
-    TRY(dalloc(h, &S.pthflw, npl)); TRY(dalloc(h, &S.pthflw_u, npl));
-    TRY(dalloc(h, &S.pthflwsum, np)); TRY(dalloc(h, &S.pthflwsum_u, np));
-    TRY(dalloc(h, &S.rivout_avg, n)); TRY(dalloc(h, &S.fldout_avg, n)); TRY(dalloc(h, &S.outflw_avg, n));
-    TRY(dalloc(h, &S.rivvel_avg, n)); TRY(dalloc(h, &S.pthout_avg, n)); TRY(dalloc(h, &S.runoff_avg, n));
-    TRY(dalloc(h, &S.outflw_max, n)); TRY(dalloc(h, &S.rivdph_max, n)); TRY(dalloc(h, &S.storge_max, n));
-    TRY(dalloc(h, &S.daminf_avg, h->o.ldamout ? n : 1));
-    TRY(dalloc(h, &S.pthflw_avg, npl)); TRY(dalloc(h, &S.pthflwsum_avg, np));
+    const size_t ns = (size_t)h->d.nseqall * h->nmem;
+    const size_t nsp = (size_t)(h->o.lpthout ? h->d.npthout : 0) * h->nmem;
+    const size_t nspl = nsp * (size_t)(h->o.lpthout ? h->d.npthlev : 0);
+    const size_t nlev = h->o.llevee ? ns : 1, ndm = h->o.ldamout ? ns : 1;
+    TRY((salloc<R>(h, &S.rivsto, ns))); TRY((salloc<R>(h, &S.fldsto, ns)));
+    TRY((salloc<R>(h, &S.levsto, nlev)));
+    TRY((salloc<R>(h, &S.damsto, ndm))); TRY((salloc<R>(h, &S.daminf, ndm)));
+    TRY((salloc<R>(h, &S.rivout, ns))); TRY((salloc<R>(h, &S.fldout, ns)));
+    TRY((salloc<R>(h, &S.qr_u, ns, false))); TRY((salloc<R>(h, &S.qf_u, ns, false)));
+    TRY((salloc<R>(h, &S.rivdph_pre, ns))); TRY((salloc<R>(h, &S.fldsto_pre, ns)));
+    TRY((salloc<R>(h, &S.rivdph, ns))); TRY((salloc<R>(h, &S.flddph, ns))); TRY((salloc<R>(h, &S.fldfrc, ns)));
+    TRY((salloc<R>(h, &S.fldare, ns))); TRY((salloc<R>(h, &S.sfcelv, ns))); TRY((salloc<R>(h, &S.storge, ns)));
+    TRY((salloc<R>(h, &S.outflw, ns))); TRY((salloc<R>(h, &S.rivvel, ns)));
+    TRY((salloc<R>(h, &S.levdph, nlev)));
+    TRY((salloc<R>(h, &S.rivinf, ns))); TRY((salloc<R>(h, &S.fldinf, ns))); TRY((salloc<R>(h, &S.pthout, ns)));
+    TRY((salloc<R>(h, &S.rate, ns, false))); TRY((salloc<R>(h, &S.runoff, ns)));
+    TRY((salloc<R>(h, &S.pthflw, nspl))); TRY((salloc<R>(h, &S.pthflw_u, nspl, false)));
+    TRY((salloc<R>(h, &S.pthflwsum, nsp))); TRY((salloc<R>(h, &S.pthflwsum_u, nsp, false)));
+    TRY((salloc<R>(h, &S.rivout_avg, ns))); TRY((salloc<R>(h, &S.fldout_avg, ns)));
+    TRY((salloc<R>(h, &S.outflw_avg, ns))); TRY((salloc<R>(h, &S.rivvel_avg, ns)));
+    TRY((salloc<R>(h, &S.pthout_avg, ns))); TRY((salloc<R>(h, &S.runoff_avg, ns)));
+    TRY((salloc<R>(h, &S.outflw_max, ns))); TRY((salloc<R>(h, &S.rivdph_max, ns)));
+    TRY((salloc<R>(h, &S.storge_max, ns)));
+    TRY((salloc<R>(h, &S.daminf_avg, ndm)));
+    TRY((salloc<R>(h, &S.pthflw_avg, nspl))); TRY((salloc<R>(h, &S.pthflwsum_avg, nsp)));
     h->hist_cap = 16;                                   // grown on demand (substeps per outer step)
-    TRY(dalloc(h, &S.partials, (size_t)h->hist_cap * h->nb_cells * NGLB));
-    TRY(dalloc(h, &S.cfl_part, (size_t)h->nb_cells));
-    TRY(dalloc(h, &S.ctl, 4));
+    h->partial_cap = 0;
+    S.partials = nullptr;
     TRY(dalloc(h, &S.glb_hist, (size_t)h->hist_cap * NGLB));
+    TRY(dalloc(h, &S.cfl, (size_t)h->ninst));
+    TRY(dalloc(h, &h->d_inst, (size_t)h->ninst));
+    S.inst = h->d_inst;
+    TRY(dalloc(h, &S.bar, 2));
+    TRY(dalloc(h, &S.err, 1));
     S.rivout_vel = S.rivout;
     R* vb = nullptr;
-    TRY(dalloc(h, &vb, n));
+    TRY(dalloc(h, &vb, ns));
     h->vel_buf = vb;
+    h->dph_a = S.rivdph;
     return 0;
 }
 
 bool is_f32(double x) { return (double)(float)x == x; }
 
-// host copies with element-type conversion
-// Host <-> device copies with element-type conversion. cell=true arrays are
-// per-cell (NSEQ order on the host, device order on the device).
+enum Kind { K_CELL, K_PTHL, K_PTH };
+
+const std::vector<int64_t>& hx_of(const cmf_hip_ctx* h, Kind k) {
+    return k == K_CELL ? h->hx_cell : (k == K_PTHL ? h->hx_pthl : h->hx_pth);
+}
+
+// Host <-> device copies of per-slot arrays with element-type conversion. Host
+// arrays hold nmem member planes, each in NSEQ (or path) order; the device holds
+// the member-minor slots in device cell order.
 template <class R>
-int put(cmf_hip_ctx* h, R* dst, const double* src, size_t n, bool cell = true) {
+int put(cmf_hip_ctx* h, R* dst, const double* src, Kind kind) {
+    const std::vector<int64_t>& hx = hx_of(h, kind);
+    const size_t n = hx.size();
     if (!src || n == 0) return 0;
-    const bool perm = cell && h->permuted;
-    if (sizeof(R) == sizeof(double) && !perm) {
+    const bool ident = !h->permuted && h->nmem == 1;
+    if (sizeof(R) == sizeof(double) && ident) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;
     }
     std::vector<R> tmp(n);
-    if (perm) for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[h->perm[k]];
-    else for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[k];
+    for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[hx[k]];
     HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
 template <class R>
-int get(cmf_hip_ctx* h, double* dst, const R* src, size_t n, bool cell = true) {
+int get(cmf_hip_ctx* h, double* dst, const R* src, Kind kind) {
+    const std::vector<int64_t>& hx = hx_of(h, kind);
+    const size_t n = hx.size();
     if (!dst || n == 0) return 0;
-    const bool perm = cell && h->permuted;
-    if (sizeof(R) == sizeof(double) && !perm) {
+    const bool ident = !h->permuted && h->nmem == 1;
+    if (sizeof(R) == sizeof(double) && ident) {
         HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;
@@ -156,8 +208,7 @@ int get(cmf_hip_ctx* h, double* dst, const R* src, size_t n, bool cell = true) {
     std::vector<R> tmp(n);
     HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (perm) for (size_t k = 0; k < n; ++k) dst[h->perm[k]] = (double)tmp[k];
-    else for (size_t k = 0; k < n; ++k) dst[k] = (double)tmp[k];
+    for (size_t k = 0; k < n; ++k) dst[hx[k]] = (double)tmp[k];
     return 0;
 }
 
@@ -197,16 +248,18 @@ int put_f32(cmf_hip_ctx* h, const float** dst, const double* src, size_t n, cons
     return 0;
 }
 
-// Device cell order: depth-first post-order of the river tree from each mouth,
-// upstream branches in ascending NSEQ order. A cell's upstream subtree precedes
-// it and a chain of cells is contiguous, so the downstream and upstream gathers of
-// a wavefront mostly touch the same or adjacent cache lines.
-bool dfs_order(int n, int nr, const std::vector<int32_t>& up_ptr, const std::vector<int32_t>& up_idx,
-               std::vector<int32_t>& perm) {
+// Device cell order: depth-first post-order of the river tree from each mouth
+// (mouths in ascending NSEQ order), upstream branches in ascending NSEQ order. A
+// cell's upstream subtree precedes it and a chain of cells is contiguous, so the
+// downstream and upstream gathers of a wavefront mostly touch the same or
+// adjacent cache lines. Domains stay contiguous: every tree lies in one domain.
+bool dfs_order(int n, const std::vector<int32_t>& nxt, const std::vector<int32_t>& up_ptr,
+               const std::vector<int32_t>& up_idx, std::vector<int32_t>& perm) {
     perm.assign((size_t)n, -1);
     std::vector<std::pair<int32_t, int32_t>> stack;
     int pos = 0;
-    for (int root = nr; root < n; ++root) {
+    for (int root = 0; root < n; ++root) {
+        if (nxt[(size_t)root] >= 0) continue;
         stack.push_back({root, up_ptr[root]});
         while (!stack.empty()) {
             auto& top = stack.back();
@@ -268,93 +321,87 @@ DevState<float>& state_of<float>(cmf_hip_ctx* h) { return h->S32; }
 template <class R>
 int set_state_t(cmf_hip_ctx* h, const cmf_state* st) {
     DevState<R>& S = state_of<R>(h);
-    const size_t n = (size_t)h->d.nseqall;
-    const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
-    const size_t npl = np * (h->o.lpthout ? (size_t)h->d.npthlev : 0);
-    auto putd = [&](double* dst, const double* src, size_t m) -> int { return put<double>(h, dst, src, m); };
-    TRY(putd(S.rivsto, st->rivsto, n));
-    TRY(putd(S.fldsto, st->fldsto, n));
-    if (h->o.llevee) TRY(putd(S.levsto, st->levsto, n));
-    if (h->o.ldamout) { TRY(putd(S.damsto, st->damsto, n)); TRY(putd(S.daminf, st->daminf, n)); }
+    auto putd = [&](double* dst, const double* src) -> int { return put<double>(h, dst, src, K_CELL); };
+    TRY(putd(S.rivsto, st->rivsto));
+    TRY(putd(S.fldsto, st->fldsto));
+    if (h->o.llevee) TRY(putd(S.levsto, st->levsto));
+    if (h->o.ldamout) { TRY(putd(S.damsto, st->damsto)); TRY(putd(S.daminf, st->daminf)); }
     // D2RIVOUT_PRE lives in the RIVOUT array; a different D2RIVOUT is kept for the
     // RIVVEL of the next substep only (outflw:75 reads the pre-update RIVOUT).
     const double* rpre = st->rivout_pre ? st->rivout_pre : st->rivout;
-    TRY(put<R>(h, S.rivout, rpre, n));
+    TRY(put<R>(h, S.rivout, rpre, K_CELL));
     if (st->rivout) {
-        TRY(put<R>(h, (R*)h->vel_buf, st->rivout, n));
+        TRY(put<R>(h, (R*)h->vel_buf, st->rivout, K_CELL));
         h->vel_pending = true;
     }
     const double* fpre = st->fldout_pre ? st->fldout_pre : st->fldout;
-    TRY(put<R>(h, S.fldout, fpre, n));
-    TRY(put<R>(h, S.rivdph_pre, st->rivdph_pre, n));
-    TRY(put<R>(h, S.fldsto_pre, st->fldsto_pre, n));
+    TRY(put<R>(h, S.fldout, fpre, K_CELL));
+    TRY(put<R>(h, S.rivdph_pre, st->rivdph_pre, K_CELL));
+    TRY(put<R>(h, S.fldsto_pre, st->fldsto_pre, K_CELL));
     if (h->o.lpthout) {
         const double* ppre = st->pthflw_pre ? st->pthflw_pre : st->pthflw;
-        TRY(put<R>(h, S.pthflw, ppre, npl, false));
-        TRY(put<R>(h, S.pthflwsum, st->pthflwsum, np, false));
+        TRY(put<R>(h, S.pthflw, ppre, K_PTHL));
+        TRY(put<R>(h, S.pthflwsum, st->pthflwsum, K_PTH));
     }
-    TRY(put<R>(h, S.rivdph, st->rivdph, n));
-    TRY(put<R>(h, S.flddph, st->flddph, n));
-    TRY(put<R>(h, S.fldfrc, st->fldfrc, n));
-    TRY(put<R>(h, S.fldare, st->fldare, n));
-    TRY(put<R>(h, S.sfcelv, st->sfcelv, n));
-    TRY(put<R>(h, S.storge, st->storge, n));
-    TRY(put<R>(h, S.outflw, st->outflw, n));
-    TRY(put<R>(h, S.rivvel, st->rivvel, n));
-    if (h->o.llevee) TRY(put<R>(h, S.levdph, st->levdph, n));
-    TRY(putd(S.rivinf, st->rivinf, n));
-    TRY(putd(S.fldinf, st->fldinf, n));
-    TRY(putd(S.pthout, st->pthout, n));
+    TRY(put<R>(h, S.rivdph, st->rivdph, K_CELL));
+    TRY(put<R>(h, S.flddph, st->flddph, K_CELL));
+    TRY(put<R>(h, S.fldfrc, st->fldfrc, K_CELL));
+    TRY(put<R>(h, S.fldare, st->fldare, K_CELL));
+    TRY(put<R>(h, S.sfcelv, st->sfcelv, K_CELL));
+    TRY(put<R>(h, S.storge, st->storge, K_CELL));
+    TRY(put<R>(h, S.outflw, st->outflw, K_CELL));
+    TRY(put<R>(h, S.rivvel, st->rivvel, K_CELL));
+    if (h->o.llevee) TRY(put<R>(h, S.levdph, st->levdph, K_CELL));
+    TRY(putd(S.rivinf, st->rivinf));
+    TRY(putd(S.fldinf, st->fldinf));
+    TRY(putd(S.pthout, st->pthout));
     return 0;
 }
 
 template <class R>
 int get_state_t(cmf_hip_ctx* h, cmf_state* st) {
     DevState<R>& S = state_of<R>(h);
-    const size_t n = (size_t)h->d.nseqall;
-    const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
-    const size_t npl = np * (h->o.lpthout ? (size_t)h->d.npthlev : 0);
-    auto getd = [&](double* dst, const double* src, size_t m) -> int { return get<double>(h, dst, src, m); };
+    auto getd = [&](double* dst, const double* src) -> int { return get<double>(h, dst, src, K_CELL); };
     HIPCHK(hipStreamSynchronize(h->stream));
-    TRY(getd(st->rivsto, S.rivsto, n));
-    TRY(getd(st->fldsto, S.fldsto, n));
-    if (h->o.llevee) TRY(getd(st->levsto, S.levsto, n));
-    if (h->o.ldamout) { TRY(getd(st->damsto, S.damsto, n)); TRY(getd(st->daminf, S.daminf, n)); }
-    TRY(get<R>(h, st->rivout, h->vel_pending ? (const R*)h->vel_buf : S.rivout, n));
-    TRY(get<R>(h, st->rivout_pre, S.rivout, n));
-    TRY(get<R>(h, st->fldout, S.fldout, n));
-    TRY(get<R>(h, st->fldout_pre, S.fldout, n));
-    TRY(get<R>(h, st->rivdph_pre, S.rivdph_pre, n));
-    TRY(get<R>(h, st->fldsto_pre, S.fldsto_pre, n));
+    TRY(getd(st->rivsto, S.rivsto));
+    TRY(getd(st->fldsto, S.fldsto));
+    if (h->o.llevee) TRY(getd(st->levsto, S.levsto));
+    if (h->o.ldamout) { TRY(getd(st->damsto, S.damsto)); TRY(getd(st->daminf, S.daminf)); }
+    TRY(get<R>(h, st->rivout, h->vel_pending ? (const R*)h->vel_buf : S.rivout, K_CELL));
+    TRY(get<R>(h, st->rivout_pre, S.rivout, K_CELL));
+    TRY(get<R>(h, st->fldout, S.fldout, K_CELL));
+    TRY(get<R>(h, st->fldout_pre, S.fldout, K_CELL));
+    TRY(get<R>(h, st->rivdph_pre, S.rivdph_pre, K_CELL));
+    TRY(get<R>(h, st->fldsto_pre, S.fldsto_pre, K_CELL));
     if (h->o.lpthout) {
-        TRY(get<R>(h, st->pthflw, S.pthflw, npl, false));
-        TRY(get<R>(h, st->pthflw_pre, S.pthflw, npl, false));
-        TRY(get<R>(h, st->pthflwsum, S.pthflwsum, np, false));
-        TRY(get<R>(h, st->pthflw_avg, S.pthflw_avg, npl, false));
-        TRY(get<R>(h, st->pthflwsum_avg, S.pthflwsum_avg, np, false));
+        TRY(get<R>(h, st->pthflw, S.pthflw, K_PTHL));
+        TRY(get<R>(h, st->pthflw_pre, S.pthflw, K_PTHL));
+        TRY(get<R>(h, st->pthflwsum, S.pthflwsum, K_PTH));
+        TRY(get<R>(h, st->pthflw_avg, S.pthflw_avg, K_PTHL));
+        TRY(get<R>(h, st->pthflwsum_avg, S.pthflwsum_avg, K_PTH));
     }
-    TRY(get<R>(h, st->rivdph, S.rivdph, n));
-    TRY(get<R>(h, st->flddph, S.flddph, n));
-    TRY(get<R>(h, st->fldfrc, S.fldfrc, n));
-    TRY(get<R>(h, st->fldare, S.fldare, n));
-    TRY(get<R>(h, st->sfcelv, S.sfcelv, n));
-    TRY(get<R>(h, st->storge, S.storge, n));
-    TRY(get<R>(h, st->outflw, S.outflw, n));
-    TRY(get<R>(h, st->rivvel, S.rivvel, n));
-    if (h->o.llevee) TRY(get<R>(h, st->levdph, S.levdph, n));
-    TRY(getd(st->rivinf, S.rivinf, n));
-    TRY(getd(st->fldinf, S.fldinf, n));
-    TRY(getd(st->pthout, S.pthout, n));
-    TRY(get<R>(h, st->rivout_avg, S.rivout_avg, n));
-    TRY(get<R>(h, st->fldout_avg, S.fldout_avg, n));
-    TRY(get<R>(h, st->outflw_avg, S.outflw_avg, n));
-    TRY(get<R>(h, st->rivvel_avg, S.rivvel_avg, n));
-    TRY(get<R>(h, st->pthout_avg, S.pthout_avg, n));
-    TRY(get<R>(h, st->runoff_avg, S.runoff_avg, n));
-    TRY(get<R>(h, st->outflw_max, S.outflw_max, n));
-    TRY(get<R>(h, st->rivdph_max, S.rivdph_max, n));
-    TRY(get<R>(h, st->storge_max, S.storge_max, n));
-    if (h->o.ldamout) TRY(get<R>(h, st->daminf_avg, S.daminf_avg, n));
+    TRY(get<R>(h, st->rivdph, S.rivdph, K_CELL));
+    TRY(get<R>(h, st->flddph, S.flddph, K_CELL));
+    TRY(get<R>(h, st->fldfrc, S.fldfrc, K_CELL));
+    TRY(get<R>(h, st->fldare, S.fldare, K_CELL));
+    TRY(get<R>(h, st->sfcelv, S.sfcelv, K_CELL));
+    TRY(get<R>(h, st->storge, S.storge, K_CELL));
+    TRY(get<R>(h, st->outflw, S.outflw, K_CELL));
+    TRY(get<R>(h, st->rivvel, S.rivvel, K_CELL));
+    if (h->o.llevee) TRY(get<R>(h, st->levdph, S.levdph, K_CELL));
+    TRY(getd(st->rivinf, S.rivinf));
+    TRY(getd(st->fldinf, S.fldinf));
+    TRY(getd(st->pthout, S.pthout));
+    TRY(get<R>(h, st->rivout_avg, S.rivout_avg, K_CELL));
+    TRY(get<R>(h, st->fldout_avg, S.fldout_avg, K_CELL));
+    TRY(get<R>(h, st->outflw_avg, S.outflw_avg, K_CELL));
+    TRY(get<R>(h, st->rivvel_avg, S.rivvel_avg, K_CELL));
+    TRY(get<R>(h, st->pthout_avg, S.pthout_avg, K_CELL));
+    TRY(get<R>(h, st->runoff_avg, S.runoff_avg, K_CELL));
+    TRY(get<R>(h, st->outflw_max, S.outflw_max, K_CELL));
+    TRY(get<R>(h, st->rivdph_max, S.rivdph_max, K_CELL));
+    TRY(get<R>(h, st->storge_max, S.storge_max, K_CELL));
+    if (h->o.ldamout) TRY(get<R>(h, st->daminf_avg, S.daminf_avg, K_CELL));
     return 0;
 }
 
@@ -363,6 +410,8 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.pgrv = h->o.pgrv; A.pmanfld = h->o.pmanfld; A.pdstmth = h->o.pdstmth;
     A.pcadp = h->o.pcadp; A.pminslp = h->o.pminslp;
     A.lfldout = h->o.lfldout; A.lpthout = h->o.lpthout; A.ldamout = h->o.ldamout; A.llevee = h->o.llevee;
+    A.budget = h->ninst == 1;
+    A.xcd = h->tune.xcd;
     return A;
 }
 
@@ -376,87 +425,303 @@ int ev_record(cmf_hip_ctx* h, size_t k) {
     return 0;
 }
 
+bool valid_bs(int bs) { return bs == 64 || bs == 128 || bs == 256 || bs == 512; }
+
 template <class R>
-int advance_t(cmf_hip_ctx* h, double* dt, int32_t nt_cap, int32_t* nt_out, double* glb, int32_t glb_cap) {
-    DevState<R>& S = state_of<R>(h);
-    const double dt_def = *dt;
-    StepArgs A = base_args(h);
-    A.dt = dt_def;
-    A.gdt = A.pgrv * dt_def;
-    size_t ne = 0;
-    if (h->timing) TRY(ev_record(h, ne++));
-    // outer-step start: CMF_PHYSICS_FLDSTG (physics:54) and CALC_ADPSTP (:58)
-    launch_stage<R>(h->M, S, A, h->o.ladpstp, dt_def, h->stream);
-    HIPCHK(hipGetLastError());
-    if (h->timing) TRY(ev_record(h, ne++));
-    int32_t nt = 1;
-    double dts = dt_def;
-    if (h->o.ladpstp) {
-        HIPCHK(hipMemcpyAsync(h->h_ctl, S.ctl, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        const double dt_min = h->h_ctl[0];
-        nt = (int32_t)(dt_def * (1.0 / dt_min) - K_0P01F) + 1;       // physics:205
-        dts = dt_def * (double)(1.0f / (float)nt);                   // physics:206
-    }
-    const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
-    if (nrun > h->hist_cap) {                           // grow the per-substep budget buffers
-        const int cap = nrun + nrun / 4;
-        double *p = nullptr, *q = nullptr;
-        TRY(dalloc(h, &p, (size_t)cap * NGLB));
-        TRY(dalloc(h, &q, (size_t)cap * h->nb_cells * NGLB));
-        (void)hipFree(S.glb_hist);
-        (void)hipFree(S.partials);
-        for (auto& a : h->allocs) if (a == (void*)S.glb_hist || a == (void*)S.partials) a = nullptr;
-        S.glb_hist = p;
+int coop_capacity(cmf_hip_ctx* h, int bs, int* out) {
+    int per_cu = 0, cus = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, substep_kernel<R>(bs), bs, 0));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    *out = per_cu * cus;
+    return 0;
+}
+
+// block size of the update phase (its tiles carry the budget partial sums)
+int upd_bs(const cmf_hip_ctx* h) { return h->tune.fused ? h->tune.bs_fused : h->tune.bs_update; }
+
+template <class R>
+int grow_partials(cmf_hip_ctx* h, DevState<R>& S, int nrun) {
+    const size_t need = (size_t)nrun * tiles_cells(h->M, upd_bs(h)) * NGLB;
+    if (need > h->partial_cap) {
+        const size_t cap = need + need / 4;
+        double* q = nullptr;
+        TRY(dalloc(h, &q, cap));
+        if (S.partials) {
+            (void)hipFree(S.partials);
+            for (auto& a : h->allocs) if (a == (void*)S.partials) a = nullptr;
+        }
         S.partials = q;
+        h->partial_cap = cap;
+    }
+    if (nrun > h->hist_cap) {
+        const int cap = nrun + nrun / 4;
+        double* p = nullptr;
+        TRY(dalloc(h, &p, (size_t)cap * NGLB));
+        (void)hipFree(S.glb_hist);
+        for (auto& a : h->allocs) if (a == (void*)S.glb_hist) a = nullptr;
+        S.glb_hist = p;
         h->hist_cap = cap;
     }
-    double nadd = 0.0;                                               // diag:108
-    for (int32_t it = 0; it < nrun; ++it) nadd = nadd + dts;
-    A.dt = dts;
-    A.gdt = A.pgrv * dts;
-    A.nadd = nadd;
-    A.nrun = nrun;
-    for (int32_t it = 0; it < nrun; ++it) {
-        A.first = (it == 0);
-        A.last = (it == nrun - 1);
+    return 0;
+}
+
+// the kernels of substep `it` on the stream (no host synchronisation)
+template <class R>
+int enqueue_substep(cmf_hip_ctx* h, const DevState<R>& Sk, StepArgs A, size_t* ne) {
+    if (h->tune.fused) {
+        HIPCHK(launch_substep<R>(h->M, Sk, A, h->tune.bs_fused, h->tune.grid_fused, h->stream));
+        if (h->timing) TRY(ev_record(h, (*ne)++));
+    } else {
+        launch_phase<R>(h->M, Sk, A, 0, h->tune.bs_outflow, h->stream);
+        if (h->timing) TRY(ev_record(h, (*ne)++));
+        launch_phase<R>(h->M, Sk, A, 1, h->tune.bs_rate, h->stream);
+        if (h->timing) TRY(ev_record(h, (*ne)++));
+        launch_phase<R>(h->M, Sk, A, 2, h->tune.bs_update, h->stream);
+        if (h->timing) TRY(ev_record(h, (*ne)++));
+    }
+    return 0;
+}
+
+// NT substeps with the RIVDPH buffer pair alternating; returns the final pair
+template <class R>
+int enqueue_substeps(cmf_hip_ctx* h, DevState<R>& S, int32_t ntmax, bool vel_first, size_t* ne) {
+    StepArgs A = base_args(h);
+    for (int32_t it = 0; it < ntmax; ++it) {
         A.it = it;
         DevState<R> Sk = S;
         Sk.rivdph_next = S.rivdph_pre;
-        Sk.rivout_vel = (h->vel_pending && it == 0) ? (const R*)h->vel_buf : S.rivout;
-        if (h->timing) TRY(ev_record(h, ne++));
-        launch_outflow<R>(h->M, Sk, A, h->stream);
-        if (h->timing) { TRY(ev_record(h, ne++)); TRY(ev_record(h, ne++)); }
-        launch_rate<R>(h->M, Sk, A, h->stream);
-        if (h->timing) { TRY(ev_record(h, ne++)); TRY(ev_record(h, ne++)); }
-        launch_update<R>(h->M, Sk, A, h->stream);
-        if (h->timing) TRY(ev_record(h, ne++));
+        Sk.rivout_vel = (vel_first && it == 0) ? (const R*)h->vel_buf : S.rivout;
+        TRY(enqueue_substep<R>(h, Sk, A, ne));
         std::swap(S.rivdph, S.rivdph_pre);                            // CALC_VARS_PRE for RIVDPH
     }
-    launch_budget(h->M, S.partials, nrun, S.glb_hist, h->stream);    // CALC_WATBAL sums, all substeps
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <class R>
+int run_substeps_graph(cmf_hip_ctx* h, DevState<R>& S, int32_t ntmax) {
+    const int parity = S.rivdph == h->dph_a ? 0 : 1;
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : h->graphs) if (g.nt == ntmax && g.parity == parity) exec = g.exec;
+    if (!exec) {
+        DevState<R> S0 = S;
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        size_t ne = 0;
+        const int rc = enqueue_substeps<R>(h, S0, ntmax, false, &ne);
+        const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+        if (rc != 0) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (ec != hipSuccess) return fail(CMF_EHIP, "substep graph capture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(CMF_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        if (h->graphs.size() >= 32) {
+            (void)hipGraphExecDestroy(h->graphs.front().exec);
+            h->graphs.erase(h->graphs.begin());
+        }
+        h->graphs.push_back({ntmax, parity, exec});
+    }
+    HIPCHK(hipGraphLaunch(exec, h->stream));
+    for (int32_t it = 0; it < ntmax; ++it) std::swap(S.rivdph, S.rivdph_pre);
+    return 0;
+}
+
+// One outer step of every instance. nt_cap > 0 caps the substeps (timing).
+template <class R>
+int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, double* dt_out, double* glb,
+              int32_t glb_cap) {
+    DevState<R>& S = state_of<R>(h);
+    const int ni = h->ninst;
+    StepArgs A = base_args(h);
+    size_t ne = 0;
+    if (h->timing) TRY(ev_record(h, ne++));
+    // outer-step start: CMF_PHYSICS_FLDSTG (physics:54) and CALC_ADPSTP (:58)
+    launch_stage<R>(h->M, S, A, h->o.ladpstp, ni, dt_def, h->stream);
+    HIPCHK(hipGetLastError());
+    if (h->timing) TRY(ev_record(h, ne++));
+    if (h->o.ladpstp) {
+        HIPCHK(hipMemcpyAsync(h->h_cfl, S.cfl, (size_t)ni * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    int32_t ntmax = 0;
+    for (int k = 0; k < ni; ++k) {
+        int32_t nt = 1;
+        double dts = dt_def;
+        if (h->o.ladpstp) {
+            double dt_min;
+            std::memcpy(&dt_min, &h->h_cfl[k], sizeof dt_min);
+            nt = (int32_t)(dt_def * (1.0 / dt_min) - K_0P01F) + 1;       // physics:205
+            dts = dt_def * (double)(1.0f / (float)nt);                   // physics:206
+        }
+        h->last_nt[(size_t)k] = nt;
+        h->last_dt[(size_t)k] = dts;
+        const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
+        double nadd = 0.0;                                               // diag:108
+        for (int32_t it = 0; it < nrun; ++it) nadd = nadd + dts;
+        h->h_inst[k] = InstStep{dts, h->o.pgrv * dts, nadd, nrun, 0};
+        if (nrun > ntmax) ntmax = nrun;
+    }
+    bool anyfix = false;
+    for (int k = 0; k < ni; ++k) {
+        h->h_inst[k].swapfix = (ntmax - h->h_inst[k].nt) & 1;
+        anyfix = anyfix || h->h_inst[k].swapfix;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_inst, h->h_inst, (size_t)ni * sizeof(InstStep), hipMemcpyHostToDevice, h->stream));
+    if (A.budget) TRY(grow_partials<R>(h, S, ntmax));
+    const bool vel_first = h->vel_pending;
+    if (h->tune.graph && !h->tune.fused && !h->timing && !vel_first) {
+        TRY(run_substeps_graph<R>(h, S, ntmax));
+    } else {
+        TRY(enqueue_substeps<R>(h, S, ntmax, vel_first, &ne));
+    }
+    if (anyfix) launch_swapfix<R>(h->M, S, h->stream);
+    if (A.budget)                                                    // CALC_WATBAL sums, all substeps
+        launch_budget(S.partials, tiles_cells(h->M, upd_bs(h)), ntmax, S.glb_hist, h->stream);
     HIPCHK(hipGetLastError());
     h->vel_pending = false;
-    if (glb && glb_cap > 0) {
-        const int32_t m = nrun < glb_cap ? nrun : glb_cap;
+    if (A.budget && glb && glb_cap > 0) {
+        const int32_t m = ntmax < glb_cap ? ntmax : glb_cap;
         HIPCHK(hipMemcpyAsync(glb, S.glb_hist, (size_t)m * NGLB * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     }
+    if (h->tune.fused)
+        HIPCHK(hipMemcpyAsync(h->h_ctl, S.err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->tune.fused && *reinterpret_cast<int32_t*>(h->h_ctl) != 0)
+        return fail(CMF_EHIP, "grid barrier of the fused substep kernel timed out");
     if (h->timing) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
         h->t_ms[3] += ms;
-        for (int32_t it = 0; it < nrun; ++it) {
-            for (int k = 0; k < 3; ++k) {
-                const size_t e0 = 2 + 6 * (size_t)it + 2 * k;
-                HIPCHK(hipEventElapsedTime(&ms, h->ev[e0], h->ev[e0 + 1]));
-                h->t_ms[k] += ms;
+        const int per = h->tune.fused ? 1 : 3;
+        for (int32_t it = 0; it < ntmax; ++it) {
+            for (int k = 0; k < per; ++k) {
+                const size_t e1 = 2 + (size_t)per * it + k;
+                HIPCHK(hipEventElapsedTime(&ms, h->ev[e1 - 1], h->ev[e1]));
+                h->t_ms[h->tune.fused ? 2 : k] += ms;
             }
         }
-        h->t_ms[4] += nrun;
+        h->t_ms[4] += ntmax;
+        h->inst_ms.assign((size_t)ni, 0.0);
+        for (int k = 0; k < ni; ++k) {
+            const size_t e1 = 1 + (size_t)per * h->h_inst[k].nt;
+            HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[e1]));
+            h->inst_ms[(size_t)k] = ms;
+        }
     }
-    *nt_out = nt;
-    *dt = dt_def;                                                    // physics:145
+    if (nt_out) for (int k = 0; k < ni; ++k) nt_out[k] = h->last_nt[(size_t)k];
+    if (dt_out) for (int k = 0; k < ni; ++k) dt_out[k] = h->last_dt[(size_t)k];
     return 0;
+}
+
+// ---- tuner: snapshot / restore of the per-slot state
+int snapshot(cmf_hip_ctx* h, bool restore) {
+    size_t total = 0;
+    for (auto& a : h->state_arrs) total += (a.bytes + 255) & ~(size_t)255;
+    if (!h->snapshot) {
+        void* q = nullptr;
+        HIPCHK(hipMalloc(&q, total));
+        h->snapshot = q;
+        h->snapshot_bytes = total;
+    }
+    size_t off = 0;
+    for (auto& a : h->state_arrs) {
+        char* snap = static_cast<char*>(h->snapshot) + off;
+        if (restore) HIPCHK(hipMemcpyAsync(*a.slot, snap, a.bytes, hipMemcpyDeviceToDevice, h->stream));
+        else HIPCHK(hipMemcpyAsync(snap, *a.slot, a.bytes, hipMemcpyDeviceToDevice, h->stream));
+        off += (a.bytes + 255) & ~(size_t)255;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+template <class R>
+int autotune_t(cmf_hip_ctx* h, double dt_def, int32_t nsub, cmf_tune* best, double* report, int32_t report_cap) {
+    DevState<R>& S = state_of<R>(h);
+    const cmf_tune keep = h->tune;
+    const bool keep_timing = h->timing;
+    const bool keep_vel = h->vel_pending;
+    // an advance moves only the RIVDPH buffer pair among the state pointers;
+    // scratch pointers (budget partials, history) may be reallocated by a trial
+    // and must keep their new values
+    R* const keep_dph = S.rivdph;
+    R* const keep_dph_pre = S.rivdph_pre;
+    auto restore_ptrs = [&]() { S.rivdph = keep_dph; S.rivdph_pre = keep_dph_pre; };
+    double keep_t[5];
+    std::memcpy(keep_t, h->t_ms, sizeof keep_t);
+    TRY(snapshot(h, false));
+    int nrep = 0;
+    auto trial = [&](const cmf_tune& t, double* per_phase, double* total) -> int {
+        h->tune = t;
+        h->timing = true;
+        for (int k = 0; k < 5; ++k) h->t_ms[k] = 0.0;
+        int32_t nt;
+        double dto;
+        // one untimed outer step warms the code, then the timed one
+        restore_ptrs(); h->vel_pending = keep_vel;
+        TRY(snapshot(h, true));
+        TRY(advance_t<R>(h, dt_def, nsub, nullptr, nullptr, nullptr, 0));
+        (void)nt; (void)dto;
+        for (int k = 0; k < 5; ++k) h->t_ms[k] = 0.0;
+        restore_ptrs(); h->vel_pending = keep_vel;
+        TRY(snapshot(h, true));
+        TRY(advance_t<R>(h, dt_def, nsub, nullptr, nullptr, nullptr, 0));
+        for (int k = 0; k < 3; ++k) per_phase[k] = h->t_ms[k];
+        *total = h->t_ms[0] + h->t_ms[1] + h->t_ms[2];
+        if (report && nrep < report_cap) {
+            double* r = report + (size_t)nrep * 8;
+            r[0] = t.fused; r[1] = t.fused ? t.bs_fused : t.bs_outflow; r[2] = t.xcd; r[3] = t.grid_fused;
+            r[4] = per_phase[0]; r[5] = per_phase[1]; r[6] = per_phase[2]; r[7] = *total;
+        }
+        ++nrep;
+        return 0;
+    };
+    cmf_tune bt = keep;
+    double best_total = 1e300;
+    for (int xcd = 0; xcd < 2; ++xcd) {
+        // per-phase block sizes: every phase is its own kernel, so each takes its best
+        double bph[3] = {1e300, 1e300, 1e300};
+        int bbs[3] = {256, 256, 256};
+        for (int bs : TUNE_BS) {
+            cmf_tune t{};
+            t.bs_outflow = t.bs_rate = t.bs_update = bs;
+            t.bs_fused = 256; t.xcd = xcd; t.graph = 0; t.fused = 0;
+            double ph[3], tot;
+            TRY(trial(t, ph, &tot));
+            for (int k = 0; k < 3; ++k) if (ph[k] < bph[k]) { bph[k] = ph[k]; bbs[k] = bs; }
+        }
+        const double tot = bph[0] + bph[1] + bph[2];
+        if (tot < best_total) {
+            best_total = tot;
+            bt = cmf_tune{};
+            bt.bs_outflow = bbs[0]; bt.bs_rate = bbs[1]; bt.bs_update = bbs[2];
+            bt.bs_fused = 256; bt.xcd = xcd; bt.fused = 0; bt.graph = keep.graph;
+        }
+        for (int bs : {256, 512}) {
+            cmf_tune t{};
+            t.bs_outflow = t.bs_rate = t.bs_update = 256;
+            t.fused = 1; t.bs_fused = bs; t.xcd = xcd;
+            int cap = 0;
+            TRY(coop_capacity<R>(h, bs, &cap));
+            if (cap <= 0) continue;
+            t.grid_fused = cap - cap % 8;
+            if (t.grid_fused <= 0) t.grid_fused = cap;
+            double ph[3], totf;
+            TRY(trial(t, ph, &totf));
+            if (totf < best_total) { best_total = totf; bt = t; bt.graph = 0; }
+        }
+    }
+    // restore the state and the caller's settings; keep the best configuration
+    restore_ptrs();
+    TRY(snapshot(h, true));
+    h->vel_pending = keep_vel;
+    h->timing = keep_timing;
+    std::memcpy(h->t_ms, keep_t, sizeof keep_t);
+    h->tune = bt;
+    if (best) *best = bt;
+    (void)hipFree(h->snapshot);
+    h->snapshot = nullptr;
+    return nrep;
 }
 
 }  // namespace
@@ -466,7 +731,8 @@ extern "C" {
 
 const char* cmf_hip_last_error(void) { return g_err.c_str(); }
 
-int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* out) {
+int cmf_hip_create_ensemble(const cmf_dims* dims, const cmf_options* opts, const cmf_ensemble* ens,
+                            cmf_handle* out) {
     if (!dims || !opts || !out) return fail(CMF_EINVAL, "NULL argument");
     *out = nullptr;
     if (dims->nseqall <= 0 || dims->nseqriv < 0 || dims->nseqriv > dims->nseqall)
@@ -477,21 +743,45 @@ int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* ou
     if (opts->ldamout && dims->ndam <= 0) return fail(CMF_EINVAL, "LDAMOUT needs NDAM>0");
     if (opts->precision != CMF_PREC_FP64 && opts->precision != CMF_PREC_MIXED)
         return fail(CMF_EINVAL, "unknown precision %d", opts->precision);
+    int ndom = 1, nmem = 1;
+    std::vector<int32_t> off{0, dims->nseqall};
+    if (ens) {
+        ndom = ens->ndom;
+        nmem = ens->members;
+        if (ndom <= 0 || !ens->dom_cells) return fail(CMF_EINVAL, "ensemble needs ndom>0 and dom_cells");
+        if (nmem <= 0 || nmem > NMEM_MAX || (nmem & (nmem - 1)))
+            return fail(CMF_EINVAL, "members=%d must be a power of two in 1..%d", nmem, NMEM_MAX);
+        off.assign(1, 0);
+        long long tot = 0;
+        for (int d = 0; d < ndom; ++d) {
+            if (ens->dom_cells[d] <= 0) return fail(CMF_EINVAL, "domain %d has %d cells", d + 1, ens->dom_cells[d]);
+            tot += ens->dom_cells[d];
+            off.push_back((int32_t)tot);
+        }
+        if (tot != dims->nseqall) return fail(CMF_EINVAL, "domain cells sum to %lld, NSEQALL=%d", tot, dims->nseqall);
+    }
+    if ((long long)dims->nseqall * nmem > 0x7fffffffLL) return fail(CMF_EINVAL, "NSEQALL*members exceeds 2^31");
     cmf_hip_ctx* h = new cmf_hip_ctx();
     h->d = *dims;
     h->o = *opts;
-    h->nb_cells = (dims->nseqall + BLOCK - 1) / BLOCK;
-    int rc = 0;
+    h->ndom = ndom; h->nmem = nmem; h->ninst = ndom * nmem;
+    while ((1 << h->lgm) < nmem) ++h->lgm;
+    h->dom_off = off;
+    h->last_nt.assign((size_t)h->ninst, 0);
+    h->last_dt.assign((size_t)h->ninst, 0.0);
+    h->tune = cmf_tune{256, 256, 256, 0, 256, 0, 0, 0};
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return fail(CMF_EHIP, "no HIP device / stream");
     }
-    if (hipHostMalloc((void**)&h->h_ctl, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-        (void)hipStreamDestroy(h->stream);
-        delete h;
+    if (hipHostMalloc((void**)&h->h_ctl, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->h_inst, (size_t)h->ninst * sizeof(InstStep), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h->h_cfl, (size_t)h->ninst * sizeof(unsigned long long), hipHostMallocDefault) !=
+            hipSuccess) {
+        cmf_hip_destroy(h);
         return fail(CMF_EHIP, "hipHostMalloc failed");
     }
-    rc = (opts->precision == CMF_PREC_FP64) ? alloc_state<double>(h, h->S64) : alloc_state<float>(h, h->S32);
+    const int rc = (opts->precision == CMF_PREC_FP64) ? alloc_state<double>(h, h->S64) : alloc_state<float>(h, h->S32);
     if (rc != 0) {
         std::string keep = g_err;
         cmf_hip_destroy(h);
@@ -502,12 +792,20 @@ int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* ou
     return 0;
 }
 
+int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* out) {
+    return cmf_hip_create_ensemble(dims, opts, nullptr, out);
+}
+
 void cmf_hip_destroy(cmf_handle h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
     for (void* p : h->allocs) if (p) (void)hipFree(p);
+    if (h->snapshot) (void)hipFree(h->snapshot);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->h_ctl) (void)hipHostFree(h->h_ctl);
+    if (h->h_inst) (void)hipHostFree(h->h_inst);
+    if (h->h_cfl) (void)hipHostFree(h->h_cfl);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -515,45 +813,75 @@ void cmf_hip_destroy(cmf_handle h) {
 int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     if (!h || !mp) return fail(CMF_EINVAL, "NULL argument");
     if (h->map_set) return fail(CMF_ESTATE, "map already set for this handle");
-    const int n = h->d.nseqall, nr = h->d.nseqriv, nl = h->d.nlfp;
+    const int n = h->d.nseqall, nl = h->d.nlfp;
     if (!mp->i1next || !mp->rivstomax || !mp->rivelv || !mp->dwnelv || !mp->fldhgt || !mp->fldstomax ||
         !mp->fldgrd || !mp->i2mask)
         return fail(CMF_EINVAL, "required map array missing");
     DevMap& M = h->M;
-    M.n = n; M.nseqriv = nr; M.nlfp = nl;
+    M.n = n; M.nseqriv = h->d.nseqriv; M.nlfp = nl;
     M.lfplain = h->o.lfplain;
     M.dfrcinc = mp->dfrcinc;
-    // topology: I1NEXT -> 0-based, upstream CSR in ascending source order
+    M.lgm = h->lgm; M.nmem = h->nmem;
+    M.ns = n * h->nmem;
+    // domain of every NSEQ cell
+    std::vector<int32_t> dom((size_t)n, 0);
+    for (int d = 0; d < h->ndom; ++d)
+        for (int i = h->dom_off[(size_t)d]; i < h->dom_off[(size_t)d + 1]; ++i) dom[(size_t)i] = d;
+    // topology: I1NEXT -> 0-based (negative codes = mouths), upstream CSR in
+    // ascending source order
     std::vector<int32_t> nxt(n);
+    int nriv = 0;
     for (int i = 0; i < n; ++i) {
         const int32_t v = mp->i1next[i];
-        if (i < nr) {
-            if (v < 1 || v > n) return fail(CMF_EMAP, "I1NEXT(%d)=%d outside 1..NSEQALL for a river cell", i + 1, v);
+        if (v > 0) {
+            if (v > n) return fail(CMF_EMAP, "I1NEXT(%d)=%d outside 1..NSEQALL", i + 1, v);
+            if (dom[(size_t)v - 1] != dom[(size_t)i])
+                return fail(CMF_EMAP, "I1NEXT(%d)=%d leaves the cell's domain", i + 1, v);
             nxt[i] = v - 1;
-        } else {
-            if (v >= 0) return fail(CMF_EMAP, "I1NEXT(%d)=%d: mouth cells need a negative code", i + 1, v);
+            ++nriv;
+        } else if (v < 0) {
             nxt[i] = v;
+        } else {
+            return fail(CMF_EMAP, "I1NEXT(%d)=0: river cells need a downstream cell, mouths a negative code", i + 1);
         }
     }
-    std::vector<int32_t> owner(nr), item(nr), up_ptr, up_idx;
-    for (int i = 0; i < nr; ++i) { owner[i] = nxt[i]; item[i] = i; }
+    if (nriv != h->d.nseqriv) return fail(CMF_EMAP, "%d cells have a downstream cell, NSEQRIV=%d", nriv, h->d.nseqriv);
+    std::vector<int32_t> owner, item, up_ptr, up_idx;
+    for (int i = 0; i < n; ++i) if (nxt[i] >= 0) { owner.push_back(nxt[i]); item.push_back(i); }
     csr(n, owner, up_ptr, up_idx, item);
     // device cell order
     const char* ord = getenv("CMF_HIP_ORDER");
     h->permuted = !(ord && std::string(ord) == "nseq");
-    if (h->permuted && !dfs_order(n, nr, up_ptr, up_idx, h->perm))
+    if (h->permuted && !dfs_order(n, nxt, up_ptr, up_idx, h->perm))
         return fail(CMF_EMAP, "river network is not a forest draining to the mouth cells");
     if (!h->permuted) { h->perm.resize((size_t)n); for (int i = 0; i < n; ++i) h->perm[i] = i; }
     h->iperm.assign((size_t)n, 0);
     for (int p = 0; p < n; ++p) h->iperm[(size_t)h->perm[p]] = p;
     const std::vector<int32_t>& P = h->perm;
     const std::vector<int32_t>& IP = h->iperm;
+    // host index of every device slot
+    {
+        const int nm = h->nmem;
+        h->hx_cell.resize((size_t)n * nm);
+        for (int p = 0; p < n; ++p)
+            for (int m = 0; m < nm; ++m) h->hx_cell[(size_t)p * nm + m] = (int64_t)m * n + P[(size_t)p];
+        const int np = h->o.lpthout ? h->d.npthout : 0, nlv = h->o.lpthout ? h->d.npthlev : 0;
+        h->hx_pth.resize((size_t)np * nm);
+        h->hx_pthl.resize((size_t)np * nm * nlv);
+        for (int q = 0; q < np; ++q)
+            for (int m = 0; m < nm; ++m) {
+                h->hx_pth[(size_t)q * nm + m] = (int64_t)m * np + q;
+                for (int l = 0; l < nlv; ++l)
+                    h->hx_pthl[(size_t)l * np * nm + (size_t)q * nm + m] = (int64_t)m * np * nlv + (int64_t)l * np + q;
+            }
+    }
     // topology in device positions; each upstream list keeps ascending NSEQ order
-    std::vector<int32_t> dnext((size_t)n), dptr((size_t)n + 1, 0), didx(up_idx.size());
+    std::vector<int32_t> dnext((size_t)n), dptr((size_t)n + 1, 0), didx(up_idx.size()), ddom((size_t)n);
     for (int p = 0; p < n; ++p) {
         const int32_t v = nxt[(size_t)P[p]];
         dnext[(size_t)p] = v >= 0 ? IP[(size_t)v] : v;
         dptr[(size_t)p + 1] = dptr[p] + (up_ptr[(size_t)P[p] + 1] - up_ptr[(size_t)P[p]]);
+        ddom[(size_t)p] = dom[(size_t)P[p]];
     }
     for (int p = 0; p < n; ++p) {
         int32_t k = dptr[p];
@@ -563,6 +891,8 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     TRY(put_raw(h, &p32, dnext.data(), (size_t)n)); M.next = p32;
     TRY(put_raw(h, &p32, dptr.data(), dptr.size())); M.up_ptr = p32;
     TRY(put_raw(h, &p32, didx.data(), didx.size())); M.up_idx = p32;
+    M.cell_dom = nullptr;
+    if (h->ndom > 1) { TRY(put_raw(h, &p32, ddom.data(), (size_t)n)); M.cell_dom = p32; }
     TRY(validate_tables(h, mp));
     TRY(put_f32(h, &M.grarea, mp->grarea, n, "GRAREA"));
     TRY(put_f32(h, &M.elevtn, mp->elevtn, n, "ELEVTN"));
@@ -580,16 +910,23 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
         TRY(put_raw(h, &p32, m2.data(), (size_t)n)); M.i2mask = p32;
     }
     // bifurcation paths (maps:531-639)
+    M.pth_dom = nullptr;
+    M.nsp = 0;
     if (h->o.lpthout) {
         const int np = h->d.npthout, nlv = h->d.npthlev;
         M.npth = np; M.npthlev = nlv;
+        M.nsp = np * h->nmem;
         if (!mp->pth_upst || !mp->pth_down || !mp->pth_dst || !mp->pth_elv || !mp->pth_wth || !mp->pth_man)
             return fail(CMF_EINVAL, "LPTHOUT: path tables missing");
-        std::vector<int32_t> pu(np), pdn(np), oo, oi, io, ii;
+        std::vector<int32_t> pu(np), pdn(np), pdom(np, 0), oo, oi, io, ii;
         for (int p = 0; p < np; ++p) {
             pu[p] = mp->pth_upst[p] > 0 ? mp->pth_upst[p] - 1 : -1;
             pdn[p] = mp->pth_down[p] > 0 ? mp->pth_down[p] - 1 : -1;
             if (pu[p] >= n || pdn[p] >= n) return fail(CMF_EMAP, "path %d endpoint outside NSEQALL", p + 1);
+            if (pu[p] >= 0 && pdn[p] >= 0 && dom[(size_t)pu[p]] != dom[(size_t)pdn[p]])
+                return fail(CMF_EMAP, "path %d joins two domains", p + 1);
+            if (pu[p] >= 0) pdom[p] = dom[(size_t)pu[p]];
+            else if (pdn[p] >= 0) pdom[p] = dom[(size_t)pdn[p]];
             if (pu[p] >= 0) pu[p] = IP[(size_t)pu[p]];
             if (pdn[p] >= 0) pdn[p] = IP[(size_t)pdn[p]];
             if (pu[p] >= 0 && pdn[p] >= 0) {
@@ -606,6 +943,7 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
         TRY(put_raw(h, &p32, oidx.data(), oidx.size())); M.pout_idx = p32;
         TRY(put_raw(h, &p32, iptr.data(), iptr.size())); M.pinf_ptr = p32;
         TRY(put_raw(h, &p32, iidx.data(), iidx.size())); M.pinf_idx = p32;
+        if (h->ndom > 1) { TRY(put_raw(h, &p32, pdom.data(), (size_t)np)); M.pth_dom = p32; }
         TRY(put_raw(h, &pd, mp->pth_dst, (size_t)np)); M.pth_dst = pd;
         TRY(put_raw(h, &pd, mp->pth_elv, (size_t)np * nlv)); M.pth_elv = pd;
         TRY(put_raw(h, &pd, mp->pth_wth, (size_t)np * nlv)); M.pth_wth = pd;
@@ -663,27 +1001,47 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
 
 int cmf_hip_set_state(cmf_handle h, const cmf_state* st) {
     if (!h || !st) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     return h->o.precision == CMF_PREC_FP64 ? set_state_t<double>(h, st) : set_state_t<float>(h, st);
 }
 
 int cmf_hip_get_state(cmf_handle h, cmf_state* st) {
     if (!h || !st) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     return h->o.precision == CMF_PREC_FP64 ? get_state_t<double>(h, st) : get_state_t<float>(h, st);
 }
 
 int cmf_hip_set_runoff(cmf_handle h, const double* runoff) {
     if (!h || !runoff) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    return h->o.precision == CMF_PREC_FP64 ? put<double>(h, h->S64.runoff, runoff, K_CELL)
+                                           : put<float>(h, h->S32.runoff, runoff, K_CELL);
+}
+
+int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* factor) {
+    if (!h || !base || !factor) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     const size_t n = (size_t)h->d.nseqall;
-    return h->o.precision == CMF_PREC_FP64 ? put<double>(h, h->S64.runoff, runoff, n)
-                                           : put<float>(h, h->S32.runoff, runoff, n);
+    if (!h->d_rbase) {
+        TRY(dalloc(h, &h->d_rbase, n));
+        TRY(dalloc(h, &h->d_rfac, (size_t)h->ninst));
+    }
+    std::vector<double> b = to_dev(h, base, n);
+    HIPCHK(hipMemcpyAsync(h->d_rbase, b.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_rfac, factor, (size_t)h->ninst * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->o.precision == CMF_PREC_FP64) launch_scale_runoff<double>(h->M, h->S64, h->d_rbase, h->d_rfac, h->stream);
+    else launch_scale_runoff<float>(h->M, h->S32, h->d_rbase, h->d_rfac, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
 }
 
 int cmf_hip_physics_fldstg(cmf_handle h) {
     if (!h) return fail(CMF_EINVAL, "NULL handle");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     StepArgs A = base_args(h);
-    if (h->o.precision == CMF_PREC_FP64) launch_stage<double>(h->M, h->S64, A, 0, 0.0, h->stream);
-    else launch_stage<float>(h->M, h->S32, A, 0, 0.0, h->stream);
+    if (h->o.precision == CMF_PREC_FP64) launch_stage<double>(h->M, h->S64, A, 0, h->ninst, 0.0, h->stream);
+    else launch_stage<float>(h->M, h->S32, A, 0, h->ninst, 0.0, h->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
@@ -693,13 +1051,72 @@ int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap, int
                                    int32_t glb_cap) {
     if (!h || !dt || !nt_out) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->ninst != 1) return fail(CMF_EINVAL, "ensemble handle: use cmf_hip_ensemble_advance");
     if (!(*dt > 0.0)) return fail(CMF_EINVAL, "DT must be positive");
-    return h->o.precision == CMF_PREC_FP64 ? advance_t<double>(h, dt, nt_cap, nt_out, glb, glb_cap)
-                                           : advance_t<float>(h, dt, nt_cap, nt_out, glb, glb_cap);
+    const double dt_def = *dt;
+    const int rc = h->o.precision == CMF_PREC_FP64 ? advance_t<double>(h, dt_def, nt_cap, nt_out, nullptr, glb, glb_cap)
+                                                   : advance_t<float>(h, dt_def, nt_cap, nt_out, nullptr, glb, glb_cap);
+    *dt = dt_def;                                                    // physics:145
+    return rc;
 }
 
 int cmf_hip_physics_advance(cmf_handle h, double* dt, int32_t* nt_out, double* glb, int32_t glb_cap) {
     return cmf_hip_physics_advance_capped(h, dt, 0, nt_out, glb, glb_cap);
+}
+
+int cmf_hip_ensemble_advance(cmf_handle h, double dt, int32_t nt_cap, int32_t* nt_out, double* dt_out) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (!(dt > 0.0)) return fail(CMF_EINVAL, "DT must be positive");
+    return h->o.precision == CMF_PREC_FP64 ? advance_t<double>(h, dt, nt_cap, nt_out, dt_out, nullptr, 0)
+                                           : advance_t<float>(h, dt, nt_cap, nt_out, dt_out, nullptr, 0);
+}
+
+int cmf_hip_instance_times(cmf_handle h, double* ms) {
+    if (!h || !ms) return fail(CMF_EINVAL, "NULL argument");
+    if (h->inst_ms.size() != (size_t)h->ninst) return fail(CMF_ESTATE, "no timed advance yet");
+    for (int k = 0; k < h->ninst; ++k) ms[k] = h->inst_ms[(size_t)k];
+    return 0;
+}
+
+int cmf_hip_set_tune(cmf_handle h, const cmf_tune* t) {
+    if (!h || !t) return fail(CMF_EINVAL, "NULL argument");
+    if (!valid_bs(t->bs_outflow) || !valid_bs(t->bs_rate) || !valid_bs(t->bs_update) || !valid_bs(t->bs_fused))
+        return fail(CMF_EINVAL, "block sizes must be 64, 128, 256 or 512");
+    cmf_tune v = *t;
+    if (v.fused) {
+        int cap = 0;
+        const int rc = h->o.precision == CMF_PREC_FP64 ? coop_capacity<double>(h, v.bs_fused, &cap)
+                                                       : coop_capacity<float>(h, v.bs_fused, &cap);
+        if (rc != 0) return rc;
+        if (v.grid_fused <= 0) v.grid_fused = cap - cap % 8 > 0 ? cap - cap % 8 : cap;
+        if (v.grid_fused > cap)
+            return fail(CMF_EINVAL, "fused grid %d exceeds the %d co-resident blocks", v.grid_fused, cap);
+    }
+    if (v.graph != h->tune.graph || v.fused != h->tune.fused || v.bs_outflow != h->tune.bs_outflow ||
+        v.bs_rate != h->tune.bs_rate || v.bs_update != h->tune.bs_update || v.bs_fused != h->tune.bs_fused ||
+        v.grid_fused != h->tune.grid_fused || v.xcd != h->tune.xcd) {
+        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+        h->graphs.clear();
+    }
+    h->tune = v;
+    return 0;
+}
+
+int cmf_hip_get_tune(cmf_handle h, cmf_tune* t) {
+    if (!h || !t) return fail(CMF_EINVAL, "NULL argument");
+    *t = h->tune;
+    return 0;
+}
+
+int cmf_hip_autotune(cmf_handle h, double dt, int32_t nsub, cmf_tune* best, double* report, int32_t report_cap) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (!(dt > 0.0) || nsub <= 0) return fail(CMF_EINVAL, "autotune needs DT>0 and nsub>0");
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+    return h->o.precision == CMF_PREC_FP64 ? autotune_t<double>(h, dt, nsub, best, report, report_cap)
+                                           : autotune_t<float>(h, dt, nsub, best, report, report_cap);
 }
 
 int cmf_hip_synchronize(cmf_handle h) {

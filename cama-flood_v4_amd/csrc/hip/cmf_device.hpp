@@ -1,7 +1,15 @@
 // cmf_device.hpp -- device-side data layout and per-cell physics of the MI355X
 // CaMa-Flood substep. Shared by the kernels (cmf_kernels.hip) and the host glue.
 //
-// Layout in HBM (one structure of arrays per handle, NSEQ order):
+// Layout in HBM (one structure of arrays per handle):
+//  * cells: the domains of the handle concatenated, each domain in depth-first
+//    river order (or NSEQ order); static arrays are indexed by cell c.
+//  * members: a handle runs NMEM = 2^lgm members (runoff realisations) of every
+//    domain. Per-member state is member-minor: state slot s = (c << lgm) | m, so
+//    the members of a cell are adjacent, static map values are read once per
+//    cell for all its members, and the neighbour gathers (j << lgm) | m of a
+//    wavefront stay as coalesced as the cells themselves. Instance k = dom*NMEM+m
+//    has its own adaptive step (InstStep). A single model is NMEM=1, one domain.
 //  * static map: the seven float32 source maps (every reference map is read from
 //    a float32 file, so float32 holds them exactly) and the NLFP float32 profile
 //    planes; the derived tables (RIVSTOMAX, RIVELV, FLDSTOMAX, FLDGRD) are
@@ -35,8 +43,8 @@ constexpr int NGLB = 11;
 enum { G_STOPRE, G_STONXT, G_STONEW, G_RIVINF, G_RIVOUT,
        G_STOPRE2, G_STONEW2, G_RIVSTO, G_FLDSTO, G_FLDARE, G_LEVSTO };
 
-constexpr int BLOCK = 256;
 constexpr int NLFP_MAX = 16;
+constexpr int NMEM_MAX = 64;                   // members per domain (power of two)
 
 template <class T> __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
@@ -50,7 +58,11 @@ __device__ __forceinline__ float vabs(float x) { return fabsf(x); }
 struct DevMap {
     int32_t n, nseqriv, nlfp, npth, npthlev, ndam;
     int32_t lfplain, nlevcell;
+    int32_t lgm, nmem;                         // members per domain = nmem = 1 << lgm
+    int32_t ns, nsp;                           // state slots: n << lgm, npth << lgm
     double dfrcinc;
+    const int32_t* cell_dom;                   // domain of each cell (NULL: one domain)
+    const int32_t* pth_dom;                    // domain of each path (NULL: one domain)
     const int32_t* next;                       // 0-based downstream, <0 for mouths
     const int32_t* up_ptr;                     // upstream CSR (n+1)
     const int32_t* up_idx;
@@ -72,6 +84,14 @@ struct DevMap {
     const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
 };
 
+// Per-instance step of the current outer step (written by the host after the
+// CFL bound is known): DT, PGRV*DT, NADD = sum of DT over the NT substeps.
+struct InstStep {
+    double dt, gdt, nadd;
+    int32_t nt;                                // substeps of this instance
+    int32_t swapfix;                           // odd number of idle substeps: RIVDPH pair swapped
+};
+
 template <class R>
 struct DevState {
     double *rivsto, *fldsto, *levsto, *damsto, *daminf;
@@ -84,25 +104,45 @@ struct DevState {
     double *rivinf, *fldinf, *pthout;
     R* rate;
     R* runoff;
-    R *pthflw, *pthflw_u, *pthflwsum, *pthflwsum_u;   // [npthlev][npth], [npth]
+    R *pthflw, *pthflw_u, *pthflwsum, *pthflwsum_u;   // [npthlev][nsp], [nsp]
     R *rivout_avg, *fldout_avg, *outflw_avg, *rivvel_avg, *pthout_avg, *runoff_avg;
     R *outflw_max, *rivdph_max, *storge_max, *daminf_avg;
     R *pthflw_avg, *pthflwsum_avg;
-    double* partials;                          // [nblocks][NGLB]
-    double* cfl_part;                          // [nblocks]
+    double* partials;                          // [nt][ntiles][NGLB] (single instance only)
     double* glb_hist;                          // [nt_cap][NGLB]
-    double* ctl;                               // [4]: dt_min
+    unsigned long long* cfl;                   // [ninst]: bits of the CFL bound (positive doubles)
+    const InstStep* inst;                      // [ninst]
+    unsigned* bar;                             // [2]: grid barrier of the fused substep kernel
+    int32_t* err;                              // [1]: device error flag (barrier time-out)
 };
 
 struct StepArgs {
-    double dt, gdt;                            // DT, PGRV*DT
     double pgrv, pmanfld, pdstmth, pcadp, pminslp;
     int32_t lfldout, lpthout, ldamout, llevee;
-    int32_t first, last;                       // first / last substep of the outer step
-    double nadd;                               // sum of DT over the outer step (last)
-    int32_t it;                                // substep index (budget record)
-    int32_t nrun;                              // substeps run in this outer step
+    int32_t it;                                // substep index within the outer step
+    int32_t budget;                            // write the per-tile water-budget sums
+    int32_t xcd;                               // XCD-contiguous tile order
 };
+
+// Cell / path slot -> static index, member and instance
+struct Slot {
+    int c, m, inst;
+};
+__device__ __forceinline__ Slot cell_slot(const DevMap& M, int s) {
+    Slot x;
+    x.c = s >> M.lgm;
+    x.m = s & (M.nmem - 1);
+    x.inst = (M.cell_dom ? M.cell_dom[x.c] * M.nmem : 0) + x.m;
+    return x;
+}
+__device__ __forceinline__ Slot path_slot(const DevMap& M, int q) {
+    Slot x;
+    x.c = q >> M.lgm;
+    x.m = q & (M.nmem - 1);
+    x.inst = (M.pth_dom ? M.pth_dom[x.c] * M.nmem : 0) + x.m;
+    return x;
+}
+__device__ __forceinline__ int sidx(const DevMap& M, int c, int m) { return (c << M.lgm) | m; }
 
 // ---------------------------------------------------------------------------
 // Floodplain profile table recomputed on the fly: SET_FLDSTG
@@ -194,7 +234,9 @@ __device__ __forceinline__ void stage_def(const DevMap& M, int i, double rivsto,
     }
 }
 
-// CMF_LEVEE_FLDSTG per cell, src/cmf_ctrl_levee_mod.F90:302-473
+// CMF_LEVEE_FLDSTG per cell, src/cmf_ctrl_levee_mod.F90:302-473. Every case
+// computes the same eight local values and the result is written once, so the
+// compiler keeps the whole inversion in registers.
 __device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivsto, double fldsto,
                                             double levsto, Stage& o) {
     const int nl = M.nlfp;
@@ -212,81 +254,80 @@ __device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivst
     const double rivstomax = rivstomax_of(M, i);
     const double rl = (double)M.rivlen[i], rw = (double)M.rivwth[i], ga = (double)M.grarea[i];
     const double dwth_inc = (ga / rl) * M.dfrcinc;
-    if (dstoall > rivstomax) {
-        if (dstoall < levbassto || dstoall >= levtopsto) {
-            // case 1 (:312-349) and case 4 (:424-461) share the layer search
-            const bool case1 = dstoall < levbassto;
-            FldTab T; T.init(M, i, rivstomax);
-            int I = 1;
-            double dsto_fil = rivstomax, dwth_fil = rw, ddph_fil = 0.0, dwth_add;
-            double smax, grd, hgt;
-            T.layer(1, smax, grd, hgt);
-            if (case1 || dstoall >= levfilsto) {
-                while (dstoall > smax) {
-                    dsto_fil = smax;
-                    dwth_fil = dwth_fil + dwth_inc;
-                    ddph_fil = ddph_fil + grd * dwth_inc;
-                    I = I + 1;
-                    if (I > nl) break;
-                    T.layer(I, smax, grd, hgt);
-                }
-                if (I <= nl) {
-                    const double dsto_add = dstoall - dsto_fil;
-                    dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rl) / grd);
-                    o.flddph = ddph_fil + grd * dwth_add;
-                } else {
-                    const double dsto_add = dstoall - dsto_fil;
-                    dwth_add = 0.0;
-                    o.flddph = ddph_fil + (dsto_add / dwth_fil) / rl;
-                }
-                if (case1) {
-                    o.rivsto = rivstomax + (rl * rw) * o.flddph;
-                    o.rivdph = (o.rivsto / rl) / rw;
-                    const double fs = pstoall - o.rivsto;
-                    o.fldsto = vmax(fs, 0.0);
-                    double fr = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
-                    fr = vmax(fr, 0.0);
-                    fr = vmin(fr, 1.0);
-                    o.fldfrc = fr;
-                    o.fldare = ga * fr;
-                    o.levsto = 0.0; o.levdph = 0.0;
-                } else {
-                    o.fldfrc = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
-                    o.fldare = ga * o.fldfrc;
-                    o.rivsto = rivstomax + (rl * rw) * o.flddph;
-                    o.rivdph = (o.rivsto / rl) / rw;
-                    const double dsto_add = ((o.flddph - levhgt) * (levdst + rw)) * rl;
-                    const double fs = (levtopsto + dsto_add) - o.rivsto;
-                    o.fldsto = vmax(fs, 0.0);
-                    const double ls = (pstoall - o.rivsto) - o.fldsto;
-                    o.levsto = vmax(ls, 0.0);
-                    o.levdph = o.flddph;
-                }
-                return;
-            }
-            // levtopsto <= dstoall < levfilsto: case 3 below
+    double r_riv = 0.0, r_fld = 0.0, r_lev = 0.0, r_rdph = 0.0, r_fdph = 0.0, r_frc = 0.0, r_are = 0.0,
+           r_ldph = 0.0;
+    // case selection (:308-311): 0 in bank; 1 below the levee base; 2 river side
+    // below the crown; 3 protected side filling; 4 above the protected-side fill
+    int kase;
+    if (!(dstoall > rivstomax)) kase = 0;
+    else if (dstoall < levbassto) kase = 1;
+    else if (dstoall < levtopsto) kase = 2;
+    else if (dstoall < levfilsto) kase = 3;
+    else kase = 4;
+    if (kase == 1 || kase == 4) {
+        // case 1 (:312-349) and case 4 (:424-461) share the layer search
+        FldTab T; T.init(M, i, rivstomax);
+        int I = 1;
+        double dsto_fil = rivstomax, dwth_fil = rw, ddph_fil = 0.0, dwth_add;
+        double smax, grd, hgt;
+        T.layer(1, smax, grd, hgt);
+        while (dstoall > smax) {
+            dsto_fil = smax;
+            dwth_fil = dwth_fil + dwth_inc;
+            ddph_fil = ddph_fil + grd * dwth_inc;
+            I = I + 1;
+            if (I > nl) break;
+            T.layer(I, smax, grd, hgt);
         }
-        if (dstoall < levtopsto) {                    // case 2 (:353-368)
-            const double dsto_add = dstoall - levbassto;
-            const double dwth_add = levdst + rw;
-            o.flddph = bashgt + (dsto_add / dwth_add) / rl;
-            o.rivsto = rivstomax + (rl * rw) * o.flddph;
-            o.rivdph = (o.rivsto / rl) / rw;
-            const double fs = pstoall - o.rivsto;
-            o.fldsto = vmax(fs, 0.0);
-            o.fldfrc = levfrc;
-            o.fldare = ga * o.fldfrc;
-            o.levsto = 0.0; o.levdph = 0.0;
-            return;
+        if (I <= nl) {
+            const double dsto_add = dstoall - dsto_fil;
+            dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rl) / grd);
+            r_fdph = ddph_fil + grd * dwth_add;
+        } else {
+            const double dsto_add = dstoall - dsto_fil;
+            dwth_add = 0.0;
+            r_fdph = ddph_fil + (dsto_add / dwth_fil) / rl;
         }
-        // case 3 (:372-420): river side full, protected side filling
-        o.flddph = levhgt;
-        o.rivsto = rivstomax + (rl * rw) * o.flddph;
-        o.rivdph = (o.rivsto / rl) / rw;
-        const double fs = levtopsto - o.rivsto;
-        o.fldsto = vmax(fs, 0.0);
-        const double ls = (pstoall - o.rivsto) - o.fldsto;
-        o.levsto = vmax(ls, 0.0);
+        if (kase == 1) {
+            r_riv = rivstomax + (rl * rw) * r_fdph;
+            r_rdph = (r_riv / rl) / rw;
+            const double fs = pstoall - r_riv;
+            r_fld = vmax(fs, 0.0);
+            double fr = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
+            fr = vmax(fr, 0.0);
+            fr = vmin(fr, 1.0);
+            r_frc = fr;
+            r_are = ga * fr;
+        } else {
+            r_frc = ((-rw + dwth_fil) + dwth_add) / (dwth_inc * (double)nl);
+            r_are = ga * r_frc;
+            r_riv = rivstomax + (rl * rw) * r_fdph;
+            r_rdph = (r_riv / rl) / rw;
+            const double dsto_add = ((r_fdph - levhgt) * (levdst + rw)) * rl;
+            const double fs = (levtopsto + dsto_add) - r_riv;
+            r_fld = vmax(fs, 0.0);
+            const double ls = (pstoall - r_riv) - r_fld;
+            r_lev = vmax(ls, 0.0);
+            r_ldph = r_fdph;
+        }
+    } else if (kase == 2) {                           // (:353-368)
+        const double dsto_add = dstoall - levbassto;
+        const double dwth_add = levdst + rw;
+        r_fdph = bashgt + (dsto_add / dwth_add) / rl;
+        r_riv = rivstomax + (rl * rw) * r_fdph;
+        r_rdph = (r_riv / rl) / rw;
+        const double fs = pstoall - r_riv;
+        r_fld = vmax(fs, 0.0);
+        r_frc = levfrc;
+        r_are = ga * r_frc;
+    } else if (kase == 3) {                           // (:372-420): river side full
+        r_fdph = levhgt;
+        r_riv = rivstomax + (rl * rw) * r_fdph;
+        r_rdph = (r_riv / rl) / rw;
+        const double fs = levtopsto - r_riv;
+        r_fld = vmax(fs, 0.0);
+        const double ls = (pstoall - r_riv) - r_fld;
+        r_lev = vmax(ls, 0.0);
         const int ilev = (int)(levfrc * (double)nl) + 1;
         double dsto_fil = levtopsto, dwth_fil = 0.0, ddph_fil = 0.0;
         FldTab T; T.init(M, i, rivstomax);
@@ -306,27 +347,26 @@ __device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivst
             const double dsto_add = dstoall - dsto_fil;
             const double dwth_add = -dwth_fil + sqrt(dwth_fil * dwth_fil + ((2.0 * dsto_add) / rl) / grd);
             const double ddph_add = dwth_add * grd;
-            o.levdph = (bashgt + ddph_fil) + ddph_add;
+            r_ldph = (bashgt + ddph_fil) + ddph_add;
             double fr = (dwth_fil + levdst) / (dwth_inc * (double)nl);
             fr = vmax(fr, 0.0);
             fr = vmin(fr, 1.0);
-            o.fldfrc = fr;
-            o.fldare = ga * fr;
+            r_frc = fr;
+            r_are = ga * fr;
         } else {
             const double dsto_add = dstoall - dsto_fil;
             const double ddph_add = (dsto_add / dwth_fil) / rl;
-            o.levdph = (bashgt + ddph_fil) + ddph_add;
-            o.fldfrc = 1.0;
-            o.fldare = ga * o.fldfrc;
+            r_ldph = (bashgt + ddph_fil) + ddph_add;
+            r_frc = 1.0;
+            r_are = ga * r_frc;
         }
-        return;
+    } else {                                          // case 0 (:463-473)
+        r_riv = pstoall;
+        const double d = (dstoall / rl) / rw;
+        r_rdph = vmax(d, 0.0);
     }
-    // case 0 (:463-473)
-    o.rivsto = pstoall;
-    const double d = (dstoall / rl) / rw;
-    o.rivdph = vmax(d, 0.0);
-    o.fldsto = 0.0; o.flddph = 0.0; o.fldfrc = 0.0; o.fldare = 0.0;
-    o.levsto = 0.0; o.levdph = 0.0;
+    o.rivsto = r_riv; o.fldsto = r_fld; o.levsto = r_lev;
+    o.rivdph = r_rdph; o.flddph = r_fdph; o.fldfrc = r_frc; o.fldare = r_are; o.levdph = r_ldph;
 }
 
 }  // namespace cmf

@@ -1,31 +1,38 @@
 // cmf_kernels.hip -- HIP kernels of one CMF_PHYSICS_ADVANCE outer step on gfx950.
 //
-// One outer step = k_stage_cfl (+ k_reduce_cfl) followed by NT substeps of three
-// phases; every phase is a grid-wide dependency of the reference's substep
-// (src/cmf_ctrl_physics_mod.F90:64-144):
-//   k_outflow   CMF_CALC_OUTFLW + CMF_DAMOUT_CALC (cells) and CMF_CALC_PTHOUT /
-//               CMF_LEVEE_OPT_PTHOUT (paths) -- one launch, two index ranges
-//   k_rate      CMF_CALC_INFLOW_LSPAMAT parts 1-2: outflow volume and limiter rate
-//   k_update    CMF_CALC_INFLOW_LSPAMAT parts 2-3 (limit, upstream gather),
-//               CMF_DAMOUT_WATBAL, CALC_VARS_PRE, CMF_CALC_STONXT,
-//               CMF_PHYSICS_FLDSTG and CMF_DIAG_AVEMAX_ADPSTP (cells) and the path
-//               limiter/averages (paths) -- one launch
-//   k_reduce_budget  deterministic fixed-order reduction of the water-budget sums
-// All neighbour accesses are gathers (downstream cell, upstream CSR list, path
-// endpoints); nothing is scattered, so results do not depend on scheduling.
+// One outer step = k_stage (FLDSTG + per-instance CFL bound) followed by NT
+// substeps of three phases; every phase is a grid-wide dependency of the
+// reference's substep (src/cmf_ctrl_physics_mod.F90:64-144):
+//   outflow  CMF_CALC_OUTFLW + CMF_DAMOUT_CALC (cells) and CMF_CALC_PTHOUT /
+//            CMF_LEVEE_OPT_PTHOUT (paths)
+//   rate     CMF_CALC_INFLOW_LSPAMAT parts 1-2: outflow volume and limiter rate
+//   update   CMF_CALC_INFLOW_LSPAMAT parts 2-3 (limit, upstream gather),
+//            CMF_DAMOUT_WATBAL, CALC_VARS_PRE, CMF_CALC_STONXT,
+//            CMF_PHYSICS_FLDSTG and CMF_DIAG_AVEMAX_ADPSTP (cells) and the path
+//            limiter/averages (paths)
+// A substep runs either as three kernels (k_outflow, k_rate, k_update) or as one
+// persistent kernel (k_substep) whose blocks loop over the same tiles and meet
+// at a grid barrier between the phases. The tile functions are shared, so both
+// forms compute the same values.
+//
+// Every work item is a state slot s = (cell << lgm) | member; all neighbour
+// accesses are gathers (downstream cell, upstream CSR list, path endpoints) of
+// the same member; nothing is scattered, so results do not depend on
+// scheduling, block size or tile order. An instance (domain, member) whose NT
+// is reached idles for the remaining substeps of the outer step.
 #include "cmf_device.hpp"
 #include "cmf_kernels.hpp"
 
 namespace cmf {
 
 // ---------------------------------------------------------------------------
-// Block reductions (fixed tree order: deterministic)
+// Block reductions (fixed tree order: deterministic for a given block size)
 // Sum over the block: a fixed butterfly inside each 64-lane wavefront, then the
-// BLOCK/64 wave sums in wave order. Result in lds[k * NWAVE] for thread-independent
+// BS/64 wave sums in wave order. Result in lds[k * NW] for thread-independent
 // reads after the final barrier.
-constexpr int NWAVE = BLOCK / 64;
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][NWAVE] */) {
+template <int BS, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][BS/64] */) {
+    constexpr int NW = BS / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -33,38 +40,67 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][N
         for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
     if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) lds[k * NWAVE + wave] = v[k];
+        for (int k = 0; k < NV; ++k) lds[k * NW + wave] = v[k];
     __syncthreads();
     if (threadIdx.x < NV) {
         double s = 0.0;
-        for (int w = 0; w < NWAVE; ++w) s += lds[threadIdx.x * NWAVE + w];
-        lds[threadIdx.x * NWAVE] = s;
+        for (int w = 0; w < NW; ++w) s += lds[threadIdx.x * NW + w];
+        lds[threadIdx.x * NW] = s;
     }
     __syncthreads();
 }
 
+// Tile t of a launch of G blocks: with xcd set and G a multiple of 8, the blocks
+// that the dispatcher places on one XCD (b, b+8, b+16, ...) take one contiguous
+// eighth of the tiles, so neighbour gathers across tile edges mostly hit the
+// same L2.
+__device__ __forceinline__ int tile_of(int b, int G, int ntiles, int xcd) {
+    if (!xcd || (G & 7)) return b;
+    const int per = G >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+struct InstView {
+    double dt, gdt, nadd;
+    int nt;
+    bool active, first, last;
+};
+__device__ __forceinline__ InstView inst_view(const InstStep* tab, int inst, int it) {
+    const InstStep& I = tab[inst];
+    InstView v;
+    v.dt = I.dt; v.gdt = I.gdt; v.nadd = I.nadd; v.nt = I.nt;
+    v.active = it < I.nt;
+    v.first = it == 0;
+    v.last = it == I.nt - 1;
+    return v;
+}
+
 // ---------------------------------------------------------------------------
-// phase 1a: river / mouth / floodplain outflow and dam operation, per cell
+// phase 1a: river / mouth / floodplain outflow and dam operation, per cell slot
 template <class R>
-__device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int i) {
-    const R gdt = R(A.gdt);
+__device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
+    const Slot x = cell_slot(M, s);
+    const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (!I.active) return;
+    const int i = x.c, m = x.m;
+    const R gdt = R(I.gdt);
     const R zero = R(0), one = R(1);
     const R rivelv = R(rivelv_of(M, i));
-    const R rivdph = S.rivdph[i], rivdph_pre = S.rivdph_pre[i];
+    const R rivdph = S.rivdph[s], rivdph_pre = S.rivdph_pre[s];
     const R sfcelv = rivelv + rivdph;                               // outflw:45-47
     const R sfcelv_pre = rivelv + rivdph_pre;
     const R flddph_pre = vmax(rivdph_pre - R(M.rivhgt[i]), zero);
     const R rivwth = R(M.rivwth[i]), rivman = R(M.rivman[i]);
-    const R qr_pre = S.rivout[i];                                   // D2RIVOUT_PRE
-    const R qr_old = S.rivout_vel[i];                               // D2RIVOUT (for RIVVEL)
+    const R qr_pre = S.rivout[s];                                   // D2RIVOUT_PRE
+    const R qr_old = S.rivout_vel[s];                               // D2RIVOUT (for RIVVEL)
     const int nxt = M.next[i];
     const bool river = nxt >= 0;                                     // mouths carry a negative code
     R dwn, dwn_pre, rdst;
     if (river) {                                                     // outflw:52-57
-        const int j = nxt;
+        const int j = nxt, sj = sidx(M, j, m);
         const R rivelv_j = R(rivelv_of(M, j));
-        dwn = rivelv_j + S.rivdph[j];
-        dwn_pre = rivelv_j + S.rivdph_pre[j];
+        dwn = rivelv_j + S.rivdph[sj];
+        dwn_pre = rivelv_j + S.rivdph_pre[sj];
         rdst = one / R(M.nxtdst[i]);
     } else {
         dwn = R(M.dwnelv[i]);
@@ -101,10 +137,10 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     // ---- floodplain: outflw:84-116 (river), :143-171 (mouth)
     if (A.lfldout) {
         const R gdtn2 = gdt * (R(A.pmanfld) * R(A.pmanfld));
-        const R dfsto = R(S.fldsto[i]);
+        const R dfsto = R(S.fldsto[s]);
         const R elevtn = R(M.elevtn[i]);
         const R rrl = one / R(M.rivlen[i]);
-        const R flddph = S.flddph[i];
+        const R flddph = S.flddph[s];
         R dslp = (sfcelv - dwn) * rdst;
         dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
         R dflw, dflw_pr, dare, dare_pr;
@@ -115,20 +151,20 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
             dare = vmax(dare - flddph * rivwth, zero);
             const R dsfc_pr = vmax(sfcelv_pre, dwn_pre);
             dflw_pr = dsfc_pr - elevtn;
-            dare_pr = S.fldsto_pre[i] * rrl;
+            dare_pr = S.fldsto_pre[s] * rrl;
             dare_pr = vmax(dare_pr - flddph_pre * rivwth, R(1.e-6));
         } else {
             dflw = sfcelv - elevtn;
             dare = vmax(dfsto * rrl - flddph * rivwth, zero);
             dflw_pr = sfcelv_pre - elevtn;
-            dare_pr = vmax(S.fldsto_pre[i] * rrl - flddph_pre * rivwth, R(1.e-6));
+            dare_pr = vmax(S.fldsto_pre[s] * rrl - flddph_pre * rivwth, R(1.e-6));
         }
         const R t = dflw * dflw_pr;
         R dflw_im = vsqrt(vmax(t, zero));
         dflw_im = vmax(dflw_im, R(1.e-6));
         R dare_im = vsqrt(dare * dare_pr);
         dare_im = vmax(dare_im, R(1.e-6));
-        const R dout_pr = S.fldout[i];                              // D2FLDOUT_PRE
+        const R dout_pr = S.fldout[s];                              // D2FLDOUT_PRE
         const R dout = (dout_pr + (gdt * dare_im) * dslp) *
                        (one / (one + ((gdtn2 * vabs(dout_pr)) * vpow(dflw_im, R(K_P4O3))) * (one / dare_im)));
         const R thr = river ? R(1.e-5) : R(K_1EM5F);
@@ -138,9 +174,9 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     }
     // ---- reverse-flow storage limiter, river cells: outflw:175-182
     if (river) {
-        R dout = ((-q_r) - q_f) * R(A.dt);
+        R dout = ((-q_r) - q_f) * R(I.dt);
         dout = vmax(dout, R(K_1EM10F));
-        R rate = (R(0.05) * S.storge[i]) / dout;
+        R rate = (R(0.05) * S.storge[s]) / dout;
         rate = vmin(rate, one);
         q_r = q_r * rate;
         q_f = q_f * rate;
@@ -150,14 +186,17 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
         const int code = M.i1dam[i];
         if (code > 0) {
             q_r = zero; q_f = zero;
-            double inf = (double)S.runoff[i];                       // :447
+            double inf = (double)S.runoff[s];                       // :447
             for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {   // :456-464, ascending
                 const int u = M.up_idx[k];
                 const int cu = M.i1dam[u];
-                if (cu == 10 || cu == 11) inf = (inf + (double)S.rivout[u]) + (double)S.fldout[u];
+                if (cu == 10 || cu == 11) {
+                    const int su = sidx(M, u, m);
+                    inf = (inf + (double)S.rivout[su]) + (double)S.fldout[su];
+                }
             }
-            S.daminf[i] = inf;
-            const double dt = A.dt;
+            S.daminf[s] = inf;
+            const double dt = I.dt;
             if (code == 10 && river) {                               // :471-493
                 const int j = nxt;
                 double dslope = ((double)M.elevtn[i] - (double)M.elevtn[j]) * (1.0 / (double)M.nxtdst[i]);
@@ -167,26 +206,26 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
                 const double darea = (double)M.rivwth[i] * rd;
                 vel = R(dvel);
                 double q = darea * dvel;
-                q_r = R(vmin(q, S.rivsto[i] / dt));
+                q_r = R(vmin(q, S.rivsto[s] / dt));
                 const double dslope_f = vmin(0.005, dslope);
-                const double fd = (double)S.flddph[i];
+                const double fd = (double)S.flddph[s];
                 const double dvel_f = ((1.0 / A.pmanfld) * sqrt(dslope_f)) * pow(fd, K_2O3F);
-                double dare_f = S.fldsto[i] * (1.0 / (double)M.rivlen[i]);
+                double dare_f = S.fldsto[s] * (1.0 / (double)M.rivlen[i]);
                 dare_f = vmax(dare_f - fd * (double)M.rivwth[i], 0.0);
                 const double qf = dare_f * dvel_f;
-                q_f = R(vmin(qf, S.fldsto[i] / dt));
+                q_f = R(vmin(qf, S.fldsto[s] / dt));
             }
             const int d = M.dam_of_cell[i];
             if (d >= 0 && M.damstat[d] > 0) {                        // :344-419
-                const double vol = S.damsto[i];
+                const double vol = S.damsto[s];
                 const double cv = M.dam_convol[d], av = M.dam_adjvol[d], ev = M.dam_emevol[d];
                 const double qn = M.dam_qn[d], qf = M.dam_qf[d], qa = M.dam_qa[d];
                 double out = 0.0;
                 if (vol <= cv) {
                     out = qn * sqrt(vol / cv);
                 } else if (vol > cv && vol <= av) {
-                    const double x = (vol - cv) / (av - cv);
-                    out = qn + (x * (x * x)) * (qa - qn);
+                    const double xx = (vol - cv) / (av - cv);
+                    out = qn + (xx * (xx * xx)) * (qa - qn);
                 } else if (vol > av && vol <= ev) {
                     if (inf >= qf) {
                         out = qn + ((vol - cv) / (ev - cv)) * (inf - qn);
@@ -198,47 +237,52 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
                 } else if (vol > ev) {
                     out = (inf >= qf) ? inf : qf;
                 }
-                out = vmin(vmin(out, vol / dt), (S.rivsto[i] + S.fldsto[i]) / dt);
+                out = vmin(vmin(out, vol / dt), (S.rivsto[s] + S.fldsto[s]) / dt);
                 out = vmax(out, 0.0);
                 q_r = R(out);
                 q_f = zero;
             }
         }
     }
-    S.qr_u[i] = q_r;
-    S.qf_u[i] = q_f;
-    if (A.last) S.rivvel[i] = vel;                                  // output-only diagnostic
+    S.qr_u[s] = q_r;
+    S.qf_u[s] = q_f;
+    if (I.last) S.rivvel[s] = vel;                                  // output-only diagnostic
     // CMF_DIAG_AVEMAX_ADPSTP (diag:113) for RIVVEL, final after OUTFLW/DAMOUT
-    R va = (A.first ? zero : S.rivvel_avg[i]) + vel * R(A.dt);
-    if (A.last) va = va / R(A.nadd);
-    S.rivvel_avg[i] = va;
+    R va = (I.first ? zero : S.rivvel_avg[s]) + vel * R(I.dt);
+    if (I.last) va = va / R(I.nadd);
+    S.rivvel_avg[s] = va;
 }
 
 // phase 1b: bifurcation path flow, pthout:20-97 / levee:497-603
 template <class R>
-__device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int p) {
-    const int np = M.npth, nl = M.npthlev;
+__device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int q) {
+    const Slot x = path_slot(M, q);
+    const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (!I.active) return;
+    const int p = x.c, m = x.m;
+    const int np = M.npth, nsp = M.nsp, nl = M.npthlev;
     const int ip = M.pth_upst[p], jp = M.pth_down[p];
     bool skip = ip < 0 || jp < 0;
     if (!skip) skip = A.llevee ? (M.i2mask[ip] == 1 || M.i2mask[jp] == 1) : (M.i2mask[ip] > 0 || M.i2mask[jp] > 0);
     if (skip) {
-        for (int l = 0; l < nl; ++l) S.pthflw_u[(size_t)l * np + p] = R(0);
-        S.pthflwsum_u[p] = R(0);
+        for (int l = 0; l < nl; ++l) S.pthflw_u[(size_t)l * nsp + q] = R(0);
+        S.pthflwsum_u[q] = R(0);
         return;
     }
+    const int si_ = sidx(M, ip, m), sj_ = sidx(M, jp, m);
     const R zero = R(0), one = R(1);
-    const R gdt = R(A.gdt);
+    const R gdt = R(I.gdt);
     const R ri = R(rivelv_of(M, ip)), rj = R(rivelv_of(M, jp));
-    const R si = ri + S.rivdph[ip], sj = rj + S.rivdph[jp];
-    const R pi = ri + S.rivdph_pre[ip], pj = rj + S.rivdph_pre[jp];
+    const R si = ri + S.rivdph[si_], sj = rj + S.rivdph[sj_];
+    const R pi = ri + S.rivdph_pre[si_], pj = rj + S.rivdph_pre[sj_];
     R dslp = (si - sj) / R(M.pth_dst[p]);
     dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
     R li = si, lj = sj;                                              // D2SFCELV_LEV
     if (A.llevee) {
         const int a = M.lev_idx[ip], b = M.lev_idx[jp];
         const int nlc = M.nlevcell;
-        if (a >= 0 && M.lev_tab[nlc + a] < 1.0) li = vmin(R(M.elevtn[ip]) + S.levdph[ip], si);
-        if (b >= 0 && M.lev_tab[nlc + b] < 1.0) lj = vmin(R(M.elevtn[jp]) + S.levdph[jp], sj);
+        if (a >= 0 && M.lev_tab[nlc + a] < 1.0) li = vmin(R(M.elevtn[ip]) + S.levdph[si_], si);
+        if (b >= 0 && M.lev_tab[nlc + b] < 1.0) lj = vmin(R(M.elevtn[jp]) + S.levdph[sj_], sj);
     }
     R sum = zero;
     for (int l = 0; l < nl; ++l) {
@@ -254,134 +298,138 @@ __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>&
             dflw = vmax(vmax(li, lj) - elv, zero);
             dflw_im = dflw;
         }
-        R q = zero;
+        R qq = zero;
         if (dflw_im > R(1.e-5)) {
             const R w = R(M.pth_wth[(size_t)l * np + p]);
             const R man = R(M.pth_man[l]);
-            const R pre = S.pthflw[(size_t)l * np + p];             // D1PTHFLW_PRE
+            const R pre = S.pthflw[(size_t)l * nsp + q];            // D1PTHFLW_PRE
             const R dout_pr = (A.llevee && l > 0) ? pre * (one / w) : pre / w;
-            q = (w * (dout_pr + (gdt * dflw_im) * dslp)) /
-                (one + ((gdt * (man * man)) * vabs(dout_pr)) * vpow(dflw_im, R(K_P7O3)));
+            qq = (w * (dout_pr + (gdt * dflw_im) * dslp)) /
+                 (one + ((gdt * (man * man)) * vabs(dout_pr)) * vpow(dflw_im, R(K_P7O3)));
         }
-        S.pthflw_u[(size_t)l * np + p] = q;
-        sum = sum + q;
+        S.pthflw_u[(size_t)l * nsp + q] = qq;
+        sum = sum + qq;
     }
     if (sum != zero) {                                               // pthout:84-95
-        R rate = (R(0.05) * vmin(S.storge[ip], S.storge[jp])) / vabs(sum * R(A.dt));
+        R rate = (R(0.05) * vmin(S.storge[si_], S.storge[sj_])) / vabs(sum * R(I.dt));
         rate = vmin(rate, one);
-        for (int l = 0; l < nl; ++l) S.pthflw_u[(size_t)l * np + p] = S.pthflw_u[(size_t)l * np + p] * rate;
+        for (int l = 0; l < nl; ++l) S.pthflw_u[(size_t)l * nsp + q] = S.pthflw_u[(size_t)l * nsp + q] * rate;
         sum = sum * rate;
     }
-    S.pthflwsum_u[p] = sum;
-}
-
-template <class R>
-__global__ __launch_bounds__(BLOCK) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int nb_cells) {
-    if ((int)blockIdx.x < nb_cells) {
-        const int i = blockIdx.x * BLOCK + threadIdx.x;
-        if (i < M.n) outflow_cell<R>(M, S, A, i);
-    } else {
-        const int p = (blockIdx.x - nb_cells) * BLOCK + threadIdx.x;
-        if (p < M.npth) pthflow_path<R>(M, S, A, p);
-    }
+    S.pthflwsum_u[q] = sum;
 }
 
 // ---------------------------------------------------------------------------
 // phase 2: outflow volume and limiter rate, inflow_lspamat:410-451
 template <class R>
-__global__ __launch_bounds__(BLOCK) void k_rate(DevMap M, DevState<R> S, StepArgs A) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= M.n) return;
-    const double dt = A.dt;
-    double so = (double)vmax(S.qr_u[i], R(0)) + (double)vmax(S.qf_u[i], R(0));
+__device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
+    const Slot x = cell_slot(M, s);
+    const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (!I.active) return;
+    const int i = x.c, m = x.m;
+    const double dt = I.dt;
+    double so = (double)vmax(S.qr_u[s], R(0)) + (double)vmax(S.qf_u[s], R(0));
     for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {
-        const int u = M.up_idx[k];
-        so = (so + (double)vmax(-S.qr_u[u], R(0))) + (double)vmax(-S.qf_u[u], R(0));
+        const int su = sidx(M, M.up_idx[k], m);
+        so = (so + (double)vmax(-S.qr_u[su], R(0))) + (double)vmax(-S.qf_u[su], R(0));
     }
     so = so * dt;
     if (A.lpthout) {
         for (int k = M.pout_ptr[i]; k < M.pout_ptr[i + 1]; ++k)
-            so = so + (double)vmax(S.pthflwsum_u[M.pout_idx[k]], R(0)) * dt;
+            so = so + (double)vmax(S.pthflwsum_u[sidx(M, M.pout_idx[k], m)], R(0)) * dt;
         for (int k = M.pinf_ptr[i]; k < M.pinf_ptr[i + 1]; ++k)
-            so = so + (double)vmax(-S.pthflwsum_u[M.pinf_idx[k]], R(0)) * dt;
+            so = so + (double)vmax(-S.pthflwsum_u[sidx(M, M.pinf_idx[k], m)], R(0)) * dt;
     }
     const R dout = vmax(R(so), R(1.e-10));
-    const R dsto = R(S.rivsto[i] + S.fldsto[i]);
-    S.rate[i] = vmin(dsto / dout, R(1));
+    const R dsto = R(S.rivsto[s] + S.fldsto[s]);
+    S.rate[s] = vmin(dsto / dout, R(1));
 }
 
 // ---------------------------------------------------------------------------
 // phase 3: limit, gather inflow, dam balance, storage update, stage, averages
 template <class R>
-__device__ __forceinline__ R limited_path_sum(const DevMap& M, const DevState<R>& S, int p) {
-    const R s = S.pthflwsum_u[p];
+__device__ __forceinline__ R limited_path_sum(const DevMap& M, const DevState<R>& S, int p, int m) {
+    const R s = S.pthflwsum_u[sidx(M, p, m)];
     const int ip = M.pth_upst[p], jp = M.pth_down[p];
     if (M.i2mask[ip] > 0 || M.i2mask[jp] > 0) return s;               // inflow_lspamat:501
-    return (s >= R(0)) ? s * S.rate[ip] : s * S.rate[jp];
+    return (s >= R(0)) ? s * S.rate[sidx(M, ip, m)] : s * S.rate[sidx(M, jp, m)];
 }
 
 template <class R>
 __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A,
-                                            int i, double (&g)[NGLB]) {
-    const double dt = A.dt;
+                                            int s, double (&g)[NGLB]) {
+    const Slot x = cell_slot(M, s);
+    const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (!I.active) return;
+    const int i = x.c, m = x.m;
+    const double dt = I.dt;
     const R zero = R(0);
     // limiter, inflow_lspamat:456-473
-    const R qr = S.qr_u[i], qf = S.qf_u[i];
-    const R ri = S.rate[i];
+    const R qr = S.qr_u[s], qf = S.qf_u[s];
+    const R ri = S.rate[s];
     const int nxt = M.next[i];
-    const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[nxt] : ri;
+    const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[sidx(M, nxt, m)] : ri;
     const R qrl = qr * r, qfl = qf * r;
     // upstream gather, inflow_lspamat:481-489
     double rivinf = 0.0, fldinf = 0.0;
     for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {
-        const int u = M.up_idx[k];
-        const R qu = S.qr_u[u], fu = S.qf_u[u];
-        const R ru = (qu > zero) ? S.rate[u] : ri;
+        const int su = sidx(M, M.up_idx[k], m);
+        const R qu = S.qr_u[su], fu = S.qf_u[su];
+        const R ru = (qu > zero) ? S.rate[su] : ri;
         rivinf = rivinf + (double)(qu * ru);
         fldinf = fldinf + (double)(fu * ru);
     }
     double pthout = 0.0;
     if (A.lpthout) {                                                  // :519-535
         for (int k = M.pout_ptr[i]; k < M.pout_ptr[i + 1]; ++k)
-            pthout = pthout + (double)limited_path_sum<R>(M, S, M.pout_idx[k]);
+            pthout = pthout + (double)limited_path_sum<R>(M, S, M.pout_idx[k], m);
         for (int k = M.pinf_ptr[i]; k < M.pinf_ptr[i + 1]; ++k)
-            pthout = pthout - (double)limited_path_sum<R>(M, S, M.pinf_idx[k]);
+            pthout = pthout - (double)limited_path_sum<R>(M, S, M.pinf_idx[k], m);
     }
-    const R rivinf_r = R(rivinf), fldinf_r = R(fldinf), pthout_r = R(pthout);
-    if (A.last) { S.rivinf[i] = rivinf; S.fldinf[i] = fldinf; S.pthout[i] = pthout; }
-    const R runoff = S.runoff[i];
+    // RIVINF/FLDINF/PTHOUT enter the storage update as the FP64 sums of the
+    // limited fluxes (inflow_lspamat:538-544 rounds them to JPRB; with R = double
+    // that is the identity). With R = float the FP64 sums are exact sums of the
+    // FP32 outflows that the upstream cells subtract, so the update conserves mass
+    // to FP64 rounding.
+    const double rivinf_d = (double)R(rivinf), fldinf_d = (double)R(fldinf), pthout_d = (double)R(pthout);
+    const double rinf = sizeof(R) == sizeof(double) ? rivinf_d : rivinf;
+    const double finf = sizeof(R) == sizeof(double) ? fldinf_d : fldinf;
+    const double pout = sizeof(R) == sizeof(double) ? pthout_d : pthout;
+    const R pthout_r = R(pthout);
+    if (I.last) { S.rivinf[s] = rivinf; S.fldinf[s] = fldinf; S.pthout[s] = pthout; }
+    const R runoff = S.runoff[s];
     // CMF_DAMOUT_WATBAL, damout:521-536
     if (A.ldamout) {
         const int d = M.dam_of_cell[i];
         if (d >= 0 && M.damstat[d] != -9999) {
-            const double inflow = ((double)rivinf_r + (double)fldinf_r) + (double)runoff;
+            const double inflow = (rinf + finf) + (double)runoff;
             const double outflw = (double)qrl + (double)qfl;
-            S.damsto[i] = (S.damsto[i] + inflow * dt) - outflw * dt;
+            S.damsto[s] = (S.damsto[s] + inflow * dt) - outflw * dt;
         }
     }
     // CALC_VARS_PRE, physics:254-276. RIVOUT_PRE/FLDOUT_PRE share the RIVOUT/FLDOUT
     // arrays; RIVDPH/RIVDPH_PRE are a buffer pair whose roles the host swaps after
     // this kernel, so the new depth goes to the buffer of the old RIVDPH_PRE.
-    double rs = S.rivsto[i], fs = S.fldsto[i];
-    S.rivout[i] = qrl;
-    S.fldout[i] = qfl;
-    S.fldsto_pre[i] = R(fs);
+    double rs = S.rivsto[s], fs = S.fldsto[s];
+    S.rivout[s] = qrl;
+    S.fldout[s] = qfl;
+    S.fldsto_pre[s] = R(fs);
     // CMF_CALC_STONXT, stonxt:54-98
     g[G_STOPRE] = rs + fs;
-    g[G_RIVINF] = (double)rivinf_r * dt + (double)fldinf_r * dt;
-    g[G_RIVOUT] = ((double)qrl * dt + (double)qfl * dt) + (double)pthout_r * dt;
-    rs = (rs + (double)rivinf_r * dt) - (double)qrl * dt;
+    g[G_RIVINF] = rinf * dt + finf * dt;
+    g[G_RIVOUT] = ((double)qrl * dt + (double)qfl * dt) + pout * dt;
+    rs = (rs + rinf * dt) - (double)qrl * dt;
     if (rs < 0.0) { fs = fs + rs; rs = 0.0; }
-    fs = ((fs + (double)fldinf_r * dt) - (double)qfl * dt) - (double)pthout_r * dt;
+    fs = ((fs + finf * dt) - (double)qfl * dt) - pout * dt;
     if (fs < 0.0) { rs = vmax(rs + fs, 0.0); fs = 0.0; }
     g[G_STONXT] = rs + fs;
     const R outflw = qrl + qfl;
-    const R fldfrc_old = S.fldfrc[i];
+    const R fldfrc_old = S.fldfrc[s];
     const R drivrof = (runoff * (R(1) - fldfrc_old)) * R(dt);
     const R dfldrof = (runoff * fldfrc_old) * R(dt);
     rs = rs + (double)drivrof;
     fs = fs + (double)dfldrof;
-    double levsto = A.llevee ? S.levsto[i] : 0.0;
+    double levsto = A.llevee ? S.levsto[s] : 0.0;
     double st = rs + fs;
     if (A.llevee) st = st + levsto;
     g[G_STONEW] = (double)R(st);
@@ -389,25 +437,25 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     Stage o;
     if (A.llevee) stage_levee(M, i, rs, fs, levsto, o);
     else stage_def(M, i, rs, fs, o);
-    S.rivsto[i] = o.rivsto;
-    S.fldsto[i] = o.fldsto;
+    S.rivsto[s] = o.rivsto;
+    S.fldsto[s] = o.fldsto;
     const R rivdph = R(o.rivdph);
-    S.rivdph_next[i] = rivdph;
-    S.flddph[i] = R(o.flddph);
-    S.fldfrc[i] = R(o.fldfrc);
-    if (A.last) {                                                   // output-only diagnostics
-        S.fldare[i] = R(o.fldare);
-        S.sfcelv[i] = R(rivelv_of(M, i)) + rivdph;
-        S.outflw[i] = outflw;
+    S.rivdph_next[s] = rivdph;
+    S.flddph[s] = R(o.flddph);
+    S.fldfrc[s] = R(o.fldfrc);
+    if (I.last) {                                                   // output-only diagnostics
+        S.fldare[s] = R(o.fldare);
+        S.sfcelv[s] = R(rivelv_of(M, i)) + rivdph;
+        S.outflw[s] = outflw;
     }
     double stg = o.rivsto + o.fldsto;
     if (A.llevee) {
-        S.levsto[i] = o.levsto;
-        S.levdph[i] = R(o.levdph);
+        S.levsto[s] = o.levsto;
+        S.levdph[s] = R(o.levdph);
         stg = stg + o.levsto;
     }
     const R storge = R(stg);
-    S.storge[i] = storge;
+    S.storge[s] = storge;
     g[G_STOPRE2] = A.llevee ? (rs + fs) + levsto : rs + fs;
     g[G_STONEW2] = stg;
     g[G_RIVSTO] = o.rivsto;
@@ -416,123 +464,237 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     g[G_LEVSTO] = A.llevee ? o.levsto : 0.0;
     // CMF_DIAG_AVEMAX_ADPSTP, diag:108-124, and GETAVE :179-186 on the last substep
     const R rdt = R(dt);
-    const bool f0 = A.first;
-    R a1 = (f0 ? zero : S.rivout_avg[i]) + qrl * rdt;
-    R a2 = (f0 ? zero : S.fldout_avg[i]) + qfl * rdt;
-    R a3 = (f0 ? zero : S.outflw_avg[i]) + outflw * rdt;
-    if (A.last) {
-        const R nadd = R(A.nadd);
+    const bool f0 = I.first;
+    R a1 = (f0 ? zero : S.rivout_avg[s]) + qrl * rdt;
+    R a2 = (f0 ? zero : S.fldout_avg[s]) + qfl * rdt;
+    R a3 = (f0 ? zero : S.outflw_avg[s]) + outflw * rdt;
+    if (I.last) {
+        const R nadd = R(I.nadd);
         a1 = a1 / nadd; a2 = a2 / nadd; a3 = a3 / nadd;
         // RUNOFF is constant over the outer step: its NT additions are replayed in
         // registers, in the same order, instead of a read-modify-write per substep
         R a5 = zero;
-        for (int k = 0; k < A.nrun; ++k) a5 = a5 + runoff * rdt;
-        S.runoff_avg[i] = a5 / nadd;
+        for (int k = 0; k < I.nt; ++k) a5 = a5 + runoff * rdt;
+        S.runoff_avg[s] = a5 / nadd;
     }
-    S.rivout_avg[i] = a1; S.fldout_avg[i] = a2; S.outflw_avg[i] = a3;
+    S.rivout_avg[s] = a1; S.fldout_avg[s] = a2; S.outflw_avg[s] = a3;
     if (A.lpthout) {
-        R a4 = (f0 ? zero : S.pthout_avg[i]) + pthout_r * rdt;
-        if (A.last) a4 = a4 / R(A.nadd);
-        S.pthout_avg[i] = a4;
+        R a4 = (f0 ? zero : S.pthout_avg[s]) + pthout_r * rdt;
+        if (I.last) a4 = a4 / R(I.nadd);
+        S.pthout_avg[s] = a4;
     } else if (f0) {
-        S.pthout_avg[i] = zero;                                     // PTHOUT = 0 without paths
+        S.pthout_avg[s] = zero;                                     // PTHOUT = 0 without paths
     }
-    S.outflw_max[i] = vmax(f0 ? zero : S.outflw_max[i], vabs(outflw));
-    S.rivdph_max[i] = vmax(f0 ? zero : S.rivdph_max[i], rivdph);
-    S.storge_max[i] = vmax(f0 ? zero : S.storge_max[i], storge);
+    S.outflw_max[s] = vmax(f0 ? zero : S.outflw_max[s], vabs(outflw));
+    S.rivdph_max[s] = vmax(f0 ? zero : S.rivdph_max[s], rivdph);
+    S.storge_max[s] = vmax(f0 ? zero : S.storge_max[s], storge);
     if (A.ldamout) {
-        R da = (f0 ? zero : S.daminf_avg[i]) + R(S.daminf[i] * dt);
-        if (A.last) da = da / R(A.nadd);
-        S.daminf_avg[i] = da;
+        R da = (f0 ? zero : S.daminf_avg[s]) + R(S.daminf[s] * dt);
+        if (I.last) da = da / R(I.nadd);
+        S.daminf_avg[s] = da;
     }
 }
 
 template <class R>
-__device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int p) {
-    const int np = M.npth, nl = M.npthlev;
+__device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int q) {
+    const Slot x = path_slot(M, q);
+    const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (!I.active) return;
+    const int p = x.c, m = x.m;
+    const int nsp = M.nsp, nl = M.npthlev;
     const int ip = M.pth_upst[p], jp = M.pth_down[p];
     const bool lim = !(ip < 0 || jp < 0) && !(M.i2mask[ip] > 0 || M.i2mask[jp] > 0);
     const R zero = R(0);
-    const R rdt = R(A.dt);
+    const R rdt = R(I.dt);
+    const R rate_i = lim ? S.rate[sidx(M, ip, m)] : zero;
+    const R rate_j = lim ? S.rate[sidx(M, jp, m)] : zero;
     R sumavg = zero;
     for (int l = 0; l < nl; ++l) {
-        R q = S.pthflw_u[(size_t)l * np + p];
-        if (lim) q = (q >= zero) ? q * S.rate[ip] : q * S.rate[jp];   // inflow_lspamat:503-509
-        S.pthflw[(size_t)l * np + p] = q;                            // D1PTHFLW == D1PTHFLW_PRE
-        R a = (A.first ? zero : S.pthflw_avg[(size_t)l * np + p]) + q * rdt;
-        if (A.last) {
-            a = a / R(A.nadd);
+        R qq = S.pthflw_u[(size_t)l * nsp + q];
+        if (lim) qq = (qq >= zero) ? qq * rate_i : qq * rate_j;       // inflow_lspamat:503-509
+        S.pthflw[(size_t)l * nsp + q] = qq;                          // D1PTHFLW == D1PTHFLW_PRE
+        R a = (I.first ? zero : S.pthflw_avg[(size_t)l * nsp + q]) + qq * rdt;
+        if (I.last) {
+            a = a / R(I.nadd);
             sumavg = sumavg + a;                                     // diag:206-213
         }
-        S.pthflw_avg[(size_t)l * np + p] = a;
+        S.pthflw_avg[(size_t)l * nsp + q] = a;
     }
-    R s = S.pthflwsum_u[p];
-    if (lim) s = (s >= zero) ? s * S.rate[ip] : s * S.rate[jp];       // :511-515
-    S.pthflwsum[p] = s;
-    if (A.last) S.pthflwsum_avg[p] = sumavg;
+    R sm = S.pthflwsum_u[q];
+    if (lim) sm = (sm >= zero) ? sm * rate_i : sm * rate_j;          // :511-515
+    S.pthflwsum[q] = sm;
+    if (I.last) S.pthflwsum_avg[q] = sumavg;
 }
 
-template <class R>
-__global__ __launch_bounds__(BLOCK) void k_update(DevMap M, DevState<R> S, StepArgs A, int nb_cells) {
-    __shared__ double lds[NGLB * NWAVE];
-    if ((int)blockIdx.x < nb_cells) {
-        const int i = blockIdx.x * BLOCK + threadIdx.x;
+// ---------------------------------------------------------------------------
+// tiles: a tile is BS consecutive slots; cell tiles first, then path tiles
+template <class R, int BS>
+__device__ __forceinline__ void outflow_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t,
+                                             int ntc) {
+    if (t < ntc) {
+        const int s = t * BS + threadIdx.x;
+        if (s < M.ns) outflow_cell<R>(M, S, A, s);
+    } else {
+        const int q = (t - ntc) * BS + threadIdx.x;
+        if (q < M.nsp) pthflow_path<R>(M, S, A, q);
+    }
+}
+
+template <class R, int BS>
+__device__ __forceinline__ void rate_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t) {
+    const int s = t * BS + threadIdx.x;
+    if (s < M.ns) rate_cell<R>(M, S, A, s);
+}
+
+template <class R, int BS>
+__device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t, int ntc,
+                                            double* lds) {
+    if (t < ntc) {
+        const int s = t * BS + threadIdx.x;
         double g[NGLB];
 #pragma unroll
         for (int k = 0; k < NGLB; ++k) g[k] = 0.0;
-        if (i < M.n) update_cell<R>(M, S, A, i, g);
-        block_sum<NGLB>(g, lds);
-        // per-substep slot: reduced once for all substeps at the end of the outer step
-        if (threadIdx.x < NGLB)
-            S.partials[((size_t)A.it * nb_cells + blockIdx.x) * NGLB + threadIdx.x] = lds[threadIdx.x * NWAVE];
+        if (s < M.ns) update_cell<R>(M, S, A, s, g);
+        if (A.budget) {
+            block_sum<BS, NGLB>(g, lds);
+            // per-substep slot: reduced once for all substeps at the end of the outer step
+            if (threadIdx.x < NGLB)
+                S.partials[((size_t)A.it * ntc + t) * NGLB + threadIdx.x] = lds[threadIdx.x * (BS / 64)];
+            __syncthreads();
+        }
     } else {
-        const int p = (blockIdx.x - nb_cells) * BLOCK + threadIdx.x;
-        if (p < M.npth) update_path<R>(M, S, A, p);
+        const int q = (t - ntc) * BS + threadIdx.x;
+        if (q < M.nsp) update_path<R>(M, S, A, q);
     }
 }
 
-// deterministic second-level reduction of the per-block budget sums: block b
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+    const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
+    if (t < nt_all) outflow_tile<R, BS>(M, S, A, t, ntc);
+}
+
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_rate(DevMap M, DevState<R> S, StepArgs A, int ntc) {
+    const int t = tile_of(blockIdx.x, gridDim.x, ntc, A.xcd);
+    if (t < ntc) rate_tile<R, BS>(M, S, A, t);
+}
+
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+    __shared__ double lds[NGLB * (BS / 64)];
+    const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
+    if (t < nt_all) update_tile<R, BS>(M, S, A, t, ntc, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent substep: blocks loop over the tiles of each phase and meet at a
+// grid barrier between phases. Launched cooperatively (all blocks resident).
+// The barrier wait is bounded: on time-out it raises S.err[0] and lets the
+// kernel finish; the host turns the flag into an error.
+constexpr long long BARRIER_SPIN_LIMIT = 1ll << 22;
+
+__device__ __forceinline__ void grid_barrier(unsigned* bar, int32_t* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();                                           // publish this block's writes
+        const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long long spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > BARRIER_SPIN_LIMIT) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// tiles [lo, hi) of block b in a persistent grid of G blocks
+__device__ __forceinline__ void tile_range(int b, int G, int ntiles, int xcd, int& first, int& step, int& end) {
+    if (xcd && (G & 7) == 0) {
+        const int x = b & 7, per = G >> 3;
+        const int lo = (int)(((long long)ntiles * x) >> 3), hi = (int)(((long long)ntiles * (x + 1)) >> 3);
+        first = lo + (b >> 3); step = per; end = hi;
+    } else {
+        first = b; step = G; end = ntiles;
+    }
+}
+
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArgs A, int ntc_o, int nt_o,
+                                                int ntc_r, int ntc_u, int nt_u) {
+    __shared__ double lds[NGLB * (BS / 64)];
+    int t0, dt, te;
+    tile_range(blockIdx.x, gridDim.x, nt_o, A.xcd, t0, dt, te);
+    for (int t = t0; t < te; t += dt) outflow_tile<R, BS>(M, S, A, t, ntc_o);
+    grid_barrier(S.bar, S.err);
+    tile_range(blockIdx.x, gridDim.x, ntc_r, A.xcd, t0, dt, te);
+    for (int t = t0; t < te; t += dt) rate_tile<R, BS>(M, S, A, t);
+    grid_barrier(S.bar, S.err);
+    tile_range(blockIdx.x, gridDim.x, nt_u, A.xcd, t0, dt, te);
+    for (int t = t0; t < te; t += dt) update_tile<R, BS>(M, S, A, t, ntc_u, lds);
+}
+
+// deterministic second-level reduction of the per-tile budget sums: block b
 // reduces substep b of the outer step
-__global__ __launch_bounds__(BLOCK) void k_reduce_budget(const double* partials, int nb, double* out) {
-    __shared__ double lds[NGLB * NWAVE];
+constexpr int RBS = 256;
+__global__ __launch_bounds__(RBS) void k_reduce_budget(const double* partials, int nb, double* out) {
+    __shared__ double lds[NGLB * (RBS / 64)];
     const double* src = partials + (size_t)blockIdx.x * nb * NGLB;
     double v[NGLB];
 #pragma unroll
     for (int k = 0; k < NGLB; ++k) v[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BLOCK)
+    for (int b = threadIdx.x; b < nb; b += RBS)
 #pragma unroll
         for (int k = 0; k < NGLB; ++k) v[k] += src[(size_t)b * NGLB + k];
-    block_sum<NGLB>(v, lds);
-    if (threadIdx.x < NGLB) out[(size_t)blockIdx.x * NGLB + threadIdx.x] = lds[threadIdx.x * NWAVE];
+    block_sum<RBS, NGLB>(v, lds);
+    if (threadIdx.x < NGLB) out[(size_t)blockIdx.x * NGLB + threadIdx.x] = lds[threadIdx.x * (RBS / 64)];
 }
 
 // ---------------------------------------------------------------------------
 // outer-step start: CMF_PHYSICS_FLDSTG and the CFL bound of CALC_ADPSTP
-template <class R>
-__global__ __launch_bounds__(BLOCK) void k_stage(DevMap M, DevState<R> S, StepArgs A, int with_cfl) {
-    __shared__ double lds[BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+__global__ void k_cfl_init(unsigned long long* cfl, int ninst, double dt_def) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ninst) cfl[k] = (unsigned long long)__double_as_longlong(dt_def);
+}
+
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_stage(DevMap M, DevState<R> S, StepArgs A, int with_cfl) {
+    const int s = blockIdx.x * BS + threadIdx.x;
     double c = 1.0e300;
-    if (i < M.n) {
+    int inst = -1;
+    if (s < M.ns) {
+        const Slot x = cell_slot(M, s);
+        const int i = x.c;
+        inst = x.inst;
         Stage o;
-        const double levsto = A.llevee ? S.levsto[i] : 0.0;
-        if (A.llevee) stage_levee(M, i, S.rivsto[i], S.fldsto[i], levsto, o);
-        else stage_def(M, i, S.rivsto[i], S.fldsto[i], o);
-        S.rivsto[i] = o.rivsto;
-        S.fldsto[i] = o.fldsto;
+        const double levsto = A.llevee ? S.levsto[s] : 0.0;
+        if (A.llevee) stage_levee(M, i, S.rivsto[s], S.fldsto[s], levsto, o);
+        else stage_def(M, i, S.rivsto[s], S.fldsto[s], o);
+        S.rivsto[s] = o.rivsto;
+        S.fldsto[s] = o.fldsto;
         const R rivdph = R(o.rivdph);
-        S.rivdph[i] = rivdph;
-        S.flddph[i] = R(o.flddph);
-        S.fldfrc[i] = R(o.fldfrc);
-        S.fldare[i] = R(o.fldare);
-        S.sfcelv[i] = R(rivelv_of(M, i)) + rivdph;
+        S.rivdph[s] = rivdph;
+        S.flddph[s] = R(o.flddph);
+        S.fldfrc[s] = R(o.fldfrc);
+        S.fldare[s] = R(o.fldare);
+        S.sfcelv[s] = R(rivelv_of(M, i)) + rivdph;
         double stg = o.rivsto + o.fldsto;
         if (A.llevee) {
-            S.levsto[i] = o.levsto;
-            S.levdph[i] = R(o.levdph);
+            S.levsto[s] = o.levsto;
+            S.levdph[s] = R(o.levdph);
             stg = stg + o.levsto;
         }
-        S.storge[i] = R(stg);
+        S.storge[s] = R(stg);
         if (with_cfl && M.i2mask[i] == 0) {                          // physics:178-197
             const double ddph = vmax((double)rivdph, 0.01);
             const double ddst = (M.next[i] >= 0) ? (double)M.nxtdst[i] : A.pdstmth;
@@ -540,69 +702,138 @@ __global__ __launch_bounds__(BLOCK) void k_stage(DevMap M, DevState<R> S, StepAr
         }
     }
     if (!with_cfl) return;
-    lds[threadIdx.x] = c;
-    __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) lds[threadIdx.x] = vmin(lds[threadIdx.x], lds[threadIdx.x + s]);
-        __syncthreads();
+    // per-instance minimum: the lanes of one member (stride nmem) reduce inside
+    // the wavefront; positive doubles order like their bit patterns, so the
+    // atomic minimum over the bits is the numeric minimum (order-independent).
+    // A wavefront starts at a multiple of 64 slots and nmem divides 64, so lane
+    // k < nmem holds member k.
+    const int lane = threadIdx.x & 63;
+    const int d = (s < M.ns && M.cell_dom) ? M.cell_dom[s >> M.lgm] : 0;
+    const int d0 = __shfl(d, 0, 64);
+    const bool one_dom = __all(d == d0 || s >= M.ns);
+    if (one_dom) {
+        for (int off = M.nmem; off < 64; off <<= 1) c = vmin(c, __shfl_xor(c, off, 64));
+        if (lane < M.nmem && c < 1.0e300)
+            atomicMin(&S.cfl[d0 * M.nmem + lane], (unsigned long long)__double_as_longlong(c));
+    } else if (inst >= 0 && c < 1.0e300) {
+        atomicMin(&S.cfl[inst], (unsigned long long)__double_as_longlong(c));
     }
-    if (threadIdx.x == 0) S.cfl_part[blockIdx.x] = lds[0];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_reduce_cfl(const double* part, int nb, double dt_def, double* ctl) {
-    __shared__ double lds[BLOCK];
-    double m = dt_def;
-    for (int b = threadIdx.x; b < nb; b += BLOCK) m = vmin(m, part[b]);
-    lds[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) lds[threadIdx.x] = vmin(lds[threadIdx.x], lds[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) ctl[0] = lds[0];
+// instances that idled an odd number of substeps hold RIVDPH in the buffer the
+// host now calls RIVDPH_PRE: swap the pair back for their slots
+template <class R>
+__global__ void k_swapfix(DevMap M, DevState<R> S) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M.ns) return;
+    const Slot x = cell_slot(M, s);
+    if (!S.inst[x.inst].swapfix) return;
+    const R a = S.rivdph[s];
+    S.rivdph[s] = S.rivdph_pre[s];
+    S.rivdph_pre[s] = a;
+}
+
+// runoff[s] = base[c] * factor[inst] (multiplicative member perturbation)
+template <class R>
+__global__ void k_scale_runoff(DevMap M, DevState<R> S, const double* base, const double* factor) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M.ns) return;
+    const Slot x = cell_slot(M, s);
+    S.runoff[s] = R(base[x.c] * factor[x.inst]);
 }
 
 // ---------------------------------------------------------------------------
 // launch wrappers
+static inline int cdiv(long long a, int b) { return (int)((a + b - 1) / b); }
+static inline int pad8(int g, int xcd) { return xcd ? (g + 7) & ~7 : g; }
+
+template <class R, int BS>
+static void launch_stage_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst,
+                            double dt_def, hipStream_t st) {
+    if (with_cfl) hipLaunchKernelGGL(k_cfl_init, dim3(cdiv(ninst, 64)), dim3(64), 0, st, S.cfl, ninst, dt_def);
+    hipLaunchKernelGGL((k_stage<R, BS>), dim3(cdiv(M.ns, BS)), dim3(BS), 0, st, M, S, A, with_cfl);
+}
+
 template <class R>
-void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, double dt_def,
+void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst, double dt_def,
                   hipStream_t st) {
-    const int nb = (M.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(k_stage<R>, dim3(nb), dim3(BLOCK), 0, st, M, S, A, with_cfl);
-    if (with_cfl) hipLaunchKernelGGL(k_reduce_cfl, dim3(1), dim3(BLOCK), 0, st, S.cfl_part, nb, dt_def, S.ctl);
+    launch_stage_bs<R, 256>(M, S, A, with_cfl, ninst, dt_def, st);
+}
+
+int tiles_cells(const DevMap& M, int bs) { return cdiv(M.ns, bs); }
+int tiles_paths(const DevMap& M, const StepArgs& A, int bs) { return A.lpthout ? cdiv(M.nsp, bs) : 0; }
+
+template <class R, int BS>
+static void launch_phases_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, hipStream_t st) {
+    const int ntc = tiles_cells(M, BS), ntp = tiles_paths(M, A, BS);
+    if (phase == 0)
+        hipLaunchKernelGGL((k_outflow<R, BS>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc, ntc + ntp);
+    else if (phase == 1)
+        hipLaunchKernelGGL((k_rate<R, BS>), dim3(pad8(ntc, A.xcd)), dim3(BS), 0, st, M, S, A, ntc);
+    else
+        hipLaunchKernelGGL((k_update<R, BS>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc, ntc + ntp);
 }
 
 template <class R>
-void launch_outflow(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st) {
-    const int nbc = (M.n + BLOCK - 1) / BLOCK;
-    const int nbp = A.lpthout ? (M.npth + BLOCK - 1) / BLOCK : 0;
-    hipLaunchKernelGGL(k_outflow<R>, dim3(nbc + nbp), dim3(BLOCK), 0, st, M, S, A, nbc);
+void launch_phase(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, int bs, hipStream_t st) {
+    switch (bs) {
+        case 64: launch_phases_bs<R, 64>(M, S, A, phase, st); break;
+        case 128: launch_phases_bs<R, 128>(M, S, A, phase, st); break;
+        case 512: launch_phases_bs<R, 512>(M, S, A, phase, st); break;
+        default: launch_phases_bs<R, 256>(M, S, A, phase, st); break;
+    }
+}
+
+template <class R, int BS>
+static const void* substep_fn() { return reinterpret_cast<const void*>(&k_substep<R, BS>); }
+
+template <class R>
+const void* substep_kernel(int bs) {
+    switch (bs) {
+        case 64: return substep_fn<R, 64>();
+        case 128: return substep_fn<R, 128>();
+        case 512: return substep_fn<R, 512>();
+        default: return substep_fn<R, 256>();
+    }
 }
 
 template <class R>
-void launch_rate(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st) {
-    const int nbc = (M.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(k_rate<R>, dim3(nbc), dim3(BLOCK), 0, st, M, S, A);
+hipError_t launch_substep(const DevMap& M, const DevState<R>& S, const StepArgs& A, int bs, int grid,
+                          hipStream_t st) {
+    const int ntc = tiles_cells(M, bs), ntp = tiles_paths(M, A, bs);
+    int ntc_o = ntc, nt_o = ntc + ntp, ntc_r = ntc, ntc_u = ntc, nt_u = ntc + ntp;
+    DevMap m = M;
+    DevState<R> s = S;
+    StepArgs a = A;
+    void* args[] = {&m, &s, &a, &ntc_o, &nt_o, &ntc_r, &ntc_u, &nt_u};
+    return hipLaunchCooperativeKernel(substep_kernel<R>(bs), dim3(grid), dim3(bs), args, 0, st);
 }
 
 template <class R>
-void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st) {
-    const int nbc = (M.n + BLOCK - 1) / BLOCK;
-    const int nbp = A.lpthout ? (M.npth + BLOCK - 1) / BLOCK : 0;
-    hipLaunchKernelGGL(k_update<R>, dim3(nbc + nbp), dim3(BLOCK), 0, st, M, S, A, nbc);
+void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st) {
+    hipLaunchKernelGGL(k_swapfix<R>, dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S);
 }
 
-void launch_budget(const DevMap& M, const double* partials, int nrun, double* glb_hist, hipStream_t st) {
-    const int nbc = (M.n + BLOCK - 1) / BLOCK;
-    hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(BLOCK), 0, st, partials, nbc, glb_hist);
+template <class R>
+void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* base, const double* factor,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_scale_runoff<R>, dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, base, factor);
 }
 
-#define CMF_INSTANTIATE(R)                                                                              \
-    template void launch_stage<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, double,      \
-                                  hipStream_t);                                                         \
-    template void launch_outflow<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);   \
-    template void launch_rate<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);      \
-    template void launch_update<R>(const DevMap&, const DevState<R>&, const StepArgs&, hipStream_t);
+void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(RBS), 0, st, partials, ntiles, glb_hist);
+}
+
+#define CMF_INSTANTIATE(R)                                                                                     \
+    template void launch_stage<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int, double,        \
+                                  hipStream_t);                                                                \
+    template void launch_phase<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int, hipStream_t); \
+    template const void* substep_kernel<R>(int);                                                               \
+    template hipError_t launch_substep<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int,        \
+                                          hipStream_t);                                                        \
+    template void launch_swapfix<R>(const DevMap&, const DevState<R>&, hipStream_t);                           \
+    template void launch_scale_runoff<R>(const DevMap&, const DevState<R>&, const double*, const double*,     \
+                                         hipStream_t);
 CMF_INSTANTIATE(double)
 CMF_INSTANTIATE(float)
 

@@ -4,16 +4,32 @@
 
 namespace cmf {
 
+// block sizes the phase kernels are built for (the autotuner picks among them)
+constexpr int TUNE_BS[] = {64, 128, 256, 512};
+
+// tiles of one substep phase: cell tiles, then path tiles (paths only with LPTHOUT)
+int tiles_cells(const DevMap& M, int bs);
+int tiles_paths(const DevMap& M, const StepArgs& A, int bs);
+
+// k_stage (+ CFL bound per instance into S.cfl when with_cfl)
 template <class R>
-void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, double dt_def,
+void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst, double dt_def,
                   hipStream_t st);
+// one substep phase as its own kernel: 0 = outflow, 1 = rate, 2 = update
 template <class R>
-void launch_outflow(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st);
+void launch_phase(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, int bs, hipStream_t st);
+// the whole substep as one persistent cooperative kernel of `grid` blocks
 template <class R>
-void launch_rate(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st);
+hipError_t launch_substep(const DevMap& M, const DevState<R>& S, const StepArgs& A, int bs, int grid,
+                          hipStream_t st);
 template <class R>
-void launch_update(const DevMap& M, const DevState<R>& S, const StepArgs& A, hipStream_t st);
-// reduce the per-block budget sums of nrun substeps into glb_hist[nrun][NGLB]
-void launch_budget(const DevMap& M, const double* partials, int nrun, double* glb_hist, hipStream_t st);
+const void* substep_kernel(int bs);
+template <class R>
+void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st);
+template <class R>
+void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* base, const double* factor,
+                         hipStream_t st);
+// reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]
+void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st);
 
 }  // namespace cmf

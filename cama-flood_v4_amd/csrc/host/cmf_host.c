@@ -374,3 +374,33 @@ int cmfh_upstream_area(int nx, int ny, const int32_t* nextx, const int32_t* next
     free(indeg); free(queue);
     return 0;
 }
+
+int cmfh_accumulate(int n, const int32_t* next, int nrec, const double* val, double* out)
+{
+    /* Kahn order over the vector network (next < 0 = mouth), so any cell order works */
+    int32_t* indeg = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!indeg || !order) { free(indeg); free(order); return -2; }
+    for (int i = 0; i < n; ++i) {
+        if (next[i] >= n) { free(indeg); free(order); return -1; }
+        if (next[i] >= 0) indeg[next[i]] += 1;
+    }
+    int qh = 0, qt = 0;
+    for (int i = 0; i < n; ++i) if (indeg[i] == 0) order[qt++] = i;
+    while (qh < qt) {
+        const int i = order[qh++];
+        if (next[i] >= 0 && --indeg[next[i]] == 0) order[qt++] = next[i];
+    }
+    if (qt != n) { free(indeg); free(order); return -3; }      /* not a forest */
+    for (int r = 0; r < nrec; ++r) {
+        const double* v = val + (size_t)r * n;
+        double* o = out + (size_t)r * n;
+        for (int i = 0; i < n; ++i) o[i] = v[i];
+        for (int k = 0; k < n; ++k) {
+            const int i = order[k];
+            if (next[i] >= 0) o[next[i]] += o[i];
+        }
+    }
+    free(indeg); free(order);
+    return 0;
+}

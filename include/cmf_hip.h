@@ -105,6 +105,24 @@ enum { CMF_GLB_STOPRE, CMF_GLB_STONXT, CMF_GLB_STONEW, CMF_GLB_RIVINF, CMF_GLB_R
 /* Create a handle on the current HIP device. No reference counterpart (the
  * reference keeps this state in module globals). */
 int cmf_hip_create(const cmf_dims* dims, const cmf_options* opts, cmf_handle* out);
+
+/* Batched ensembles (no reference counterpart: the reference runs one model per
+ * process). ndom regional domains are concatenated in the map passed to
+ * cmf_hip_set_map (domain d owns the next dom_cells[d] NSEQ cells; I1NEXT, paths
+ * and dams stay inside a domain) and every domain runs `members` realisations
+ * (a power of two, 1..64). Instance k = d*members + m has its own adaptive time
+ * step. Per-cell arrays of cmf_state and cmf_hip_set_runoff then hold `members`
+ * planes of NSEQALL values (member-major); path arrays hold `members` planes of
+ * NPTHLEV*NPTHOUT (or NPTHOUT). dims holds the totals over all domains.
+ * cmf_hip_create(d, o, h) is cmf_hip_create_ensemble(d, o, NULL, h): one domain,
+ * one member. */
+typedef struct cmf_ensemble {
+    int32_t ndom;
+    const int32_t* dom_cells;          /* [ndom] */
+    int32_t members;
+} cmf_ensemble;
+int cmf_hip_create_ensemble(const cmf_dims* dims, const cmf_options* opts, const cmf_ensemble* ens,
+                            cmf_handle* out);
 void cmf_hip_destroy(cmf_handle h);
 const char* cmf_hip_last_error(void);
 
@@ -122,6 +140,9 @@ int cmf_hip_get_state(cmf_handle h, cmf_state* st);
 
 /* Runoff forcing D2RUNOFF [m3/s] (written by CMF_FORCING_PUT). */
 int cmf_hip_set_runoff(cmf_handle h, const double* runoff);
+/* Ensemble forcing: D2RUNOFF of instance k = base[cell] * factor[k] (base holds
+ * NSEQALL values, factor ndom*members). No reference counterpart. */
+int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* factor);
 
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);
@@ -136,6 +157,36 @@ int cmf_hip_physics_advance(cmf_handle h, double* dt, int32_t* nt_out,
  * Same as cmf_hip_physics_advance otherwise. No reference counterpart. */
 int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap,
                                    int32_t* nt_out, double* glb, int32_t glb_cap);
+
+/* One outer step of length dt of every instance of an ensemble handle (also
+ * accepted for a single model). nt_out[k] / dt_out[k] (either may be NULL)
+ * receive NT and DT of instance k; nt_cap > 0 caps the substeps (timing). No
+ * budget sums are kept for more than one instance. No reference counterpart. */
+int cmf_hip_ensemble_advance(cmf_handle h, double dt, int32_t nt_cap, int32_t* nt_out, double* dt_out);
+
+/* With kernel timing enabled: milliseconds from the start of the last advance to
+ * the end of the last substep of each instance (ms[ndom*members]). */
+int cmf_hip_instance_times(cmf_handle h, double* ms);
+
+/* Launch configuration (no reference counterpart). Block sizes 64/128/256/512
+ * per substep phase; fused = 1 runs a substep as one persistent cooperative
+ * kernel of grid_fused blocks (0 = all co-resident blocks) of bs_fused threads;
+ * xcd = 1 gives each XCD a contiguous range of tiles; graph = 1 replays the
+ * substep launches of an outer step from a captured HIP graph (phase kernels
+ * only). The configuration changes no state value, only the rounding of the
+ * logged budget sums. */
+typedef struct cmf_tune {
+    int32_t bs_outflow, bs_rate, bs_update, fused, bs_fused, grid_fused, xcd, graph;
+} cmf_tune;
+int cmf_hip_set_tune(cmf_handle h, const cmf_tune* t);
+int cmf_hip_get_tune(cmf_handle h, cmf_tune* t);
+/* Time nsub substeps of an outer step of length dt under each candidate
+ * configuration, keep the fastest (also written to *best if not NULL) and
+ * restore the model state. report (optional) receives 8 values per trial:
+ * fused, block size, xcd, fused grid, outflow/rate/update ms, total ms.
+ * Returns the number of trials (>= 0) or a negative CMF_E* code. */
+int cmf_hip_autotune(cmf_handle h, double dt, int32_t nsub, cmf_tune* best, double* report,
+                     int32_t report_cap);
 
 /* Wait for all work of the handle (no reference counterpart). */
 int cmf_hip_synchronize(cmf_handle h);

@@ -78,6 +78,13 @@ int cmfh_priority_flood(int nx, int ny, const float* elev, const uint8_t* land, 
 int cmfh_upstream_area(int nx, int ny, const int32_t* nextx, const int32_t* nexty,
                        const double* cellarea, double* uparea);
 
+/* Upstream accumulation on the vector network: out[r][i] = val[r][i] plus the
+ * values of every cell draining to i, for nrec records of n cells. next[i] is
+ * the 0-based downstream cell (< 0 at mouths). Used for the runoff climatology
+ * of the spin-up surrogate (no reference counterpart). Returns 0, or -3 if the
+ * network has a cycle. */
+int cmfh_accumulate(int n, const int32_t* next, int nrec, const double* val, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,3 +87,25 @@ def rel_err(a, b, floor=1e-30):
     b = np.asarray(b, np.float64)
     scale = max(float(np.max(np.abs(b))) if b.size else 0.0, floor)
     return float(np.max(np.abs(a - b))) / scale if a.size else 0.0
+
+
+def prepare_domain(nx, ny, seed, lpth=True, llev=True, ldam=True, levfrac=0.2, pdstmth=10000.0):
+    """(map, network, initial state) of one regional domain for the ensemble
+    tests; every domain of one ensemble uses the same switches."""
+    from oracle.oracle import OracleModel
+    net = synth.make_network(nx, ny, seed=seed)
+    if lpth:
+        synth.add_bifurcations(net, max(10, net.ncells // 20), 5, seed)
+    if llev:
+        synth.add_levees(net, levfrac, seed, hmin=0.5, hmax=3.0, fmin=0.05, fmax=0.5)
+    if ldam:
+        synth.add_dams(net, max(3, net.ncells // 100), seed)
+    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam, pdstmth=pdstmth)
+    m = maps.build_map(net, opts)
+    st = maps.cold_start(m)
+    om = OracleModel(m, st)
+    om.physics_fldstg()
+    st = {k: v.copy() for k, v in om.st.items()}
+    if ldam:
+        maps.init_dams(m, net, st)
+    return m, net, st

@@ -12,10 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(cmfh?_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(cmf[h]?_[a-z0-9_]+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("header,lib", [("cmf_hip.h", "libcmf_hip.so"), ("cmf_host.h", "libcmf_host.so")])
+@pytest.mark.parametrize("header,lib", [("cmf_hip.h", "libcmf_hip.so"), ("cmf_mlp.h", "libcmf_hip.so"),
+                                        ("cmf_host.h", "libcmf_host.so")])
 def test_exports(header, lib):
     path = os.path.join(ROOT, "cama-flood_v4_amd", "cmf", lib)
     assert os.path.exists(path), f"{lib} not built"
@@ -29,4 +30,5 @@ def test_exports(header, lib):
 def test_python_bindings_match_headers():
     from cmf import _abi
     assert set(declared("cmf_hip.h")) == set(_abi.HIP_FUNCS)
+    assert set(declared("cmf_mlp.h")) == set(_abi.MLP_FUNCS)
     assert set(declared("cmf_host.h")) == set(_abi.HOST_FUNCS)
