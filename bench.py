@@ -67,13 +67,23 @@ def device_model(net, m, precision):
     return gm
 
 
-def run_gpu(net, m, rvec, precision, steps, warmup, dt):
+def run_gpu(net, m, rvec, precision, steps, warmup, dt, tune="auto"):
     gm = device_model(net, m, precision)
     rec = 0
     for _ in range(warmup):
         gm.set_runoff(rvec[rec % len(rvec)]); rec += 1
         gm.physics_advance(dt)
     gm.synchronize()
+    if tune == "auto":
+        gm.set_runoff(rvec[rec % len(rvec)])
+        best, trials = gm.autotune(dt, nsub=12)       # restores the state it times on
+        for r in trials:
+            log(f"  trial fused={int(r[0])} bs={int(r[1])} xcd={int(r[2])} grid={int(r[3])}: "
+                f"outflow {r[4]:.3f} rate {r[5]:.3f} update {r[6]:.3f} total {r[7]:.3f} ms / 12 substeps")
+        log(f"{precision}: autotuned launch configuration {best}")
+    elif tune:
+        gm.set_tune(**tune)
+    tune_used = gm.get_tune()
     # timed region: runoff for each step is uploaded before the clock starts
     sum_nt = 0
     t_total = 0.0
@@ -91,7 +101,7 @@ def run_gpu(net, m, rvec, precision, steps, warmup, dt):
     gm.physics_advance(dt)
     kt = gm.kernel_times(False)
     gm.close()
-    return sum_nt, t_total, kt
+    return sum_nt, t_total, kt, tune_used
 
 
 def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
@@ -262,7 +272,12 @@ def ensemble_section(a):
 
 def roofline(kt, n_cells, bytes_tab):
     steps = max(kt[4], 1.0)
-    per = {"outflow": kt[0] / steps, "rate": kt[1] / steps, "update": kt[2] / steps}
+    if kt[0] == 0.0 and kt[1] == 0.0:
+        # fused persistent substep: one kernel carries all three phases
+        per = {"substep": kt[2] / steps}
+        bytes_tab = dict(bytes_tab, substep=sum(bytes_tab.values()))
+    else:
+        per = {"outflow": kt[0] / steps, "rate": kt[1] / steps, "update": kt[2] / steps}
     dom = max(per, key=per.get)
     ms = per[dom]
     achieved = bytes_tab[dom] * n_cells / (ms * 1e-3) / 1e9
@@ -282,12 +297,23 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     ap.add_argument("--ens-days", type=int, default=3)
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 3 x 256 MiB")
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding is not "
               "part of this build", file=sys.stderr)
@@ -297,7 +323,7 @@ def main():
     rvec = [m.vec(ro[r]) for r in range(ro.shape[0])]
     n = m.n
     log("FP64 run")
-    sum_nt, t, kt = run_gpu(net, m, rvec, "fp64", a.steps, a.warmup, dt)
+    sum_nt, t, kt, tune64 = run_gpu(net, m, rvec, "fp64", a.steps, a.warmup, dt)
     value = n * sum_nt / t
     dom, per, roof = roofline(kt, n, BYTES_FP64)
     log(f"FP64: {value:.4g} cell-substeps/s, sum NT {sum_nt}, {t:.3f}s; kernels ms {per}")
@@ -306,7 +332,7 @@ def main():
         from cmf import maps
         os.environ["CMF_HIP_ORDER"] = "nseq"
         m3 = maps.build_map(net, maps.Options(llevee=True, ldamout=True))
-        s3, t3, kt3 = run_gpu(net, m3, rvec, "fp64", a.steps, a.warmup, dt)
+        s3, t3, kt3, _ = run_gpu(net, m3, rvec, "fp64", a.steps, a.warmup, dt, tune=tune64)
         del os.environ["CMF_HIP_ORDER"]
         order_cmp = dict(value=n * s3 / t3, kernels_ms=roofline(kt3, n, BYTES_FP64)[1])
         log(f"FP64 NSEQ order: {order_cmp['value']:.4g} cell-substeps/s; kernels ms {order_cmp['kernels_ms']}")
@@ -315,10 +341,10 @@ def main():
         net2, m2, _ = net, None, None
         from cmf import maps
         m2 = maps.build_map(net, maps.Options(llevee=True, ldamout=True))
-        sum_nt2, t2, kt2 = run_gpu(net, m2, rvec, "mixed", a.steps, a.warmup, dt)
+        sum_nt2, t2, kt2, tune32 = run_gpu(net, m2, rvec, "mixed", a.steps, a.warmup, dt)
         dom2, per2, roof2 = roofline(kt2, n, BYTES_MIXED)
         mixed = dict(value=n * sum_nt2 / t2, unit="cell-substeps/s", sum_nt=sum_nt2,
-                     ms_per_step=1e3 * t2 / a.steps, roofline=roof2)
+                     ms_per_step=1e3 * t2 / a.steps, roofline=roof2, tune=tune32)
         log(f"mixed: {mixed['value']:.4g} cell-substeps/s; kernels ms {per2}")
     ens = None
     if not a.no_ensemble:
@@ -343,7 +369,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"glb_06min-shaped synthetic network {a.nx}x{a.ny}, {n} cells, "
                                f"3% levees, 2500 dams, hourly runoff, DT=3600 s adaptive",
-                   "cells": n, "sum_nt": sum_nt, "precision": "fp64"},
+                   "cells": n, "sum_nt": sum_nt, "precision": "fp64", "tune": tune64},
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,

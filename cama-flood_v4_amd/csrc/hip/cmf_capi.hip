@@ -430,7 +430,7 @@ bool valid_bs(int bs) { return bs == 64 || bs == 128 || bs == 256 || bs == 512; 
 template <class R>
 int coop_capacity(cmf_hip_ctx* h, int bs, int* out) {
     int per_cu = 0, cus = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, substep_kernel<R>(bs), bs, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, substep_kernel<R>(h->M, bs), bs, 0));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     *out = per_cu * cus;
     return 0;

@@ -124,38 +124,55 @@ struct StepArgs {
     int32_t xcd;                               // XCD-contiguous tile order
 };
 
-// Cell / path slot -> static index, member and instance
+// Cell / path slot -> static index, member and instance. MB = false is the
+// single-instance handle (one domain, one member): slot = cell, instance 0,
+// known at compile time.
 struct Slot {
     int c, m, inst;
 };
+template <bool MB>
 __device__ __forceinline__ Slot cell_slot(const DevMap& M, int s) {
     Slot x;
+    if (!MB) { x.c = s; x.m = 0; x.inst = 0; return x; }
     x.c = s >> M.lgm;
     x.m = s & (M.nmem - 1);
     x.inst = (M.cell_dom ? M.cell_dom[x.c] * M.nmem : 0) + x.m;
     return x;
 }
+template <bool MB>
 __device__ __forceinline__ Slot path_slot(const DevMap& M, int q) {
     Slot x;
+    if (!MB) { x.c = q; x.m = 0; x.inst = 0; return x; }
     x.c = q >> M.lgm;
     x.m = q & (M.nmem - 1);
     x.inst = (M.pth_dom ? M.pth_dom[x.c] * M.nmem : 0) + x.m;
     return x;
 }
-__device__ __forceinline__ int sidx(const DevMap& M, int c, int m) { return (c << M.lgm) | m; }
+template <bool MB>
+__device__ __forceinline__ int sidx(const DevMap& M, int c, int m) { return MB ? ((c << M.lgm) | m) : c; }
 
 // ---------------------------------------------------------------------------
 // Floodplain profile table recomputed on the fly: SET_FLDSTG
 // (src/cmf_ctrl_maps_mod.F90:984-995) and, identically, CMF_LEVEE_INIT :169-181.
+struct Geom {                 // float32 map values of one cell, widened
+    double rivlen, rivwth, grarea, rivhgt;
+};
+__device__ __forceinline__ Geom load_geom(const DevMap& M, int i) {
+    Geom g;
+    g.rivlen = (double)M.rivlen[i]; g.rivwth = (double)M.rivwth[i];
+    g.grarea = (double)M.grarea[i]; g.rivhgt = (double)M.rivhgt[i];
+    return g;
+}
+
 struct FldTab {
     const float* h;          // &fldhgt[i], stride n
     int n;
     double rivlen, rivwth, dwthinc, stopre, hpre;
-    __device__ __forceinline__ void init(const DevMap& M, int i, double rivstomax) {
+    __device__ __forceinline__ void init(const DevMap& M, int i, const Geom& g, double rivstomax) {
         h = M.fldhgt + i; n = M.n;
-        rivlen = (double)M.rivlen[i];
-        rivwth = (double)M.rivwth[i];
-        dwthinc = ((double)M.grarea[i] * (1.0 / rivlen)) * M.dfrcinc;
+        rivlen = g.rivlen;
+        rivwth = g.rivwth;
+        dwthinc = (g.grarea * (1.0 / rivlen)) * M.dfrcinc;
         stopre = rivstomax; hpre = 0.0;
     }
     // next layer l (1-based, consecutive calls): FLDSTOMAX(l), FLDGRD(l), FLDHGT(l)
@@ -173,6 +190,9 @@ __device__ __forceinline__ double rivstomax_of(const DevMap& M, int i) {
     // maps:716-721
     return M.lfplain ? ((double)M.rivlen[i] * (double)M.rivwth[i]) * (double)M.rivhgt[i] : K_1E18F;
 }
+__device__ __forceinline__ double rivstomax_of(const DevMap& M, const Geom& g) {
+    return M.lfplain ? (g.rivlen * g.rivwth) * g.rivhgt : K_1E18F;
+}
 __device__ __forceinline__ double rivelv_of(const DevMap& M, int i) {
     return (double)M.elevtn[i] - (double)M.rivhgt[i];            // maps:722
 }
@@ -183,16 +203,17 @@ struct Stage {
 };
 
 // CMF_CALC_FLDSTG_DEF per cell, src/cmf_calc_fldstg_mod.F90:49-95
-__device__ __forceinline__ void stage_def(const DevMap& M, int i, double rivsto, double fldsto, Stage& o) {
+__device__ __forceinline__ void stage_def(const DevMap& M, int i, const Geom& geo, double rivsto, double fldsto,
+                                          Stage& o) {
     const double pstoall = rivsto + fldsto;
     const double dstoall = pstoall;
-    const double rivstomax = rivstomax_of(M, i);
-    const double rivlen = (double)M.rivlen[i], rivwth = (double)M.rivwth[i];
+    const double rivstomax = rivstomax_of(M, geo);
+    const double rivlen = geo.rivlen, rivwth = geo.rivwth;
     o.levsto = 0.0; o.levdph = 0.0;
     if (pstoall > rivstomax) {
         const int nl = M.nlfp;
-        FldTab T; T.init(M, i, rivstomax);
-        const double dwth_inc = ((double)M.grarea[i] / rivlen) * M.dfrcinc;
+        FldTab T; T.init(M, i, geo, rivstomax);
+        const double dwth_inc = (geo.grarea / rivlen) * M.dfrcinc;
         int I = 1;
         double dsto_fil = rivstomax, dwth_fil = rivwth, ddph_fil = 0.0;
         double smax, grd, hgt;
@@ -225,7 +246,7 @@ __device__ __forceinline__ void stage_def(const DevMap& M, int i, double rivsto,
         fr = vmax(fr, 0.0);
         fr = vmin(fr, 1.0);
         o.fldfrc = fr;
-        o.fldare = (double)M.grarea[i] * fr;
+        o.fldare = geo.grarea * fr;
     } else {                                          // :87-95
         o.rivsto = pstoall;
         const double d = (dstoall / rivlen) / rivwth;
@@ -234,25 +255,34 @@ __device__ __forceinline__ void stage_def(const DevMap& M, int i, double rivsto,
     }
 }
 
+// levee parameters of one cell (sentinels of CMF_LEVEE_INIT for cells without one)
+struct Lev {
+    double levhgt, levfrc, bashgt, levdst, levbassto, levtopsto, levfilsto;
+};
+__device__ __forceinline__ Lev load_lev(const DevMap& M, int i) {
+    Lev L{0.0, 1.0, K_LEVSENT, K_LEVSENT, K_LEVSENT, K_LEVSENT, K_LEVSENT};
+    const int li = M.lev_idx[i];
+    const int nlc = M.nlevcell;
+    if (li >= 0) {
+        L.levhgt = M.lev_tab[li]; L.levfrc = M.lev_tab[nlc + li]; L.bashgt = M.lev_tab[2 * nlc + li];
+        L.levdst = M.lev_tab[3 * nlc + li]; L.levbassto = M.lev_tab[4 * nlc + li];
+        L.levtopsto = M.lev_tab[5 * nlc + li]; L.levfilsto = M.lev_tab[6 * nlc + li];
+    }
+    return L;
+}
+
 // CMF_LEVEE_FLDSTG per cell, src/cmf_ctrl_levee_mod.F90:302-473. Every case
 // computes the same eight local values and the result is written once, so the
 // compiler keeps the whole inversion in registers.
-__device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivsto, double fldsto,
-                                            double levsto, Stage& o) {
+__device__ __forceinline__ void stage_levee(const DevMap& M, int i, const Geom& geo, const Lev& L, double rivsto,
+                                            double fldsto, double levsto, Stage& o) {
     const int nl = M.nlfp;
-    const int li = M.lev_idx[i];
-    const int nlc = M.nlevcell;
-    double levhgt = 0.0, levfrc = 1.0, bashgt = K_LEVSENT, levdst = K_LEVSENT;
-    double levbassto = K_LEVSENT, levtopsto = K_LEVSENT, levfilsto = K_LEVSENT;
-    if (li >= 0) {
-        levhgt = M.lev_tab[li]; levfrc = M.lev_tab[nlc + li]; bashgt = M.lev_tab[2 * nlc + li];
-        levdst = M.lev_tab[3 * nlc + li]; levbassto = M.lev_tab[4 * nlc + li];
-        levtopsto = M.lev_tab[5 * nlc + li]; levfilsto = M.lev_tab[6 * nlc + li];
-    }
+    const double levhgt = L.levhgt, levfrc = L.levfrc, bashgt = L.bashgt, levdst = L.levdst;
+    const double levbassto = L.levbassto, levtopsto = L.levtopsto, levfilsto = L.levfilsto;
     const double pstoall = (rivsto + fldsto) + levsto;
     const double dstoall = pstoall;
-    const double rivstomax = rivstomax_of(M, i);
-    const double rl = (double)M.rivlen[i], rw = (double)M.rivwth[i], ga = (double)M.grarea[i];
+    const double rivstomax = rivstomax_of(M, geo);
+    const double rl = geo.rivlen, rw = geo.rivwth, ga = geo.grarea;
     const double dwth_inc = (ga / rl) * M.dfrcinc;
     double r_riv = 0.0, r_fld = 0.0, r_lev = 0.0, r_rdph = 0.0, r_fdph = 0.0, r_frc = 0.0, r_are = 0.0,
            r_ldph = 0.0;
@@ -266,7 +296,7 @@ __device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivst
     else kase = 4;
     if (kase == 1 || kase == 4) {
         // case 1 (:312-349) and case 4 (:424-461) share the layer search
-        FldTab T; T.init(M, i, rivstomax);
+        FldTab T; T.init(M, i, geo, rivstomax);
         int I = 1;
         double dsto_fil = rivstomax, dwth_fil = rw, ddph_fil = 0.0, dwth_add;
         double smax, grd, hgt;
@@ -330,7 +360,7 @@ __device__ __forceinline__ void stage_levee(const DevMap& M, int i, double rivst
         r_lev = vmax(ls, 0.0);
         const int ilev = (int)(levfrc * (double)nl) + 1;
         double dsto_fil = levtopsto, dwth_fil = 0.0, ddph_fil = 0.0;
-        FldTab T; T.init(M, i, rivstomax);
+        FldTab T; T.init(M, i, geo, rivstomax);
         double smax = 0.0, grd = 0.0, hgt = 0.0;
         for (int l = 1; l < ilev && l <= nl; ++l) T.layer(l, smax, grd, hgt);
         int I = ilev;

@@ -77,31 +77,44 @@ __device__ __forceinline__ InstView inst_view(const InstStep* tab, int inst, int
 
 // ---------------------------------------------------------------------------
 // phase 1a: river / mouth / floodplain outflow and dam operation, per cell slot
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
-    const Slot x = cell_slot(M, s);
+    const Slot x = cell_slot<MB>(M, s);
     const InstView I = inst_view(S.inst, x.inst, A.it);
     if (!I.active) return;
     const int i = x.c, m = x.m;
-    const R gdt = R(I.gdt);
-    const R zero = R(0), one = R(1);
-    const R rivelv = R(rivelv_of(M, i));
+    // every load that does not depend on the downstream index is issued first
+    const int nxt = M.next[i];
     const R rivdph = S.rivdph[s], rivdph_pre = S.rivdph_pre[s];
-    const R sfcelv = rivelv + rivdph;                               // outflw:45-47
-    const R sfcelv_pre = rivelv + rivdph_pre;
-    const R flddph_pre = vmax(rivdph_pre - R(M.rivhgt[i]), zero);
-    const R rivwth = R(M.rivwth[i]), rivman = R(M.rivman[i]);
     const R qr_pre = S.rivout[s];                                   // D2RIVOUT_PRE
     const R qr_old = S.rivout_vel[s];                               // D2RIVOUT (for RIVVEL)
-    const int nxt = M.next[i];
+    const float f_elevtn = M.elevtn[i], f_rivhgt = M.rivhgt[i], f_rivwth = M.rivwth[i], f_rivman = M.rivman[i];
+    const float f_nxtdst = M.nxtdst[i], f_rivlen = M.rivlen[i];
+    const R p_rivvel_avg = S.rivvel_avg[s];
+    const R storge = S.storge[s];
+    double fldsto_d = 0.0;
+    R flddph = R(0), fldsto_pre = R(0), fldout_pre = R(0);
+    if (A.lfldout) {
+        fldsto_d = S.fldsto[s];
+        flddph = S.flddph[s];
+        fldsto_pre = S.fldsto_pre[s];
+        fldout_pre = S.fldout[s];                                   // D2FLDOUT_PRE
+    }
+    const R gdt = R(I.gdt);
+    const R zero = R(0), one = R(1);
+    const R rivelv = R((double)f_elevtn - (double)f_rivhgt);        // maps:722
+    const R sfcelv = rivelv + rivdph;                               // outflw:45-47
+    const R sfcelv_pre = rivelv + rivdph_pre;
+    const R flddph_pre = vmax(rivdph_pre - R(f_rivhgt), zero);
+    const R rivwth = R(f_rivwth), rivman = R(f_rivman);
     const bool river = nxt >= 0;                                     // mouths carry a negative code
     R dwn, dwn_pre, rdst;
     if (river) {                                                     // outflw:52-57
-        const int j = nxt, sj = sidx(M, j, m);
+        const int j = nxt, sj = sidx<MB>(M, j, m);
         const R rivelv_j = R(rivelv_of(M, j));
         dwn = rivelv_j + S.rivdph[sj];
         dwn_pre = rivelv_j + S.rivdph_pre[sj];
-        rdst = one / R(M.nxtdst[i]);
+        rdst = one / R(f_nxtdst);
     } else {
         dwn = R(M.dwnelv[i]);
         dwn_pre = dwn;
@@ -137,10 +150,9 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     // ---- floodplain: outflw:84-116 (river), :143-171 (mouth)
     if (A.lfldout) {
         const R gdtn2 = gdt * (R(A.pmanfld) * R(A.pmanfld));
-        const R dfsto = R(S.fldsto[s]);
-        const R elevtn = R(M.elevtn[i]);
-        const R rrl = one / R(M.rivlen[i]);
-        const R flddph = S.flddph[s];
+        const R dfsto = R(fldsto_d);
+        const R elevtn = R(f_elevtn);
+        const R rrl = one / R(f_rivlen);
         R dslp = (sfcelv - dwn) * rdst;
         dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
         R dflw, dflw_pr, dare, dare_pr;
@@ -151,20 +163,20 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
             dare = vmax(dare - flddph * rivwth, zero);
             const R dsfc_pr = vmax(sfcelv_pre, dwn_pre);
             dflw_pr = dsfc_pr - elevtn;
-            dare_pr = S.fldsto_pre[s] * rrl;
+            dare_pr = fldsto_pre * rrl;
             dare_pr = vmax(dare_pr - flddph_pre * rivwth, R(1.e-6));
         } else {
             dflw = sfcelv - elevtn;
             dare = vmax(dfsto * rrl - flddph * rivwth, zero);
             dflw_pr = sfcelv_pre - elevtn;
-            dare_pr = vmax(S.fldsto_pre[s] * rrl - flddph_pre * rivwth, R(1.e-6));
+            dare_pr = vmax(fldsto_pre * rrl - flddph_pre * rivwth, R(1.e-6));
         }
         const R t = dflw * dflw_pr;
         R dflw_im = vsqrt(vmax(t, zero));
         dflw_im = vmax(dflw_im, R(1.e-6));
         R dare_im = vsqrt(dare * dare_pr);
         dare_im = vmax(dare_im, R(1.e-6));
-        const R dout_pr = S.fldout[s];                              // D2FLDOUT_PRE
+        const R dout_pr = fldout_pre;
         const R dout = (dout_pr + (gdt * dare_im) * dslp) *
                        (one / (one + ((gdtn2 * vabs(dout_pr)) * vpow(dflw_im, R(K_P4O3))) * (one / dare_im)));
         const R thr = river ? R(1.e-5) : R(K_1EM5F);
@@ -176,7 +188,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     if (river) {
         R dout = ((-q_r) - q_f) * R(I.dt);
         dout = vmax(dout, R(K_1EM10F));
-        R rate = (R(0.05) * S.storge[s]) / dout;
+        R rate = (R(0.05) * storge) / dout;
         rate = vmin(rate, one);
         q_r = q_r * rate;
         q_f = q_f * rate;
@@ -191,7 +203,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
                 const int u = M.up_idx[k];
                 const int cu = M.i1dam[u];
                 if (cu == 10 || cu == 11) {
-                    const int su = sidx(M, u, m);
+                    const int su = sidx<MB>(M, u, m);
                     inf = (inf + (double)S.rivout[su]) + (double)S.fldout[su];
                 }
             }
@@ -248,15 +260,15 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     S.qf_u[s] = q_f;
     if (I.last) S.rivvel[s] = vel;                                  // output-only diagnostic
     // CMF_DIAG_AVEMAX_ADPSTP (diag:113) for RIVVEL, final after OUTFLW/DAMOUT
-    R va = (I.first ? zero : S.rivvel_avg[s]) + vel * R(I.dt);
+    R va = (I.first ? zero : p_rivvel_avg) + vel * R(I.dt);
     if (I.last) va = va / R(I.nadd);
     S.rivvel_avg[s] = va;
 }
 
 // phase 1b: bifurcation path flow, pthout:20-97 / levee:497-603
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int q) {
-    const Slot x = path_slot(M, q);
+    const Slot x = path_slot<MB>(M, q);
     const InstView I = inst_view(S.inst, x.inst, A.it);
     if (!I.active) return;
     const int p = x.c, m = x.m;
@@ -269,7 +281,7 @@ __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>&
         S.pthflwsum_u[q] = R(0);
         return;
     }
-    const int si_ = sidx(M, ip, m), sj_ = sidx(M, jp, m);
+    const int si_ = sidx<MB>(M, ip, m), sj_ = sidx<MB>(M, jp, m);
     const R zero = R(0), one = R(1);
     const R gdt = R(I.gdt);
     const R ri = R(rivelv_of(M, ip)), rj = R(rivelv_of(M, jp));
@@ -321,24 +333,24 @@ __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>&
 
 // ---------------------------------------------------------------------------
 // phase 2: outflow volume and limiter rate, inflow_lspamat:410-451
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
-    const Slot x = cell_slot(M, s);
+    const Slot x = cell_slot<MB>(M, s);
     const InstView I = inst_view(S.inst, x.inst, A.it);
     if (!I.active) return;
     const int i = x.c, m = x.m;
     const double dt = I.dt;
     double so = (double)vmax(S.qr_u[s], R(0)) + (double)vmax(S.qf_u[s], R(0));
     for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {
-        const int su = sidx(M, M.up_idx[k], m);
+        const int su = sidx<MB>(M, M.up_idx[k], m);
         so = (so + (double)vmax(-S.qr_u[su], R(0))) + (double)vmax(-S.qf_u[su], R(0));
     }
     so = so * dt;
     if (A.lpthout) {
         for (int k = M.pout_ptr[i]; k < M.pout_ptr[i + 1]; ++k)
-            so = so + (double)vmax(S.pthflwsum_u[sidx(M, M.pout_idx[k], m)], R(0)) * dt;
+            so = so + (double)vmax(S.pthflwsum_u[sidx<MB>(M, M.pout_idx[k], m)], R(0)) * dt;
         for (int k = M.pinf_ptr[i]; k < M.pinf_ptr[i + 1]; ++k)
-            so = so + (double)vmax(-S.pthflwsum_u[sidx(M, M.pinf_idx[k], m)], R(0)) * dt;
+            so = so + (double)vmax(-S.pthflwsum_u[sidx<MB>(M, M.pinf_idx[k], m)], R(0)) * dt;
     }
     const R dout = vmax(R(so), R(1.e-10));
     const R dsto = R(S.rivsto[s] + S.fldsto[s]);
@@ -347,33 +359,52 @@ __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S,
 
 // ---------------------------------------------------------------------------
 // phase 3: limit, gather inflow, dam balance, storage update, stage, averages
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ R limited_path_sum(const DevMap& M, const DevState<R>& S, int p, int m) {
-    const R s = S.pthflwsum_u[sidx(M, p, m)];
+    const R s = S.pthflwsum_u[sidx<MB>(M, p, m)];
     const int ip = M.pth_upst[p], jp = M.pth_down[p];
     if (M.i2mask[ip] > 0 || M.i2mask[jp] > 0) return s;               // inflow_lspamat:501
-    return (s >= R(0)) ? s * S.rate[sidx(M, ip, m)] : s * S.rate[sidx(M, jp, m)];
+    return (s >= R(0)) ? s * S.rate[sidx<MB>(M, ip, m)] : s * S.rate[sidx<MB>(M, jp, m)];
 }
 
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A,
                                             int s, double (&g)[NGLB]) {
-    const Slot x = cell_slot(M, s);
+    const Slot x = cell_slot<MB>(M, s);
     const InstView I = inst_view(S.inst, x.inst, A.it);
     if (!I.active) return;
     const int i = x.c, m = x.m;
     const double dt = I.dt;
     const R zero = R(0);
-    // limiter, inflow_lspamat:456-473
+    const bool f0 = I.first;
+    // every load that does not depend on a gathered index is issued first, so it
+    // is in flight while the upstream and downstream gathers resolve
     const R qr = S.qr_u[s], qf = S.qf_u[s];
     const R ri = S.rate[s];
     const int nxt = M.next[i];
-    const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[sidx(M, nxt, m)] : ri;
+    const int k0 = M.up_ptr[i], k1 = M.up_ptr[i + 1];
+    const R runoff = S.runoff[s];
+    const double rs0 = S.rivsto[s], fs0 = S.fldsto[s];
+    const R fldfrc_old = S.fldfrc[s];
+    const double levsto = A.llevee ? S.levsto[s] : 0.0;
+    const Geom geo = load_geom(M, i);
+    const float f_elevtn = M.elevtn[i];
+    Lev lev{};
+    if (A.llevee) lev = load_lev(M, i);
+    const int dam = A.ldamout ? M.dam_of_cell[i] : -1;
+    const R p_ravg = S.rivout_avg[s], p_favg = S.fldout_avg[s], p_oavg = S.outflw_avg[s];
+    const R p_omax = S.outflw_max[s], p_dmax = S.rivdph_max[s], p_smax = S.storge_max[s];
+    const R p_pavg = A.lpthout ? S.pthout_avg[s] : zero;
+    R p_davg = zero;
+    double daminf = 0.0, damsto = 0.0;
+    if (A.ldamout) { p_davg = S.daminf_avg[s]; daminf = S.daminf[s]; }
+    // limiter, inflow_lspamat:456-473
+    const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[sidx<MB>(M, nxt, m)] : ri;
     const R qrl = qr * r, qfl = qf * r;
     // upstream gather, inflow_lspamat:481-489
     double rivinf = 0.0, fldinf = 0.0;
-    for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {
-        const int su = sidx(M, M.up_idx[k], m);
+    for (int k = k0; k < k1; ++k) {
+        const int su = sidx<MB>(M, M.up_idx[k], m);
         const R qu = S.qr_u[su], fu = S.qf_u[su];
         const R ru = (qu > zero) ? S.rate[su] : ri;
         rivinf = rivinf + (double)(qu * ru);
@@ -382,9 +413,9 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     double pthout = 0.0;
     if (A.lpthout) {                                                  // :519-535
         for (int k = M.pout_ptr[i]; k < M.pout_ptr[i + 1]; ++k)
-            pthout = pthout + (double)limited_path_sum<R>(M, S, M.pout_idx[k], m);
+            pthout = pthout + (double)limited_path_sum<R, MB>(M, S, M.pout_idx[k], m);
         for (int k = M.pinf_ptr[i]; k < M.pinf_ptr[i + 1]; ++k)
-            pthout = pthout - (double)limited_path_sum<R>(M, S, M.pinf_idx[k], m);
+            pthout = pthout - (double)limited_path_sum<R, MB>(M, S, M.pinf_idx[k], m);
     }
     // RIVINF/FLDINF/PTHOUT enter the storage update as the FP64 sums of the
     // limited fluxes (inflow_lspamat:538-544 rounds them to JPRB; with R = double
@@ -397,20 +428,17 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const double pout = sizeof(R) == sizeof(double) ? pthout_d : pthout;
     const R pthout_r = R(pthout);
     if (I.last) { S.rivinf[s] = rivinf; S.fldinf[s] = fldinf; S.pthout[s] = pthout; }
-    const R runoff = S.runoff[s];
     // CMF_DAMOUT_WATBAL, damout:521-536
-    if (A.ldamout) {
-        const int d = M.dam_of_cell[i];
-        if (d >= 0 && M.damstat[d] != -9999) {
-            const double inflow = (rinf + finf) + (double)runoff;
-            const double outflw = (double)qrl + (double)qfl;
-            S.damsto[s] = (S.damsto[s] + inflow * dt) - outflw * dt;
-        }
+    if (dam >= 0 && M.damstat[dam] != -9999) {
+        damsto = S.damsto[s];
+        const double inflow = (rinf + finf) + (double)runoff;
+        const double outflw = (double)qrl + (double)qfl;
+        S.damsto[s] = (damsto + inflow * dt) - outflw * dt;
     }
     // CALC_VARS_PRE, physics:254-276. RIVOUT_PRE/FLDOUT_PRE share the RIVOUT/FLDOUT
     // arrays; RIVDPH/RIVDPH_PRE are a buffer pair whose roles the host swaps after
     // this kernel, so the new depth goes to the buffer of the old RIVDPH_PRE.
-    double rs = S.rivsto[s], fs = S.fldsto[s];
+    double rs = rs0, fs = fs0;
     S.rivout[s] = qrl;
     S.fldout[s] = qfl;
     S.fldsto_pre[s] = R(fs);
@@ -424,19 +452,17 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     if (fs < 0.0) { rs = vmax(rs + fs, 0.0); fs = 0.0; }
     g[G_STONXT] = rs + fs;
     const R outflw = qrl + qfl;
-    const R fldfrc_old = S.fldfrc[s];
     const R drivrof = (runoff * (R(1) - fldfrc_old)) * R(dt);
     const R dfldrof = (runoff * fldfrc_old) * R(dt);
     rs = rs + (double)drivrof;
     fs = fs + (double)dfldrof;
-    double levsto = A.llevee ? S.levsto[s] : 0.0;
     double st = rs + fs;
     if (A.llevee) st = st + levsto;
     g[G_STONEW] = (double)R(st);
     // CMF_PHYSICS_FLDSTG
     Stage o;
-    if (A.llevee) stage_levee(M, i, rs, fs, levsto, o);
-    else stage_def(M, i, rs, fs, o);
+    if (A.llevee) stage_levee(M, i, geo, lev, rs, fs, levsto, o);
+    else stage_def(M, i, geo, rs, fs, o);
     S.rivsto[s] = o.rivsto;
     S.fldsto[s] = o.fldsto;
     const R rivdph = R(o.rivdph);
@@ -445,7 +471,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     S.fldfrc[s] = R(o.fldfrc);
     if (I.last) {                                                   // output-only diagnostics
         S.fldare[s] = R(o.fldare);
-        S.sfcelv[s] = R(rivelv_of(M, i)) + rivdph;
+        S.sfcelv[s] = R((double)f_elevtn - geo.rivhgt) + rivdph;
         S.outflw[s] = outflw;
     }
     double stg = o.rivsto + o.fldsto;
@@ -464,10 +490,9 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     g[G_LEVSTO] = A.llevee ? o.levsto : 0.0;
     // CMF_DIAG_AVEMAX_ADPSTP, diag:108-124, and GETAVE :179-186 on the last substep
     const R rdt = R(dt);
-    const bool f0 = I.first;
-    R a1 = (f0 ? zero : S.rivout_avg[s]) + qrl * rdt;
-    R a2 = (f0 ? zero : S.fldout_avg[s]) + qfl * rdt;
-    R a3 = (f0 ? zero : S.outflw_avg[s]) + outflw * rdt;
+    R a1 = (f0 ? zero : p_ravg) + qrl * rdt;
+    R a2 = (f0 ? zero : p_favg) + qfl * rdt;
+    R a3 = (f0 ? zero : p_oavg) + outflw * rdt;
     if (I.last) {
         const R nadd = R(I.nadd);
         a1 = a1 / nadd; a2 = a2 / nadd; a3 = a3 / nadd;
@@ -479,25 +504,25 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     }
     S.rivout_avg[s] = a1; S.fldout_avg[s] = a2; S.outflw_avg[s] = a3;
     if (A.lpthout) {
-        R a4 = (f0 ? zero : S.pthout_avg[s]) + pthout_r * rdt;
+        R a4 = (f0 ? zero : p_pavg) + pthout_r * rdt;
         if (I.last) a4 = a4 / R(I.nadd);
         S.pthout_avg[s] = a4;
     } else if (f0) {
         S.pthout_avg[s] = zero;                                     // PTHOUT = 0 without paths
     }
-    S.outflw_max[s] = vmax(f0 ? zero : S.outflw_max[s], vabs(outflw));
-    S.rivdph_max[s] = vmax(f0 ? zero : S.rivdph_max[s], rivdph);
-    S.storge_max[s] = vmax(f0 ? zero : S.storge_max[s], storge);
+    S.outflw_max[s] = vmax(f0 ? zero : p_omax, vabs(outflw));
+    S.rivdph_max[s] = vmax(f0 ? zero : p_dmax, rivdph);
+    S.storge_max[s] = vmax(f0 ? zero : p_smax, storge);
     if (A.ldamout) {
-        R da = (f0 ? zero : S.daminf_avg[s]) + R(S.daminf[s] * dt);
+        R da = (f0 ? zero : p_davg) + R(daminf * dt);
         if (I.last) da = da / R(I.nadd);
         S.daminf_avg[s] = da;
     }
 }
 
-template <class R>
+template <class R, bool MB>
 __device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& S, const StepArgs& A, int q) {
-    const Slot x = path_slot(M, q);
+    const Slot x = path_slot<MB>(M, q);
     const InstView I = inst_view(S.inst, x.inst, A.it);
     if (!I.active) return;
     const int p = x.c, m = x.m;
@@ -506,8 +531,8 @@ __device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& 
     const bool lim = !(ip < 0 || jp < 0) && !(M.i2mask[ip] > 0 || M.i2mask[jp] > 0);
     const R zero = R(0);
     const R rdt = R(I.dt);
-    const R rate_i = lim ? S.rate[sidx(M, ip, m)] : zero;
-    const R rate_j = lim ? S.rate[sidx(M, jp, m)] : zero;
+    const R rate_i = lim ? S.rate[sidx<MB>(M, ip, m)] : zero;
+    const R rate_j = lim ? S.rate[sidx<MB>(M, jp, m)] : zero;
     R sumavg = zero;
     for (int l = 0; l < nl; ++l) {
         R qq = S.pthflw_u[(size_t)l * nsp + q];
@@ -528,25 +553,25 @@ __device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& 
 
 // ---------------------------------------------------------------------------
 // tiles: a tile is BS consecutive slots; cell tiles first, then path tiles
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __device__ __forceinline__ void outflow_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t,
                                              int ntc) {
     if (t < ntc) {
         const int s = t * BS + threadIdx.x;
-        if (s < M.ns) outflow_cell<R>(M, S, A, s);
+        if (s < M.ns) outflow_cell<R, MB>(M, S, A, s);
     } else {
         const int q = (t - ntc) * BS + threadIdx.x;
-        if (q < M.nsp) pthflow_path<R>(M, S, A, q);
+        if (q < M.nsp) pthflow_path<R, MB>(M, S, A, q);
     }
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __device__ __forceinline__ void rate_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t) {
     const int s = t * BS + threadIdx.x;
-    if (s < M.ns) rate_cell<R>(M, S, A, s);
+    if (s < M.ns) rate_cell<R, MB>(M, S, A, s);
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t, int ntc,
                                             double* lds) {
     if (t < ntc) {
@@ -554,7 +579,7 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
         double g[NGLB];
 #pragma unroll
         for (int k = 0; k < NGLB; ++k) g[k] = 0.0;
-        if (s < M.ns) update_cell<R>(M, S, A, s, g);
+        if (s < M.ns) update_cell<R, MB>(M, S, A, s, g);
         if (A.budget) {
             block_sum<BS, NGLB>(g, lds);
             // per-substep slot: reduced once for all substeps at the end of the outer step
@@ -564,27 +589,27 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
         }
     } else {
         const int q = (t - ntc) * BS + threadIdx.x;
-        if (q < M.nsp) update_path<R>(M, S, A, q);
+        if (q < M.nsp) update_path<R, MB>(M, S, A, q);
     }
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
-    if (t < nt_all) outflow_tile<R, BS>(M, S, A, t, ntc);
+    if (t < nt_all) outflow_tile<R, BS, MB>(M, S, A, t, ntc);
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_rate(DevMap M, DevState<R> S, StepArgs A, int ntc) {
     const int t = tile_of(blockIdx.x, gridDim.x, ntc, A.xcd);
-    if (t < ntc) rate_tile<R, BS>(M, S, A, t);
+    if (t < ntc) rate_tile<R, BS, MB>(M, S, A, t);
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     __shared__ double lds[NGLB * (BS / 64)];
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
-    if (t < nt_all) update_tile<R, BS>(M, S, A, t, ntc, lds);
+    if (t < nt_all) update_tile<R, BS, MB>(M, S, A, t, ntc, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -629,19 +654,19 @@ __device__ __forceinline__ void tile_range(int b, int G, int ntiles, int xcd, in
     }
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArgs A, int ntc_o, int nt_o,
                                                 int ntc_r, int ntc_u, int nt_u) {
     __shared__ double lds[NGLB * (BS / 64)];
     int t0, dt, te;
     tile_range(blockIdx.x, gridDim.x, nt_o, A.xcd, t0, dt, te);
-    for (int t = t0; t < te; t += dt) outflow_tile<R, BS>(M, S, A, t, ntc_o);
+    for (int t = t0; t < te; t += dt) outflow_tile<R, BS, MB>(M, S, A, t, ntc_o);
     grid_barrier(S.bar, S.err);
     tile_range(blockIdx.x, gridDim.x, ntc_r, A.xcd, t0, dt, te);
-    for (int t = t0; t < te; t += dt) rate_tile<R, BS>(M, S, A, t);
+    for (int t = t0; t < te; t += dt) rate_tile<R, BS, MB>(M, S, A, t);
     grid_barrier(S.bar, S.err);
     tile_range(blockIdx.x, gridDim.x, nt_u, A.xcd, t0, dt, te);
-    for (int t = t0; t < te; t += dt) update_tile<R, BS>(M, S, A, t, ntc_u, lds);
+    for (int t = t0; t < te; t += dt) update_tile<R, BS, MB>(M, S, A, t, ntc_u, lds);
 }
 
 // deterministic second-level reduction of the per-tile budget sums: block b
@@ -667,19 +692,20 @@ __global__ void k_cfl_init(unsigned long long* cfl, int ninst, double dt_def) {
     if (k < ninst) cfl[k] = (unsigned long long)__double_as_longlong(dt_def);
 }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_stage(DevMap M, DevState<R> S, StepArgs A, int with_cfl) {
     const int s = blockIdx.x * BS + threadIdx.x;
     double c = 1.0e300;
     int inst = -1;
     if (s < M.ns) {
-        const Slot x = cell_slot(M, s);
+        const Slot x = cell_slot<MB>(M, s);
         const int i = x.c;
         inst = x.inst;
         Stage o;
         const double levsto = A.llevee ? S.levsto[s] : 0.0;
-        if (A.llevee) stage_levee(M, i, S.rivsto[s], S.fldsto[s], levsto, o);
-        else stage_def(M, i, S.rivsto[s], S.fldsto[s], o);
+        const Geom geo = load_geom(M, i);
+        if (A.llevee) stage_levee(M, i, geo, load_lev(M, i), S.rivsto[s], S.fldsto[s], levsto, o);
+        else stage_def(M, i, geo, S.rivsto[s], S.fldsto[s], o);
         S.rivsto[s] = o.rivsto;
         S.fldsto[s] = o.fldsto;
         const R rivdph = R(o.rivdph);
@@ -722,11 +748,11 @@ __global__ __launch_bounds__(BS) void k_stage(DevMap M, DevState<R> S, StepArgs 
 
 // instances that idled an odd number of substeps hold RIVDPH in the buffer the
 // host now calls RIVDPH_PRE: swap the pair back for their slots
-template <class R>
+template <class R, bool MB>
 __global__ void k_swapfix(DevMap M, DevState<R> S) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M.ns) return;
-    const Slot x = cell_slot(M, s);
+    const Slot x = cell_slot<MB>(M, s);
     if (!S.inst[x.inst].swapfix) return;
     const R a = S.rivdph[s];
     S.rivdph[s] = S.rivdph_pre[s];
@@ -734,67 +760,78 @@ __global__ void k_swapfix(DevMap M, DevState<R> S) {
 }
 
 // runoff[s] = base[c] * factor[inst] (multiplicative member perturbation)
-template <class R>
+template <class R, bool MB>
 __global__ void k_scale_runoff(DevMap M, DevState<R> S, const double* base, const double* factor) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M.ns) return;
-    const Slot x = cell_slot(M, s);
+    const Slot x = cell_slot<MB>(M, s);
     S.runoff[s] = R(base[x.c] * factor[x.inst]);
 }
 
 // ---------------------------------------------------------------------------
-// launch wrappers
+// launch wrappers: MB (several instances) is chosen from the map, the block
+// size from the launch configuration
 static inline int cdiv(long long a, int b) { return (int)((a + b - 1) / b); }
 static inline int pad8(int g, int xcd) { return xcd ? (g + 7) & ~7 : g; }
-
-template <class R, int BS>
-static void launch_stage_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst,
-                            double dt_def, hipStream_t st) {
-    if (with_cfl) hipLaunchKernelGGL(k_cfl_init, dim3(cdiv(ninst, 64)), dim3(64), 0, st, S.cfl, ninst, dt_def);
-    hipLaunchKernelGGL((k_stage<R, BS>), dim3(cdiv(M.ns, BS)), dim3(BS), 0, st, M, S, A, with_cfl);
-}
+static inline bool multi(const DevMap& M) { return M.nmem > 1 || M.cell_dom != nullptr; }
 
 template <class R>
 void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst, double dt_def,
                   hipStream_t st) {
-    launch_stage_bs<R, 256>(M, S, A, with_cfl, ninst, dt_def, st);
+    if (with_cfl) hipLaunchKernelGGL(k_cfl_init, dim3(cdiv(ninst, 64)), dim3(64), 0, st, S.cfl, ninst, dt_def);
+    if (multi(M)) hipLaunchKernelGGL((k_stage<R, 256, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, A, with_cfl);
+    else hipLaunchKernelGGL((k_stage<R, 256, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, A, with_cfl);
 }
 
 int tiles_cells(const DevMap& M, int bs) { return cdiv(M.ns, bs); }
 int tiles_paths(const DevMap& M, const StepArgs& A, int bs) { return A.lpthout ? cdiv(M.nsp, bs) : 0; }
 
-template <class R, int BS>
+template <class R, int BS, bool MB>
 static void launch_phases_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, hipStream_t st) {
     const int ntc = tiles_cells(M, BS), ntp = tiles_paths(M, A, BS);
     if (phase == 0)
-        hipLaunchKernelGGL((k_outflow<R, BS>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc, ntc + ntp);
+        hipLaunchKernelGGL((k_outflow<R, BS, MB>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc,
+                           ntc + ntp);
     else if (phase == 1)
-        hipLaunchKernelGGL((k_rate<R, BS>), dim3(pad8(ntc, A.xcd)), dim3(BS), 0, st, M, S, A, ntc);
+        hipLaunchKernelGGL((k_rate<R, BS, MB>), dim3(pad8(ntc, A.xcd)), dim3(BS), 0, st, M, S, A, ntc);
     else
-        hipLaunchKernelGGL((k_update<R, BS>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc, ntc + ntp);
+        hipLaunchKernelGGL((k_update<R, BS, MB>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc,
+                           ntc + ntp);
+}
+
+template <class R, bool MB>
+static void launch_phase_mb(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, int bs,
+                            hipStream_t st) {
+    switch (bs) {
+        case 64: launch_phases_bs<R, 64, MB>(M, S, A, phase, st); break;
+        case 128: launch_phases_bs<R, 128, MB>(M, S, A, phase, st); break;
+        case 512: launch_phases_bs<R, 512, MB>(M, S, A, phase, st); break;
+        default: launch_phases_bs<R, 256, MB>(M, S, A, phase, st); break;
+    }
 }
 
 template <class R>
 void launch_phase(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, int bs, hipStream_t st) {
+    if (multi(M)) launch_phase_mb<R, true>(M, S, A, phase, bs, st);
+    else launch_phase_mb<R, false>(M, S, A, phase, bs, st);
+}
+
+template <class R, int BS, bool MB>
+static const void* substep_fn() { return reinterpret_cast<const void*>(&k_substep<R, BS, MB>); }
+
+template <class R, bool MB>
+static const void* substep_kernel_mb(int bs) {
     switch (bs) {
-        case 64: launch_phases_bs<R, 64>(M, S, A, phase, st); break;
-        case 128: launch_phases_bs<R, 128>(M, S, A, phase, st); break;
-        case 512: launch_phases_bs<R, 512>(M, S, A, phase, st); break;
-        default: launch_phases_bs<R, 256>(M, S, A, phase, st); break;
+        case 64: return substep_fn<R, 64, MB>();
+        case 128: return substep_fn<R, 128, MB>();
+        case 512: return substep_fn<R, 512, MB>();
+        default: return substep_fn<R, 256, MB>();
     }
 }
 
-template <class R, int BS>
-static const void* substep_fn() { return reinterpret_cast<const void*>(&k_substep<R, BS>); }
-
 template <class R>
-const void* substep_kernel(int bs) {
-    switch (bs) {
-        case 64: return substep_fn<R, 64>();
-        case 128: return substep_fn<R, 128>();
-        case 512: return substep_fn<R, 512>();
-        default: return substep_fn<R, 256>();
-    }
+const void* substep_kernel(const DevMap& M, int bs) {
+    return multi(M) ? substep_kernel_mb<R, true>(bs) : substep_kernel_mb<R, false>(bs);
 }
 
 template <class R>
@@ -806,18 +843,19 @@ hipError_t launch_substep(const DevMap& M, const DevState<R>& S, const StepArgs&
     DevState<R> s = S;
     StepArgs a = A;
     void* args[] = {&m, &s, &a, &ntc_o, &nt_o, &ntc_r, &ntc_u, &nt_u};
-    return hipLaunchCooperativeKernel(substep_kernel<R>(bs), dim3(grid), dim3(bs), args, 0, st);
+    return hipLaunchCooperativeKernel(substep_kernel<R>(M, bs), dim3(grid), dim3(bs), args, 0, st);
 }
 
 template <class R>
 void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st) {
-    hipLaunchKernelGGL(k_swapfix<R>, dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S);
+    hipLaunchKernelGGL((k_swapfix<R, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S);
 }
 
 template <class R>
 void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* base, const double* factor,
                          hipStream_t st) {
-    hipLaunchKernelGGL(k_scale_runoff<R>, dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, base, factor);
+    if (multi(M)) hipLaunchKernelGGL((k_scale_runoff<R, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, base, factor);
+    else hipLaunchKernelGGL((k_scale_runoff<R, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, base, factor);
 }
 
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {
@@ -828,7 +866,7 @@ void launch_budget(const double* partials, int ntiles, int nrun, double* glb_his
     template void launch_stage<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int, double,        \
                                   hipStream_t);                                                                \
     template void launch_phase<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int, hipStream_t); \
-    template const void* substep_kernel<R>(int);                                                               \
+    template const void* substep_kernel<R>(const DevMap&, int);                                                \
     template hipError_t launch_substep<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int,        \
                                           hipStream_t);                                                        \
     template void launch_swapfix<R>(const DevMap&, const DevState<R>&, hipStream_t);                           \

@@ -23,7 +23,7 @@ template <class R>
 hipError_t launch_substep(const DevMap& M, const DevState<R>& S, const StepArgs& A, int bs, int grid,
                           hipStream_t st);
 template <class R>
-const void* substep_kernel(int bs);
+const void* substep_kernel(const DevMap& M, int bs);
 template <class R>
 void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st);
 template <class R>
