@@ -340,10 +340,23 @@ __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S,
     if (!I.active) return;
     const int i = x.c, m = x.m;
     const double dt = I.dt;
-    double so = (double)vmax(S.qr_u[s], R(0)) + (double)vmax(S.qf_u[s], R(0));
-    for (int k = M.up_ptr[i]; k < M.up_ptr[i + 1]; ++k) {
-        const int su = sidx<MB>(M, M.up_idx[k], m);
-        so = (so + (double)vmax(-S.qr_u[su], R(0))) + (double)vmax(-S.qf_u[su], R(0));
+    const R qr = S.qr_u[s], qf = S.qf_u[s];
+    const int k0 = M.up_ptr[i], k1 = M.up_ptr[i + 1];
+    const double sto = S.rivsto[s] + S.fldsto[s];                  // issued ahead of the gathers
+    double so = (double)vmax(qr, R(0)) + (double)vmax(qf, R(0));
+    // the first two upstream cells (most confluences) are gathered together, the
+    // sum keeps the ascending order of the list
+    if (k0 < k1) {
+        const bool two = k0 + 1 < k1;
+        const int s0 = sidx<MB>(M, M.up_idx[k0], m);
+        const int s1 = two ? sidx<MB>(M, M.up_idx[k0 + 1], m) : s0;
+        const R q0 = S.qr_u[s0], f0_ = S.qf_u[s0], q1 = S.qr_u[s1], f1_ = S.qf_u[s1];
+        so = (so + (double)vmax(-q0, R(0))) + (double)vmax(-f0_, R(0));
+        if (two) so = (so + (double)vmax(-q1, R(0))) + (double)vmax(-f1_, R(0));
+        for (int k = k0 + 2; k < k1; ++k) {
+            const int su = sidx<MB>(M, M.up_idx[k], m);
+            so = (so + (double)vmax(-S.qr_u[su], R(0))) + (double)vmax(-S.qf_u[su], R(0));
+        }
     }
     so = so * dt;
     if (A.lpthout) {
@@ -353,7 +366,7 @@ __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S,
             so = so + (double)vmax(-S.pthflwsum_u[sidx<MB>(M, M.pinf_idx[k], m)], R(0)) * dt;
     }
     const R dout = vmax(R(so), R(1.e-10));
-    const R dsto = R(S.rivsto[s] + S.fldsto[s]);
+    const R dsto = R(sto);
     S.rate[s] = vmin(dsto / dout, R(1));
 }
 
@@ -403,12 +416,28 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const R qrl = qr * r, qfl = qf * r;
     // upstream gather, inflow_lspamat:481-489
     double rivinf = 0.0, fldinf = 0.0;
-    for (int k = k0; k < k1; ++k) {
-        const int su = sidx<MB>(M, M.up_idx[k], m);
-        const R qu = S.qr_u[su], fu = S.qf_u[su];
-        const R ru = (qu > zero) ? S.rate[su] : ri;
-        rivinf = rivinf + (double)(qu * ru);
-        fldinf = fldinf + (double)(fu * ru);
+    if (k0 < k1) {
+        // the first two upstream cells are gathered together (ascending order kept)
+        const bool two = k0 + 1 < k1;
+        const int s0 = sidx<MB>(M, M.up_idx[k0], m);
+        const int s1 = two ? sidx<MB>(M, M.up_idx[k0 + 1], m) : s0;
+        const R q0 = S.qr_u[s0], fu0 = S.qf_u[s0], rr0 = S.rate[s0];
+        const R q1 = S.qr_u[s1], fu1 = S.qf_u[s1], rr1 = S.rate[s1];
+        const R ru0 = (q0 > zero) ? rr0 : ri;
+        rivinf = rivinf + (double)(q0 * ru0);
+        fldinf = fldinf + (double)(fu0 * ru0);
+        if (two) {
+            const R ru1 = (q1 > zero) ? rr1 : ri;
+            rivinf = rivinf + (double)(q1 * ru1);
+            fldinf = fldinf + (double)(fu1 * ru1);
+        }
+        for (int k = k0 + 2; k < k1; ++k) {
+            const int su = sidx<MB>(M, M.up_idx[k], m);
+            const R qu = S.qr_u[su], fu = S.qf_u[su];
+            const R ru = (qu > zero) ? S.rate[su] : ri;
+            rivinf = rivinf + (double)(qu * ru);
+            fldinf = fldinf + (double)(fu * ru);
+        }
     }
     double pthout = 0.0;
     if (A.lpthout) {                                                  // :519-535
