@@ -239,6 +239,9 @@ def ensemble_section(a):
     for prec in ("mixed", "fp64"):
         r = run_ensemble(nets, ms, prec, a.ens_days, 1, a.seed, tune="auto")
         res[prec] = r
+        steps = max(r["kt"][4], 1.0)
+        r["kernels_ms"] = {k: round(r["kt"][i] / steps, 4) for i, k in enumerate(("outflow", "rate", "update"))}
+        log(f"ensemble {prec}: kernels ms per substep {r['kernels_ms']}")
         log(f"ensemble {prec}: {r['work'] / r['t']:.4g} member-cell-substeps/s; tune {r['tune']}; "
             f"latency p50 {np.percentile(r['lat'], 50):.1f} ms p99 {np.percentile(r['lat'], 99):.1f} ms")
     # mixed vs FP64 storage error per instance after the same days
@@ -262,8 +265,9 @@ def ensemble_section(a):
         ms_per_day=1e3 * r["t"] / a.ens_days,
         latency_ms={"p50": float(np.percentile(r["lat"], 50)), "p99": float(np.percentile(r["lat"], 99)),
                     "samples": int(r["lat"].size)},
-        tune=r["tune"],
+        tune=r["tune"], kernels_ms=r["kernels_ms"],
         fp64={"value": rd["work"] / rd["t"], "ms_per_day": 1e3 * rd["t"] / a.ens_days,
+              "kernels_ms": rd["kernels_ms"],
               "latency_ms": {"p50": float(np.percentile(rd["lat"], 50)),
                              "p99": float(np.percentile(rd["lat"], 99))}, "tune": rd["tune"]},
         mixed_vs_fp64_storage={"total_rel_max": float(max(rel_tot)), "cell_rel_max": float(max(rel_max))},

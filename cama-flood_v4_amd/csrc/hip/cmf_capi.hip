@@ -892,7 +892,22 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     TRY(put_raw(h, &p32, dptr.data(), dptr.size())); M.up_ptr = p32;
     TRY(put_raw(h, &p32, didx.data(), didx.size())); M.up_idx = p32;
     M.cell_dom = nullptr;
-    if (h->ndom > 1) { TRY(put_raw(h, &p32, ddom.data(), (size_t)n)); M.cell_dom = p32; }
+    M.ndom_arg = 0;
+    if (h->ndom > 1) {
+        TRY(put_raw(h, &p32, ddom.data(), (size_t)n)); M.cell_dom = p32;
+        // domains stay contiguous in device order (every river tree lies in one
+        // domain and the depth-first order visits mouths in NSEQ order)
+        bool contiguous = true;
+        for (int p = 1; p < n; ++p) contiguous = contiguous && ddom[(size_t)p] >= ddom[(size_t)p - 1];
+        if (contiguous && h->ndom <= NDOM_ARG) {
+            M.ndom_arg = h->ndom;
+            int d = 0;
+            M.dom_off_arg[0] = 0;
+            for (int p = 0; p < n; ++p)
+                while (d < h->ndom - 1 && ddom[(size_t)p] > d) M.dom_off_arg[++d] = p;
+            while (d < h->ndom) M.dom_off_arg[++d] = n;
+        }
+    }
     TRY(validate_tables(h, mp));
     TRY(put_f32(h, &M.grarea, mp->grarea, n, "GRAREA"));
     TRY(put_f32(h, &M.elevtn, mp->elevtn, n, "ELEVTN"));

@@ -45,6 +45,7 @@ enum { G_STOPRE, G_STONXT, G_STONEW, G_RIVINF, G_RIVOUT,
 
 constexpr int NLFP_MAX = 16;
 constexpr int NMEM_MAX = 64;                   // members per domain (power of two)
+constexpr int NDOM_ARG = 32;                   // domain offsets carried in the kernel arguments
 
 template <class T> __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
@@ -62,6 +63,8 @@ struct DevMap {
     int32_t ns, nsp;                           // state slots: n << lgm, npth << lgm
     double dfrcinc;
     const int32_t* cell_dom;                   // domain of each cell (NULL: one domain)
+    int32_t ndom_arg;                          // > 0: domains found from dom_off_arg instead
+    int32_t dom_off_arg[NDOM_ARG + 1];         // first cell of each domain (device order)
     const int32_t* pth_dom;                    // domain of each path (NULL: one domain)
     const int32_t* next;                       // 0-based downstream, <0 for mouths
     const int32_t* up_ptr;                     // upstream CSR (n+1)
@@ -136,7 +139,14 @@ __device__ __forceinline__ Slot cell_slot(const DevMap& M, int s) {
     if (!MB) { x.c = s; x.m = 0; x.inst = 0; return x; }
     x.c = s >> M.lgm;
     x.m = s & (M.nmem - 1);
-    x.inst = (M.cell_dom ? M.cell_dom[x.c] * M.nmem : 0) + x.m;
+    int d = 0;
+    if (M.ndom_arg > 0) {
+        // compare against the offsets in the kernel arguments: no dependent load
+        for (int k = 1; k < M.ndom_arg; ++k) d += x.c >= M.dom_off_arg[k];
+    } else if (M.cell_dom) {
+        d = M.cell_dom[x.c];
+    }
+    x.inst = d * M.nmem + x.m;
     return x;
 }
 template <bool MB>

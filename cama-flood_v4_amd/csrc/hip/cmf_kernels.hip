@@ -80,10 +80,10 @@ __device__ __forceinline__ InstView inst_view(const InstStep* tab, int inst, int
 template <class R, bool MB>
 __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
     const Slot x = cell_slot<MB>(M, s);
-    const InstView I = inst_view(S.inst, x.inst, A.it);
-    if (!I.active) return;
     const int i = x.c, m = x.m;
-    // every load that does not depend on the downstream index is issued first
+    // every load that does not depend on the downstream index is issued first,
+    // together with the instance's step (an idle instance only wastes the loads)
+    const InstView I = inst_view(S.inst, x.inst, A.it);
     const int nxt = M.next[i];
     const R rivdph = S.rivdph[s], rivdph_pre = S.rivdph_pre[s];
     const R qr_pre = S.rivout[s];                                   // D2RIVOUT_PRE
@@ -100,6 +100,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
         fldsto_pre = S.fldsto_pre[s];
         fldout_pre = S.fldout[s];                                   // D2FLDOUT_PRE
     }
+    if (!I.active) return;
     const R gdt = R(I.gdt);
     const R zero = R(0), one = R(1);
     const R rivelv = R((double)f_elevtn - (double)f_rivhgt);        // maps:722
@@ -336,13 +337,16 @@ __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>&
 template <class R, bool MB>
 __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
     const Slot x = cell_slot<MB>(M, s);
-    const InstView I = inst_view(S.inst, x.inst, A.it);
-    if (!I.active) return;
     const int i = x.c, m = x.m;
-    const double dt = I.dt;
+    const InstView I = inst_view(S.inst, x.inst, A.it);
     const R qr = S.qr_u[s], qf = S.qf_u[s];
     const int k0 = M.up_ptr[i], k1 = M.up_ptr[i + 1];
-    const double sto = S.rivsto[s] + S.fldsto[s];                  // issued ahead of the gathers
+    // RIVSTO + FLDSTO, issued ahead of the gathers. Without levees STORGE holds
+    // exactly this sum (written with RIVSTO and FLDSTO by the last stage
+    // inversion, and converted to R as the rate is), so one array is read.
+    const double sto = A.llevee ? S.rivsto[s] + S.fldsto[s] : (double)S.storge[s];
+    if (!I.active) return;
+    const double dt = I.dt;
     double so = (double)vmax(qr, R(0)) + (double)vmax(qf, R(0));
     // the first two upstream cells (most confluences) are gathered together, the
     // sum keeps the ascending order of the list
@@ -384,12 +388,9 @@ template <class R, bool MB>
 __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A,
                                             int s, double (&g)[NGLB]) {
     const Slot x = cell_slot<MB>(M, s);
-    const InstView I = inst_view(S.inst, x.inst, A.it);
-    if (!I.active) return;
     const int i = x.c, m = x.m;
-    const double dt = I.dt;
+    const InstView I = inst_view(S.inst, x.inst, A.it);
     const R zero = R(0);
-    const bool f0 = I.first;
     // every load that does not depend on a gathered index is issued first, so it
     // is in flight while the upstream and downstream gathers resolve
     const R qr = S.qr_u[s], qf = S.qf_u[s];
@@ -405,12 +406,19 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     Lev lev{};
     if (A.llevee) lev = load_lev(M, i);
     const int dam = A.ldamout ? M.dam_of_cell[i] : -1;
-    const R p_ravg = S.rivout_avg[s], p_favg = S.fldout_avg[s], p_oavg = S.outflw_avg[s];
+    // the mixed path forms OUTFLW's average from the RIVOUT and FLDOUT averages on
+    // the last substep instead of accumulating it (it differs by FP32 rounding)
+    constexpr bool kOutSum = sizeof(R) == sizeof(float);
+    const R p_ravg = S.rivout_avg[s], p_favg = S.fldout_avg[s];
+    const R p_oavg = kOutSum ? zero : S.outflw_avg[s];
     const R p_omax = S.outflw_max[s], p_dmax = S.rivdph_max[s], p_smax = S.storge_max[s];
     const R p_pavg = A.lpthout ? S.pthout_avg[s] : zero;
     R p_davg = zero;
     double daminf = 0.0, damsto = 0.0;
     if (A.ldamout) { p_davg = S.daminf_avg[s]; daminf = S.daminf[s]; }
+    if (!I.active) return;
+    const double dt = I.dt;
+    const bool f0 = I.first;
     // limiter, inflow_lspamat:456-473
     const R r = (nxt >= 0 && !(qr > zero)) ? S.rate[sidx<MB>(M, nxt, m)] : ri;
     const R qrl = qr * r, qfl = qf * r;
@@ -524,14 +532,15 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     R a3 = (f0 ? zero : p_oavg) + outflw * rdt;
     if (I.last) {
         const R nadd = R(I.nadd);
-        a1 = a1 / nadd; a2 = a2 / nadd; a3 = a3 / nadd;
+        a1 = a1 / nadd; a2 = a2 / nadd; a3 = kOutSum ? a1 + a2 : a3 / nadd;
         // RUNOFF is constant over the outer step: its NT additions are replayed in
         // registers, in the same order, instead of a read-modify-write per substep
         R a5 = zero;
         for (int k = 0; k < I.nt; ++k) a5 = a5 + runoff * rdt;
         S.runoff_avg[s] = a5 / nadd;
     }
-    S.rivout_avg[s] = a1; S.fldout_avg[s] = a2; S.outflw_avg[s] = a3;
+    S.rivout_avg[s] = a1; S.fldout_avg[s] = a2;
+    if (!kOutSum || I.last) S.outflw_avg[s] = a3;
     if (A.lpthout) {
         R a4 = (f0 ? zero : p_pavg) + pthout_r * rdt;
         if (I.last) a4 = a4 / R(I.nadd);
@@ -763,7 +772,7 @@ __global__ __launch_bounds__(BS) void k_stage(DevMap M, DevState<R> S, StepArgs 
     // A wavefront starts at a multiple of 64 slots and nmem divides 64, so lane
     // k < nmem holds member k.
     const int lane = threadIdx.x & 63;
-    const int d = (s < M.ns && M.cell_dom) ? M.cell_dom[s >> M.lgm] : 0;
+    const int d = (s < M.ns) ? (inst - (inst & (M.nmem - 1))) / M.nmem : 0;
     const int d0 = __shfl(d, 0, 64);
     const bool one_dom = __all(d == d0 || s >= M.ns);
     if (one_dom) {
