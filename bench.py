@@ -150,6 +150,79 @@ def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
                            f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
 
 
+def initial_state(net, m):
+    """Cold start, initial CMF_PHYSICS_FLDSTG on the device (FP64) and dam
+    initialisation (drv_control:239-252); host arrays shared by every run."""
+    import dataclasses
+    from cmf import maps
+    from cmf.model import CMFModel
+    st = maps.cold_start(m)
+    gm0 = CMFModel(m, "fp64", opts=dataclasses.replace(m.opts, ldamout=False))
+    gm0.set_state(st)
+    gm0.physics_fldstg()
+    st.update(gm0.get_state(["rivsto", "fldsto", "rivdph", "flddph", "fldfrc", "fldare", "sfcelv", "storge",
+                             "levsto", "levdph"]))
+    gm0.close()
+    if m.opts.ldamout:
+        maps.init_dams(m, net, st)
+    return st
+
+
+def precision_report(seed):
+    """Mixed (FP32 flux/stage, FP64 storage) against FP64 on identical inputs for
+    the three single-model configurations; relative errors are max|mixed - fp64|
+    over max|fp64| per field (cells) and |sum difference| / sum for the total
+    storage, the worst over the run."""
+    from cmf import maps, synth
+    from cmf.model import CMFModel
+    cfgs = [
+        ("C1 1-degree 360x180, default physics, daily DT, 30 days", 360, 180, 30, 86400.0, dict(), 0),
+        ("C2 glb_15min-shaped 1440x720 + 6e4 bifurcation paths x 5 levels, daily DT, 365 days", 1440, 720, 365,
+         86400.0, dict(lpthout=True), 60000),
+        ("C3 glb_06min-shaped 3600x1800, 3% levees, 2500 dams, hourly DT, 24 steps", 3600, 1800, 24, 3600.0,
+         dict(llevee=True, ldamout=True), 0),
+    ]
+    out = []
+    for name, nx, ny, nstep, dt, flags, npth in cfgs:
+        t0 = time.time()
+        net = synth.make_network(nx, ny, seed=seed)
+        if npth:
+            synth.add_bifurcations(net, npth, 5, seed)
+        if flags.get("llevee"):
+            synth.add_levees(net, 0.03, seed)
+        if flags.get("ldamout"):
+            synth.add_dams(net, 2500, seed)
+        nrec = min(nstep, 32)
+        ro = synth.make_runoff(net, nrec, seed)
+        m = maps.build_map(net, maps.Options(**flags))
+        st = initial_state(net, m)
+        runs = {p: CMFModel(m, p) for p in ("fp64", "mixed")}
+        for g in runs.values():
+            g.set_state(st)
+        worst = {"storage_cell": 0.0, "storage_total": 0.0, "outflow": 0.0, "depth": 0.0}
+        nts = {"fp64": 0, "mixed": 0}
+        for k in range(nstep):
+            r = m.vec(ro[k % nrec])
+            for p, g in runs.items():
+                g.set_runoff(r)
+                nts[p] += g.physics_advance(dt)
+            a = runs["fp64"].get_state(["rivsto", "fldsto", "levsto", "outflw", "rivdph"])
+            b = runs["mixed"].get_state(["rivsto", "fldsto", "levsto", "outflw", "rivdph"])
+            sa = a["rivsto"] + a["fldsto"] + (a["levsto"] if "levsto" in a else 0.0)
+            sb = b["rivsto"] + b["fldsto"] + (b["levsto"] if "levsto" in b else 0.0)
+            worst["storage_cell"] = max(worst["storage_cell"], float(np.abs(sb - sa).max() / np.abs(sa).max()))
+            worst["storage_total"] = max(worst["storage_total"], float(abs(sb.sum() - sa.sum()) / sa.sum()))
+            worst["outflow"] = max(worst["outflow"], float(np.abs(b["outflw"] - a["outflw"]).max()
+                                                          / np.abs(a["outflw"]).max()))
+            worst["depth"] = max(worst["depth"], float(np.abs(b["rivdph"] - a["rivdph"]).max()
+                                                      / np.abs(a["rivdph"]).max()))
+        for g in runs.values():
+            g.close()
+        out.append(dict(config=name, cells=m.n, steps=nstep, dt=dt, substeps=nts, rel_err=worst))
+        log(f"{name}: {worst} ({time.time() - t0:.1f}s)")
+    return out
+
+
 ENS_DOMAINS = 16
 ENS_MEMBERS = 4           # members per domain: 64 member instances
 ENS_SIGMA = 0.3           # multiplicative lognormal runoff perturbation
@@ -303,12 +376,17 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only report mixed-vs-FP64 errors of the C1/C2/C3 configurations (JSON)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (HBM counter calibration)")
     ap.add_argument("--ens-days", type=int, default=3)
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
+    if a.precision_report:
+        print(json.dumps({"mixed_vs_fp64": precision_report(a.seed)}), flush=True)
+        return
     if a.calibrate_copy:
         import torch
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")

@@ -103,39 +103,71 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     if (!I.active) return;
     const R gdt = R(I.gdt);
     const R zero = R(0), one = R(1);
-    const R rivelv = R((double)f_elevtn - (double)f_rivhgt);        // maps:722
-    const R sfcelv = rivelv + rivdph;                               // outflw:45-47
-    const R sfcelv_pre = rivelv + rivdph_pre;
     const R flddph_pre = vmax(rivdph_pre - R(f_rivhgt), zero);
     const R rivwth = R(f_rivwth), rivman = R(f_rivman);
     const bool river = nxt >= 0;                                     // mouths carry a negative code
-    R dwn, dwn_pre, rdst;
-    if (river) {                                                     // outflw:52-57
-        const int j = nxt, sj = sidx<MB>(M, j, m);
-        const R rivelv_j = R(rivelv_of(M, j));
-        dwn = rivelv_j + S.rivdph[sj];
-        dwn_pre = rivelv_j + S.rivdph_pre[sj];
-        rdst = one / R(f_nxtdst);
+    // Water-surface terms of outflw:45-58. With R = double they are evaluated as
+    // the reference writes them (SFCELV = RIVELV + RIVDPH, slopes and depths from
+    // SFCELV). With R = float the same quantities are formed from depths and the
+    // exact FP64 elevation difference, rounded once: a ~250 m surface elevation
+    // keeps only ~1.5e-5 m in FP32, which the slope (a difference of a few cm)
+    // would otherwise inherit.
+    //   hs  = SFCELV - DWNELV          (slope numerator)
+    //   hr  = max(SFCELV, DWNELV) - RIVELV,  hrp the same with the _PRE values
+    //   (mouths: hr = RIVDPH, hrp = RIVDPH_PRE)
+    //   fa  = surface above ELEVTN for the floodplain (outflw:88, :147), fap the _PRE one
+    R hs, hr, hrp, fa, fap, rdst;
+    if (sizeof(R) == sizeof(double)) {
+        const R rivelv = R((double)f_elevtn - (double)f_rivhgt);    // maps:722
+        const R elevtn = R(f_elevtn);
+        const R sfcelv = rivelv + rivdph;                           // outflw:45-47
+        const R sfcelv_pre = rivelv + rivdph_pre;
+        if (river) {                                                 // outflw:52-57
+            const int j = nxt, sj = sidx<MB>(M, j, m);
+            const R rivelv_j = R(rivelv_of(M, j));
+            const R dwn = rivelv_j + S.rivdph[sj];
+            const R dwn_pre = rivelv_j + S.rivdph_pre[sj];
+            const R dsfc = vmax(sfcelv, dwn), dsfc_pr = vmax(sfcelv_pre, dwn_pre);
+            hs = sfcelv - dwn;
+            hr = dsfc - rivelv;
+            hrp = dsfc_pr - rivelv;
+            fa = dsfc - elevtn;
+            fap = dsfc_pr - elevtn;
+            rdst = one / R(f_nxtdst);
+        } else {
+            const R dwn = R(M.dwnelv[i]);
+            hs = sfcelv - dwn;
+            hr = rivdph;
+            hrp = rivdph_pre;
+            fa = sfcelv - elevtn;
+            fap = sfcelv_pre - elevtn;
+            rdst = one / R(A.pdstmth);
+        }
     } else {
-        dwn = R(M.dwnelv[i]);
-        dwn_pre = dwn;
-        rdst = one / R(A.pdstmth);
+        const double e_i = (double)f_elevtn - (double)f_rivhgt;
+        if (river) {
+            const int j = nxt, sj = sidx<MB>(M, j, m);
+            const R dz = R(rivelv_of(M, j) - e_i);                   // RIVELV(j) - RIVELV(i)
+            const R dj = S.rivdph[sj], dj_pre = S.rivdph_pre[sj];
+            hs = (rivdph - dj) - dz;
+            hr = vmax(rivdph, dz + dj);
+            hrp = vmax(rivdph_pre, dz + dj_pre);
+            rdst = one / R(f_nxtdst);
+        } else {
+            hs = rivdph - R(M.dwnelv[i] - e_i);
+            hr = rivdph;
+            hrp = rivdph_pre;
+            rdst = one / R(A.pdstmth);
+        }
+        const R hgt = R(f_rivhgt);                                   // ELEVTN = RIVELV + RIVHGT
+        fa = hr - hgt;
+        fap = hrp - hgt;
     }
     // ---- river channel: outflw:60-80 (river), :119-139 (mouth)
     R q_r, q_f = zero, vel;
     {
-        R dslp, dflw, dflw_pr;
-        if (river) {
-            const R dsfc = vmax(sfcelv, dwn);
-            dslp = (sfcelv - dwn) * rdst;
-            dflw = dsfc - rivelv;
-            const R dsfc_pr = vmax(sfcelv_pre, dwn_pre);
-            dflw_pr = dsfc_pr - rivelv;
-        } else {
-            dslp = (sfcelv - dwn) * rdst;
-            dflw = rivdph;
-            dflw_pr = rivdph_pre;
-        }
+        const R dslp = hs * rdst;
+        const R dflw = hr, dflw_pr = hrp;
         R dare = rivwth * dflw;
         dare = vmax(dare, R(1.e-10));
         R dflw_im = vsqrt(dflw * dflw_pr);
@@ -148,28 +180,26 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
         q_r = mask ? dout : zero;
         vel = mask ? dvel : zero;
     }
-    // ---- floodplain: outflw:84-116 (river), :143-171 (mouth)
+    // ---- floodplain: outflw:84-116 (river), :143-171 (mouth). Surface heights
+    // above ELEVTN = RIVELV + RIVHGT: max(SFCELV, DWNELV) - ELEVTN = hr - RIVHGT.
     if (A.lfldout) {
         const R gdtn2 = gdt * (R(A.pmanfld) * R(A.pmanfld));
         const R dfsto = R(fldsto_d);
-        const R elevtn = R(f_elevtn);
         const R rrl = one / R(f_rivlen);
-        R dslp = (sfcelv - dwn) * rdst;
+        R dslp = hs * rdst;
         dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
         R dflw, dflw_pr, dare, dare_pr;
         if (river) {
-            const R dsfc = vmax(sfcelv, dwn);
-            dflw = vmax(dsfc - elevtn, zero);
+            dflw = vmax(fa, zero);
             dare = dfsto * rrl;
             dare = vmax(dare - flddph * rivwth, zero);
-            const R dsfc_pr = vmax(sfcelv_pre, dwn_pre);
-            dflw_pr = dsfc_pr - elevtn;
+            dflw_pr = fap;
             dare_pr = fldsto_pre * rrl;
             dare_pr = vmax(dare_pr - flddph_pre * rivwth, R(1.e-6));
         } else {
-            dflw = sfcelv - elevtn;
+            dflw = fa;
             dare = vmax(dfsto * rrl - flddph * rivwth, zero);
-            dflw_pr = sfcelv_pre - elevtn;
+            dflw_pr = fap;
             dare_pr = vmax(fldsto_pre * rrl - flddph_pre * rivwth, R(1.e-6));
         }
         const R t = dflw * dflw_pr;
@@ -285,30 +315,56 @@ __device__ __forceinline__ void pthflow_path(const DevMap& M, const DevState<R>&
     const int si_ = sidx<MB>(M, ip, m), sj_ = sidx<MB>(M, jp, m);
     const R zero = R(0), one = R(1);
     const R gdt = R(I.gdt);
-    const R ri = R(rivelv_of(M, ip)), rj = R(rivelv_of(M, jp));
-    const R si = ri + S.rivdph[si_], sj = rj + S.rivdph[sj_];
-    const R pi = ri + S.rivdph_pre[si_], pj = rj + S.rivdph_pre[sj_];
-    R dslp = (si - sj) / R(M.pth_dst[p]);
+    // Surface terms; with R = float in difference form (see outflow_cell): each
+    // surface height relative to a level elevation is a depth plus the exact FP64
+    // elevation difference rounded once.
+    constexpr bool kDiff = sizeof(R) == sizeof(float);
+    const double ri_d = rivelv_of(M, ip), rj_d = rivelv_of(M, jp);
+    const R di = S.rivdph[si_], dj = S.rivdph[sj_];
+    const R dpi = S.rivdph_pre[si_], dpj = S.rivdph_pre[sj_];
+    const R ri = R(ri_d), rj = R(rj_d);
+    const R si = ri + di, sj = rj + dj;
+    const R pi = ri + dpi, pj = rj + dpj;
+    R dslp = (kDiff ? (di - dj) + R(ri_d - rj_d) : si - sj) / R(M.pth_dst[p]);
     dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
     R li = si, lj = sj;                                              // D2SFCELV_LEV
+    bool lev_i = false, lev_j = false;
+    R ldi = zero, ldj = zero;
     if (A.llevee) {
         const int a = M.lev_idx[ip], b = M.lev_idx[jp];
         const int nlc = M.nlevcell;
-        if (a >= 0 && M.lev_tab[nlc + a] < 1.0) li = vmin(R(M.elevtn[ip]) + S.levdph[si_], si);
-        if (b >= 0 && M.lev_tab[nlc + b] < 1.0) lj = vmin(R(M.elevtn[jp]) + S.levdph[sj_], sj);
+        lev_i = a >= 0 && M.lev_tab[nlc + a] < 1.0;
+        lev_j = b >= 0 && M.lev_tab[nlc + b] < 1.0;
+        if (lev_i) { ldi = S.levdph[si_]; li = vmin(R(M.elevtn[ip]) + ldi, si); }
+        if (lev_j) { ldj = S.levdph[sj_]; lj = vmin(R(M.elevtn[jp]) + ldj, sj); }
     }
     R sum = zero;
     for (int l = 0; l < nl; ++l) {
-        const R elv = R(M.pth_elv[(size_t)l * np + p]);
+        const double elv_d = M.pth_elv[(size_t)l * np + p];
+        const R elv = R(elv_d);
+        // max(si, sj) - elv, max(pi, pj) - elv and max(li, lj) - elv
+        R hs, hp, hl;
+        if (kDiff) {
+            const R ai = R(ri_d - elv_d), aj = R(rj_d - elv_d);
+            hs = vmax(di + ai, dj + aj);
+            hp = vmax(dpi + ai, dpj + aj);
+            const R lvi = lev_i ? vmin(ldi + R((double)M.elevtn[ip] - elv_d), di + ai) : di + ai;
+            const R lvj = lev_j ? vmin(ldj + R((double)M.elevtn[jp] - elv_d), dj + aj) : dj + aj;
+            hl = vmax(lvi, lvj);
+        } else {
+            hs = vmax(si, sj) - elv;
+            hp = vmax(pi, pj) - elv;
+            hl = vmax(li, lj) - elv;
+        }
         R dflw, dflw_im;
         if (!A.llevee || l == 0) {
-            dflw = vmax(vmax(si, sj) - elv, zero);
-            R dflw_pr = vmax(vmax(pi, pj) - elv, zero);
+            dflw = vmax(hs, zero);
+            R dflw_pr = vmax(hp, zero);
             dflw_im = vsqrt(dflw * dflw_pr);
             if (A.llevee) { if (dflw_im <= zero) dflw_im = dflw; }
             else dflw_im = vmax(dflw_im, vsqrt(dflw * R(0.01)));
         } else {
-            dflw = vmax(vmax(li, lj) - elv, zero);
+            dflw = vmax(hl, zero);
             dflw_im = dflw;
         }
         R qq = zero;
