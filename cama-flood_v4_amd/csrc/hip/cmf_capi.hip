@@ -410,7 +410,7 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.pgrv = h->o.pgrv; A.pmanfld = h->o.pmanfld; A.pdstmth = h->o.pdstmth;
     A.pcadp = h->o.pcadp; A.pminslp = h->o.pminslp;
     A.lfldout = h->o.lfldout; A.lpthout = h->o.lpthout; A.ldamout = h->o.ldamout; A.llevee = h->o.llevee;
-    A.budget = h->ninst == 1;
+    A.budget = h->ninst == 1 && h->nmem == 1 && h->ndom == 1;   // the kernels keep budgets only then
     A.xcd = h->tune.xcd;
     return A;
 }

@@ -674,7 +674,9 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
 #pragma unroll
         for (int k = 0; k < NGLB; ++k) g[k] = 0.0;
         if (s < M.ns) update_cell<R, MB>(M, S, A, s, g);
-        if (A.budget) {
+        // budget sums are kept for single-instance handles only (MB = false), so
+        // the ensemble kernels carry no budget registers
+        if (!MB && A.budget) {
             block_sum<BS, NGLB>(g, lds);
             // per-substep slot: reduced once for all substeps at the end of the outer step
             if (threadIdx.x < NGLB)
