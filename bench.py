@@ -347,6 +347,62 @@ def ensemble_section(a):
     )
 
 
+def spinup_section(a):
+    """Learned spin-up warm start: labels from a batched FP64 spin-up of
+    glb_15min-shaped basins, training by the project's BF16 matrix-core kernels,
+    then cold-start vs warm-start FP64 spin-up of an application network."""
+    from cmf import maps, synth
+    from cmf import surrogate as sg
+    from cmf.spinup import make_climatology
+    t0 = time.time()
+    basins = sg.make_basins(a.spin_basins, seed=a.seed + 7)
+    X, Y, res = sg.equilibrium_labels(basins, tol=1e-6, max_years=8, log=lambda s_: log("labels " + s_))
+    t_labels = time.time() - t0
+    log(f"spin-up labels: {len(basins)} basins, {X.shape[0]} cells, cold-start years "
+        f"median {int(np.median(res.years))}, {t_labels:.1f}s")
+    t1 = time.time()
+    sur, hist = sg.train(X, Y, epochs=a.spin_epochs, batch=1024, lr=2e-3, seed=0,
+                         log=lambda s_: log("train " + s_))
+    t_train = time.time() - t1
+    nx, ny = (7200, 3600) if a.spinup_full else (1440, 720)
+    t2 = time.time()
+    net = synth.make_network(nx, ny, seed=a.seed + 11)
+    m = maps.build_map(net, maps.Options())
+    clim = make_climatology(net, m, a.seed + 11, mean_mm_day=1.5, amp=0.5, phase=3.0)
+    feats = sg.features(m, clim)
+    net_dev = sg.MLP(max_batch=1 << 18)
+    net_dev.set_params(sur.params)
+    xs = (feats - sur.x_mu) / sur.x_sd
+    net_dev.forward(xs[: 1 << 16])                       # warm-up
+    ti = time.perf_counter()
+    y = net_dev.forward(xs)
+    t_inf = time.perf_counter() - ti
+    net_dev.close()
+    flops = 2.0 * m.n * (24 * 256 + 2 * 256 * 256 + 256 * 2)
+    log(f"application network {nx}x{ny}: {m.n} cells ({time.time() - t2:.1f}s); inference {1e3 * t_inf:.1f} ms "
+        f"({flops / t_inf / 1e12:.2f} TFLOP/s incl. host transfers)")
+    basin = sg.Basin(net, m, clim)
+    cmp_ = sg.compare_spinup(sur, basin, years=3, log=lambda s_: log("spin-up " + s_))
+    c, w = cmp_["cold"], cmp_["warm"]
+    out = dict(
+        application=f"{'glb_03min' if a.spinup_full else 'glb_15min'}-shaped {nx}x{ny}, {m.n} cells, "
+                    f"monthly gamma climatology repeated, DT=86400",
+        cold_years=c["years"], warm_years=w["years"],
+        years_ratio=(c["years"] / w["years"]) if w["years"] else None,
+        cold_days_to_0p1pct=c["days"], warm_days_to_0p1pct=w["days"],
+        days_ratio=c["days"] / max(w["days"], 1),
+        init_rel_err={"cold": c["init_rel_err"], "warm": w["init_rel_err"]},
+        annual_change={"cold": c["annual_change"], "warm": w["annual_change"]},
+        training={"basins": len(basins), "cells": int(X.shape[0]), "epochs": a.spin_epochs,
+                  "loss_first": hist[0], "loss_last": hist[-1], "label_seconds": round(t_labels, 1),
+                  "train_seconds": round(t_train, 1)},
+        inference={"cells": m.n, "ms": round(1e3 * t_inf, 2), "tflops_incl_transfers": round(flops / t_inf / 1e12, 3)},
+    )
+    log(f"spin-up: cold {c['years']} y / {c['days']} d, warm {w['years']} y / {w['days']} d; initial storage error "
+        f"cold {c['init_rel_err']:.3e} warm {w['init_rel_err']:.3e}")
+    return out
+
+
 def roofline(kt, n_cells, bytes_tab):
     steps = max(kt[4], 1.0)
     if kt[0] == 0.0 and kt[1] == 0.0:
@@ -376,6 +432,15 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--tune", type=str, default="auto",
+                    help='launch configuration of the single-model runs: "auto" (autotuner) or a JSON object '
+                         'of cmf_tune fields')
+    ap.add_argument("--no-spinup", action="store_true")
+    ap.add_argument("--spinup-full", action="store_true",
+                    help="apply the surrogate to the 7200x3600 glb_03min-shaped network (several minutes)")
+    ap.add_argument("--spinup-only", action="store_true")
+    ap.add_argument("--spin-basins", type=int, default=200)
+    ap.add_argument("--spin-epochs", type=int, default=40)
     ap.add_argument("--precision-report", action="store_true",
                     help="only report mixed-vs-FP64 errors of the C1/C2/C3 configurations (JSON)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -384,6 +449,9 @@ def main():
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
+    if a.spinup_only:
+        print(json.dumps({"spinup": spinup_section(a)}), flush=True)
+        return
     if a.precision_report:
         print(json.dumps({"mixed_vs_fp64": precision_report(a.seed)}), flush=True)
         return
@@ -405,7 +473,8 @@ def main():
     rvec = [m.vec(ro[r]) for r in range(ro.shape[0])]
     n = m.n
     log("FP64 run")
-    sum_nt, t, kt, tune64 = run_gpu(net, m, rvec, "fp64", a.steps, a.warmup, dt)
+    tune_arg = "auto" if a.tune == "auto" else json.loads(a.tune)
+    sum_nt, t, kt, tune64 = run_gpu(net, m, rvec, "fp64", a.steps, a.warmup, dt, tune=tune_arg)
     value = n * sum_nt / t
     dom, per, roof = roofline(kt, n, BYTES_FP64)
     log(f"FP64: {value:.4g} cell-substeps/s, sum NT {sum_nt}, {t:.3f}s; kernels ms {per}")
@@ -423,7 +492,7 @@ def main():
         net2, m2, _ = net, None, None
         from cmf import maps
         m2 = maps.build_map(net, maps.Options(llevee=True, ldamout=True))
-        sum_nt2, t2, kt2, tune32 = run_gpu(net, m2, rvec, "mixed", a.steps, a.warmup, dt)
+        sum_nt2, t2, kt2, tune32 = run_gpu(net, m2, rvec, "mixed", a.steps, a.warmup, dt, tune=tune_arg)
         dom2, per2, roof2 = roofline(kt2, n, BYTES_MIXED)
         mixed = dict(value=n * sum_nt2 / t2, unit="cell-substeps/s", sum_nt=sum_nt2,
                      ms_per_step=1e3 * t2 / a.steps, roofline=roof2, tune=tune32)
@@ -433,6 +502,9 @@ def main():
         ens = ensemble_section(a)
         ens["vs_fp64_single_gpu"] = ens["value"] / value
         log(f"ensemble (mixed) / FP64 single model: {ens['vs_fp64_single_gpu']:.2f}x")
+    spin = None
+    if not a.no_spinup:
+        spin = spinup_section(a)
     cpu = None
     if not a.no_cpu:
         log("CPU baseline (reference OpenMP)")
@@ -457,6 +529,7 @@ def main():
         "mixed": mixed,
         "fp64_nseq_order": order_cmp,
         "ensemble": ens,
+        "spinup": spin,
     }
     print(json.dumps(out), flush=True)
 
