@@ -95,6 +95,7 @@ MLP_FUNCS = {
     "cmf_mlp_destroy": (None, [H]),
     "cmf_mlp_last_error": (C.c_char_p, []),
     "cmf_mlp_param_count": (C.c_int64, [H]),
+    "cmf_mlp_set_loss": (I, [H, C.c_float]),
     "cmf_mlp_set_params": (I, [H, P]),
     "cmf_mlp_get_params": (I, [H, P, P]),
     "cmf_mlp_forward": (I, [H, P, C.c_int64, P]),

@@ -357,3 +357,38 @@ def concat_maps(ms) -> tuple:
         for k in _DAM:
             setattr(out, k, cat(k))
     return out, [m.n for m in ms]
+
+
+def basin_of(m: CMFMap) -> np.ndarray:
+    """Index of the mouth each cell drains to (its river basin)."""
+    b = np.full(m.n, -1, np.int64)
+    mouths = np.flatnonzero(m.next < 0)
+    b[mouths] = mouths
+    # NSEQ order puts every cell before its downstream cell: walk it backwards
+    order = np.arange(m.n)[::-1]
+    for i in order:
+        if b[i] < 0:
+            b[i] = b[m.next[i]]
+    return b
+
+
+def subset_map(m: CMFMap, cells: np.ndarray) -> CMFMap:
+    """The map restricted to `cells` (whole basins, ascending NSEQ order kept):
+    downstream links renumbered, upstream lists rebuilt. Options without paths,
+    levees or dams (the spin-up basins use the default physics)."""
+    cells = np.sort(np.asarray(cells, np.int64))
+    new = np.full(m.n, -1, np.int64)
+    new[cells] = np.arange(cells.size)
+    nxt = m.next[cells]
+    nn = np.where(nxt >= 0, new[np.where(nxt >= 0, nxt, 0)], nxt).astype(np.int32)
+    if np.any((nxt >= 0) & (nn < 0)):
+        raise ValueError("subset_map: cells must be whole basins")
+    n = cells.size
+    owners = np.flatnonzero(nn >= 0)
+    up_ptr, up_idx = _csr(n, nn[owners], owners)
+    s = lambda a: np.ascontiguousarray(a[cells])
+    sp = lambda a: np.ascontiguousarray(a[:, cells])
+    return CMFMap(m.nx, m.ny, n, int((nn >= 0).sum()), m.nlfp, m.opts, s(m.seqx), s(m.seqy), nn, up_ptr,
+                  up_idx if len(up_idx) else np.zeros(1, np.int32), s(m.grarea), s(m.elevtn), s(m.nxtdst),
+                  s(m.rivlen), s(m.rivwth), s(m.rivhgt), s(m.rivman), s(m.rivstomax), s(m.rivelv), s(m.dwnelv),
+                  sp(m.fldhgt), sp(m.fldstomax), sp(m.fldgrd), m.dfrcinc, s(m.i2mask))

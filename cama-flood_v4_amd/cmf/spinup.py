@@ -121,7 +121,7 @@ def spinup(model, clims: List[Climatology], offs: Sequence[int], tol: float = 1e
                     nsub += nt
             if not model.ensemble:
                 daily.append(glb[-1, 7] + glb[-1, 8])
-        st = model.get_state(["rivsto", "fldsto"])
+        st = model.get_state(["rivsto", "fldsto", "rivdph"])
         s = domain_storage(st, offs)
         ch = np.abs(s - hist[-1]) / np.maximum(s, 1e-30)
         hist.append(s)

@@ -15,10 +15,16 @@ Features (24 per cell, all standardised with training-set statistics):
   15 log mean upstream-accumulated runoff   16 CV of its monthly means   17 its seasonal amplitude
   18 log of its value on the first day of the year over its mean (the equilibrium is
      taken at the start of the climatological year)
-  19 log Manning depth of the accumulated runoff of the last 30 days of the year
-     ((Q n / (W sqrt(slope)))^(3/5): the steady uniform-flow depth the channel
-     relaxes to)   20 log downstream distance   21 log RIVSTOMAX
-  22 log1p(top floodplain capacity / RIVSTOMAX)   23 mouth flag
+  19 log Manning depth at the cell's spill point of the accumulated runoff of the
+     last 30 days of the year ((Q n / (W sqrt(slope)))^(3/5): the depth that carries
+     the flow out of the flat reach)   20 log1p length [km] of the flat reach
+     downstream of the cell (slope < 2e-5, up to the spill point)
+  21 asinh of the cell's elevation above its spill point [m]
+  22 log1p(top floodplain capacity / RIVSTOMAX)   23 log1p flat area [km2] upstream
+  The spill point is the first cell downstream that is not flat (the cell itself
+  when it is not flat). Features 19-21 and 23 describe filled depressions (lakes):
+  their water level is set at the spill point, and their floodplain storage is
+  the slow part of a spin-up.
 Targets: log(RIVSTO / (RIVLEN*RIVWTH) + 1e-3) and log1p(FLDSTO / (RIVLEN*RIVWTH)).
 """
 from __future__ import annotations
@@ -36,6 +42,7 @@ from .spinup import Climatology, accumulate, days_to_equilibrium, make_climatolo
 
 N_IN, HIDDEN, N_OUT = 24, 256, 2
 EPS_DEPTH = 1e-3
+FLAT_SLOPE = 2e-5
 
 
 class SurrogateError(RuntimeError):
@@ -69,13 +76,36 @@ def features(m: CMFMap, clim: Climatology) -> np.ndarray:
     day0 = accumulate(m, clim.daily(0))
     X[:, 18] = np.log(np.maximum(day0, 1e-12) / np.maximum(mean, 1e-12))
     q30 = accumulate(m, np.mean([clim.daily(d) for d in range(335, 365)], axis=0))
-    hman = (q30 * m.rivman / (m.rivwth * np.sqrt(slope))) ** 0.6
+    flat = slope < FLAT_SLOPE
+    reach, spill = flat_reach_km(m, flat)
+    hman = (q30[spill] * m.rivman[spill] / (m.rivwth[spill] * np.sqrt(slope[spill]))) ** 0.6
     X[:, 19] = np.log(hman + EPS_DEPTH)
-    X[:, 20] = np.log(m.nxtdst)
-    X[:, 21] = np.log(m.rivstomax)
+    X[:, 20] = np.log1p(reach)
+    X[:, 21] = np.arcsinh(m.elevtn - m.elevtn[spill])
     X[:, 22] = np.log1p(m.fldstomax[-1] / m.rivstomax)
-    X[:, 23] = (~riv).astype(np.float64)
+    X[:, 23] = np.log1p(accumulate(m, np.where(flat, m.grarea, 0.0)) * 1e-6)
     return X.astype(np.float32)
+
+
+def flat_reach_km(m: CMFMap, flat: np.ndarray):
+    """(length [km] of the flat reach from each cell down to its spill point, the
+    spill point): the spill point is the first cell downstream that is not flat,
+    or the mouth; a cell that is not flat is its own spill point with length 0.
+    NSEQ order puts every cell before its downstream cell, so one backward pass
+    suffices."""
+    nxt = m.next
+    L = np.zeros(m.n)
+    spill = np.arange(m.n)
+    d = m.nxtdst * 1e-3
+    for i in range(m.n - 1, -1, -1):
+        if flat[i]:
+            j = nxt[i]
+            if j >= 0:
+                L[i] = d[i] + L[j]
+                spill[i] = spill[j]
+            else:
+                L[i] = d[i]
+    return L, spill
 
 
 def targets(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray) -> np.ndarray:
@@ -143,6 +173,10 @@ class MLP:
         y = np.empty((a.shape[0], self.n_out), np.float32)
         self._check(self.lib.cmf_mlp_forward(self.h, a.ctypes.data, a.shape[0], y.ctypes.data))
         return y
+
+    def set_loss(self, w_over: float) -> None:
+        """Weight of over-predictions in the squared-error loss (1 = plain MSE)."""
+        self._check(self.lib.cmf_mlp_set_loss(self.h, float(w_over)))
 
     def grad(self, x: np.ndarray, t: np.ndarray) -> float:
         a = np.ascontiguousarray(x, np.float32)
@@ -214,11 +248,14 @@ class Surrogate:
 
 
 def train(X: np.ndarray, Y: np.ndarray, epochs: int = 40, batch: int = 1024, lr: float = 1e-3, seed: int = 0,
-          log=None):
-    """Standardise, train on the device, return (Surrogate, loss history)."""
+          log=None, w_over: float = 1.0):
+    """Standardise, train on the device, return (Surrogate, loss history).
+    w_over > 1 weights over-predictions more: a warm start that errs low fills up
+    at the inflow rate, one that errs high in a filled depression drains slowly."""
     x_mu, x_sd = X.mean(axis=0), X.std(axis=0) + 1e-6
     y_mu, y_sd = Y.mean(axis=0), Y.std(axis=0) + 1e-6
     net = MLP(max_batch=max(batch, 1024), seed=seed)
+    net.set_loss(w_over)
     hist = net.fit((X - x_mu) / x_sd, (Y - y_mu) / y_sd, epochs, batch, lr, seed, log)
     s = Surrogate(net.params(), x_mu.astype(np.float32), x_sd.astype(np.float32), y_mu.astype(np.float32),
                   y_sd.astype(np.float32))
@@ -234,6 +271,37 @@ class Basin:
     net: object
     m: CMFMap
     clim: Climatology
+
+
+def basins_from_networks(nets: int, per_net: int, seed: int, nx: int = 1440, ny: int = 720,
+                         min_cells: int = 30):
+    """River basins cut from glb_15min-shaped global networks: the `per_net`
+    largest basins of each, so the set spans small coastal basins to continental
+    ones with filled depressions. Every basin gets its own runoff climatology on
+    the network's gamma(k=0.5) spatial field: mean 0.3-6 mm/day, seasonal
+    amplitude 0-0.9, random phase and month-to-month factors."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nets):
+        s_ = seed * 100 + k
+        net = synth.make_network(nx, ny, seed=s_)
+        m = maps.build_map(net, maps.Options())
+        base = m.vec(synth.make_runoff(net, 1, s_, mean_mm_day=1.0)[0])        # m3/s at 1 mm/day mean
+        b = maps.basin_of(m)
+        ids, cnt = np.unique(b, return_counts=True)
+        order = np.argsort(-cnt)
+        for i in order[:per_net]:
+            if cnt[i] < min_cells:
+                break
+            cells = np.flatnonzero(b == ids[i])
+            sub = maps.subset_map(m, cells)
+            mean = float(np.exp(rng.uniform(np.log(0.3), np.log(6.0))))
+            amp, phase = float(rng.uniform(0.0, 0.9)), float(rng.uniform(0.0, 12.0))
+            cyc = 1.0 + amp * np.sin(2.0 * np.pi * (np.arange(12) - phase) / 12.0)
+            fac = np.exp(rng.normal(-0.02, 0.2, 12))
+            monthly = np.maximum(cyc * fac, 0.0)[:, None] * (mean * base[cells])[None, :]
+            out.append(Basin(net, sub, Climatology(np.ascontiguousarray(monthly), mean, amp, phase)))
+    return out
 
 
 def make_basins(count: int, seed: int, cells=(400, 3000), res: float = 0.25, opts=None):
@@ -306,27 +374,56 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None)
     return X, Y, res
 
 
-def compare_spinup(sur: Surrogate, basin: Basin, years: int = 3, tol: float = 1e-3, log=None) -> dict:
-    """Cold-start and warm-start FP64 spin-ups of one basin. The cold start runs
-    `years` years; its last year is the periodic equilibrium trajectory that both
-    runs are judged against."""
+def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1e-3, log=None) -> dict:
+    """Cold-start and warm-start FP64 spin-ups of one network, run together as
+    two members of one handle (same map and forcing, own adaptive steps), until
+    both annual storage changes are below tol or `years` pass. The equilibrium
+    that both are judged against is the cold run's last year."""
+    import time
     from .model import CMFModel
+    from .spinup import DAYS
     m, clim = basin.m, basin.clim
-    out = {}
     riv, fld = sur.warm_start(m, clim)
-    for name, st in (("cold", cold_state(m)), ("warm", warm_state(m, riv, fld))):
-        g = CMFModel(m, "fp64")
-        g.set_state(st)
-        r = spinup(g, [clim], [0, m.n], tol=tol, max_years=years, stop_tol=0.0, log=log)
-        g.close()
-        out[name] = r
-    eq = out["cold"].daily[-365:]
-    s_eq0 = out["cold"].storage[-1, 0]
+    cold, warm = cold_state(m), warm_state(m, riv, fld)
+    model = CMFModel(m, "fp64", members=2, dom_cells=[m.n])
+    model.set_state({k: np.stack([cold[k], warm[k]]) for k in ("rivsto", "fldsto", "rivdph_pre")})
+    model.physics_fldstg()
+    t0 = time.time()
+
+    def total():
+        st = model.get_state(["rivsto", "fldsto"])
+        return (st["rivsto"] + st["fldsto"]).sum(axis=1)
+
+    s0 = total()
+    hist, daily = [s0], []
+    conv = [0, 0]
+    one = np.ones(2)
+    for y in range(1, years + 1):
+        for d in range(DAYS):
+            model.set_runoff_scaled(clim.daily(d), one)
+            model.ensemble_advance(86400.0)
+            daily.append(total())
+        s_ = hist[-1]
+        now = daily[-1]
+        ch = np.abs(now - s_) / now
+        hist.append(now)
+        for k in range(2):
+            if conv[k] == 0 and ch[k] < tol:
+                conv[k] = y
+        if log:
+            log(f"year {y}: annual change cold {ch[0]:.3e} warm {ch[1]:.3e} ({time.time() - t0:.1f}s)")
+        if conv[0] and conv[1]:
+            break
+    model.close()
+    daily = np.asarray(daily)
+    hist = np.asarray(hist)
+    eq = daily[-DAYS:, 0]
     res = {}
-    for name in ("cold", "warm"):
-        r = out[name]
-        res[name] = dict(years=int(r.years[0]), days=days_to_equilibrium(r.daily, eq, tol),
-                         init_rel_err=float(abs(r.storage[0, 0] - s_eq0) / s_eq0),
-                         annual_change=[float(abs(r.storage[y, 0] - r.storage[y - 1, 0]) / r.storage[y, 0])
-                                        for y in range(1, r.storage.shape[0])])
+    for k, name in enumerate(("cold", "warm")):
+        res[name] = dict(years=int(conv[k]), days=days_to_equilibrium(daily[:, k], eq, tol),
+                         init_rel_err=float(abs(hist[0, k] - hist[-1, 0]) / hist[-1, 0]),
+                         annual_change=[float(abs(hist[y, k] - hist[y - 1, k]) / hist[y, k])
+                                        for y in range(1, hist.shape[0])])
+    res["years_run"] = int(hist.shape[0] - 1)
+    res["seconds"] = round(time.time() - t0, 1)
     return res

@@ -166,17 +166,19 @@ __global__ void k_bias(float* __restrict__ z, const float* __restrict__ b, int r
     if (i < (long long)rows * cols) z[i] = z[i] + b[i % cols];
 }
 
-// mean squared error over rows*cols values: G = bf16(2 (Y - T) / (rows*cols)),
-// per-block partial sums of (Y - T)^2 into part[blockIdx.x]
+// weighted mean squared error over rows*cols values, weight w_over where the
+// prediction exceeds the target and 1 elsewhere: G = bf16(2 w (Y - T) / (rows*cols)),
+// per-block partial sums of w (Y - T)^2 into part[blockIdx.x]
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t, int n,
-                                             uint16_t* __restrict__ g, float* __restrict__ part) {
+                                             float w_over, uint16_t* __restrict__ g, float* __restrict__ part) {
     __shared__ float red[256];
     const int i = blockIdx.x * 256 + threadIdx.x;
     float e2 = 0.f;
     if (i < n) {
         const float d = y[i] - t[i];
-        e2 = d * d;
-        g[i] = f2bf(2.f * d / (float)n);
+        const float w = d > 0.f ? w_over : 1.f;
+        e2 = w * (d * d);
+        g[i] = f2bf((2.f * w) * d / (float)n);
     }
     red[threadIdx.x] = e2;
     __syncthreads();
@@ -257,6 +259,7 @@ struct cmf_mlp {
     float *tx = nullptr, *tt = nullptr;
     long long tn = 0;
     float* h_scratch = nullptr;            // pinned
+    float w_over = 1.f;                    // loss weight of over-predictions
 };
 
 namespace {
@@ -303,7 +306,7 @@ int grad_batch(cmf_mlp* h, int rows, float* loss) {
     forward(h, rows);
     const int ny = rows * h->n_out;
     const unsigned nb = nblk(ny);
-    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, h->stream, h->y, h->t, ny, h->gy, h->part);
+    hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, h->stream, h->y, h->t, ny, h->w_over, h->gy, h->part);
     // output layer: dW4 = H3^T GY, db4 = colsum GY, D = GY W4^T
     const uint16_t* G = h->gy;
     int gcols = h->n_out;
@@ -394,6 +397,12 @@ void cmf_mlp_destroy(cmf_mlp_handle h) {
 }
 
 int64_t cmf_mlp_param_count(cmf_mlp_handle h) { return h ? h->np : -1; }
+
+int cmf_mlp_set_loss(cmf_mlp_handle h, float w_over) {
+    if (!h || !(w_over > 0.f)) return mfail(CMF_MLP_EINVAL, "loss weight must be positive");
+    h->w_over = w_over;
+    return 0;
+}
 
 int cmf_mlp_set_params(cmf_mlp_handle h, const float* params) {
     if (!h || !params) return mfail(CMF_MLP_EINVAL, "NULL argument");

@@ -31,6 +31,11 @@ void cmf_mlp_destroy(cmf_mlp_handle h);
 const char* cmf_mlp_last_error(void);
 int64_t cmf_mlp_param_count(cmf_mlp_handle h);
 
+/* Loss of cmf_mlp_grad / cmf_mlp_train: mean of w (y - t)^2 with w = w_over where
+ * y > t and 1 elsewhere (w_over = 1, the default, is the mean squared error; a
+ * larger w_over trains a low-biased predictor). */
+int cmf_mlp_set_loss(cmf_mlp_handle h, float w_over);
+
 /* Load FP32 parameters (resets the Adam moments); read parameters and/or the
  * gradient of the last cmf_mlp_grad / training batch (either pointer may be NULL). */
 int cmf_mlp_set_params(cmf_mlp_handle h, const float* params);

@@ -5,7 +5,8 @@ Restates, in numpy, the forward and backward pass of the per-cell perceptron
 n_in -> H -> H -> H -> n_out (ReLU) with the same BF16 quantisation points as the
 HIP kernels (cama-flood_v4_amd/csrc/hip/cmf_mlp.hip):
   X_q = bf16(X), W_q = bf16(W); Z_l = H_{l-1} W_q,l + b_l; H_l = bf16(relu(Z_l))
-  Y = H_3 W_q,4 + b_4;  L = mean((Y - T)^2);  G_Y = bf16(2 (Y - T) / n)
+  Y = H_3 W_q,4 + b_4;  L = mean(w (Y - T)^2);  G_Y = bf16(2 w (Y - T) / n),
+  w = w_over where Y > T, else 1
   dW_l = H_{l-1}^T G_l;  db_l = sum_rows G_l;  G_{l-1} = bf16((G_l W_q,l^T) [H_{l-1} > 0])
 With quant=False every bf16() is the identity and the arithmetic is float64: that
 form is pinned by central finite differences of the loss (tests/test_mlp_oracle.py).
@@ -78,16 +79,18 @@ def forward(p, x, n_in, hidden, n_out, quant=True):
             return z, hs
 
 
-def loss_grad(p, x, t, n_in, hidden, n_out, quant=True):
-    """(loss, packed gradient) of the mean squared error."""
+def loss_grad(p, x, t, n_in, hidden, n_out, quant=True, w_over=1.0):
+    """(loss, packed gradient) of the weighted mean squared error: weight w_over
+    where the prediction exceeds the target, 1 elsewhere."""
     q = bf16 if quant else (lambda a: np.asarray(a, np.float64))
     ft = np.float32 if quant else np.float64
     layers = unpack(np.asarray(p, ft), n_in, hidden, n_out)
     y, hs = forward(p, x, n_in, hidden, n_out, quant)
     d = y.astype(np.float64) - np.asarray(t, np.float64)
     n = d.size
-    loss = float(np.mean(d * d))
-    g = q((2.0 * d / n).astype(ft))
+    w = np.where(d > 0, w_over, 1.0)
+    loss = float(np.mean(w * d * d))
+    g = q((2.0 * w * d / n).astype(ft))
     grads = [None] * 4
     for l in range(3, -1, -1):
         hin = hs[l].astype(np.float64)

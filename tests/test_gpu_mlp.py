@@ -70,14 +70,16 @@ def test_forward_matches_oracle():
 
 
 @pytest.mark.gpu
-def test_gradient_matches_oracle():
+@pytest.mark.parametrize("w_over", [1.0, 4.0])
+def test_gradient_matches_oracle(w_over):
     from cmf.surrogate import MLP
     p, x, t = _net(2)
     net = MLP(max_batch=512)
     net.set_params(p)
+    net.set_loss(w_over)
     loss = net.grad(x, t)
     g = net.grads()
-    lo, go = mo.loss_grad(p, x, t, 24, 256, 2)
+    lo, go = mo.loss_grad(p, x, t, 24, 256, 2, w_over=w_over)
     assert abs(loss - lo) <= 1e-4 * abs(lo)
     o = 0
     for l, (di, do) in enumerate(mo.layer_dims(24, 256, 2)):

@@ -69,3 +69,4 @@ def test_surrogate_features_finite():
     y = surrogate.targets(m, m.rivstomax * 0.5, m.rivstomax * 0.1)
     riv, fld = surrogate.storages(m, y)
     assert np.allclose(riv, m.rivstomax * 0.5, rtol=1e-5) and np.allclose(fld, m.rivstomax * 0.1, rtol=1e-4)
+

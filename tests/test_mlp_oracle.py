@@ -63,3 +63,23 @@ def test_bf16_gradient_close_to_exact():
     _, gq = mo.loss_grad(p, x, t, n_in, hid, n_out, quant=True)
     _, ge = mo.loss_grad(p.astype(np.float64), x, t, n_in, hid, n_out, quant=False)
     assert np.linalg.norm(gq - ge) / np.linalg.norm(ge) < 3e-2
+
+
+def test_weighted_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(7)
+    n_in, hid, n_out, rows = 4, 6, 2, 5
+    p = mo.init_params(n_in, hid, n_out, 3).astype(np.float64)
+    p += rng.normal(0, 0.05, p.size)                 # non-zero biases
+    x = rng.normal(size=(rows, n_in))
+    t = rng.normal(size=(rows, n_out))
+    loss, g = mo.loss_grad(p, x, t, n_in, hid, n_out, quant=False, w_over=5.0)
+    y, _ = mo.forward(p, x, n_in, hid, n_out, quant=False)
+    d = y - t
+    assert np.isclose(loss, np.mean(np.where(d > 0, 5.0, 1.0) * d * d))
+    eps = 1e-6
+    for k in rng.choice(p.size, 40, replace=False):
+        e = np.zeros_like(p)
+        e[k] = eps
+        fd = (mo.loss_grad(p + e, x, t, n_in, hid, n_out, quant=False, w_over=5.0)[0]
+              - mo.loss_grad(p - e, x, t, n_in, hid, n_out, quant=False, w_over=5.0)[0]) / (2 * eps)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
