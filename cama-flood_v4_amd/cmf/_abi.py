@@ -8,6 +8,13 @@ I = C.c_int
 I32P = C.POINTER(C.c_int32)
 DP = C.POINTER(C.c_double)
 
+class cmfh_channel(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nlfp", C.c_int32), ("lfldout", C.c_int32), ("next", P)] + \
+               [(k, P) for k in ("elevtn rivelv rivhgt rivwth rivlen rivman nxtdst grarea dwnelv "
+                                 "rivstomax fldstomax fldgrd").split()] + \
+               [(k, C.c_double) for k in ("dfrcinc", "pmanfld", "pdstmth")]
+
+
 HOST_FUNCS = {
     "cmfh_calc_1d_seq": (I, [I, I, P, P, P, P, P, P]),
     "cmfh_build_upstream_csr": (I, [I, I, P, P, P]),
@@ -17,6 +24,7 @@ HOST_FUNCS = {
     "cmfh_priority_flood": (I, [I, I, P, P, C.c_float, P, P, P]),
     "cmfh_upstream_area": (I, [I, I, P, P, P, P]),
     "cmfh_accumulate": (I, [I, P, I, P, P]),
+    "cmfh_steady_state": (I, [C.POINTER(cmfh_channel), P, P, P, P, P, P]),
 }
 
 

@@ -14,6 +14,7 @@ daily runoff is the periodic linear interpolation between month centres.
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 from typing import List, Optional, Sequence
 
@@ -69,6 +70,28 @@ def accumulate(m: CMFMap, vals: np.ndarray) -> np.ndarray:
     if rc != 0:
         raise RuntimeError(f"accumulate failed ({rc})")
     return out.reshape(np.shape(vals)) if np.ndim(vals) == 1 else out
+
+
+def steady_state(m: CMFMap, q: np.ndarray) -> dict:
+    """Steady state of the local-inertial outflow carrying the constant outflow
+    q [m3/s] out of every cell (include/cmf_host.h cmfh_steady_state): rivsto,
+    fldsto, sfcelv, rivout, fldout."""
+    from ._abi import cmfh_channel
+    lib = _native.host()
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    keep = {k: f64(getattr(m, k)) for k in ("elevtn", "rivelv", "rivhgt", "rivwth", "rivlen", "rivman",
+                                              "nxtdst", "grarea", "dwnelv", "rivstomax", "fldstomax", "fldgrd")}
+    nxt = np.ascontiguousarray(np.where(m.next >= 0, m.next, -1), np.int32)
+    ch = cmfh_channel(n=m.n, nlfp=m.nlfp, lfldout=int(m.opts.lfldout), next=ptr(nxt),
+                      dfrcinc=m.dfrcinc, pmanfld=m.opts.pmanfld, pdstmth=m.opts.pdstmth,
+                      **{k: ptr(v) for k, v in keep.items()})
+    qq = f64(q)
+    out = {k: np.empty(m.n) for k in ("rivsto", "fldsto", "sfcelv", "rivout", "fldout")}
+    rc = lib.cmfh_steady_state(C.byref(ch), ptr(qq), *(ptr(out[k]) for k in
+                                                        ("rivsto", "fldsto", "sfcelv", "rivout", "fldout")))
+    if rc != 0:
+        raise RuntimeError(f"steady state failed ({rc})")
+    return out
 
 
 def domain_storage(st: dict, offs: Sequence[int]) -> np.ndarray:

@@ -11,7 +11,11 @@ the spin-up and the judge of the warm start's drift.
 
 Features (24 per cell, all standardised with training-set statistics):
   0 log10 upstream area [km2]       1 log10 channel slope        2 log river width
-  3 log river depth (bank height)   4 log river length           5-14 log1p floodplain layer heights
+  3 log river depth (bank height)   4 log river length
+  5-10 log1p floodplain layer heights of layers 1, 2, 4, 6, 8, 10
+  11-12 the steady state of the outflow equations under the annual-mean
+     accumulated runoff (cmf.spinup.steady_state), in target units
+  13-14 the same under the mean of the last 7 days of the year
   15 log mean upstream-accumulated runoff   16 CV of its monthly means   17 its seasonal amplitude
   18 log of its value on the first day of the year over its mean (the equilibrium is
      taken at the start of the climatological year)
@@ -38,11 +42,13 @@ import numpy as np
 from . import _native
 from . import maps, synth
 from .maps import CMFMap
-from .spinup import Climatology, accumulate, days_to_equilibrium, make_climatology, spinup
+from .spinup import Climatology, accumulate, days_to_equilibrium, make_climatology, spinup, steady_state
 
 N_IN, HIDDEN, N_OUT = 24, 256, 2
 EPS_DEPTH = 1e-3
 FLAT_SLOPE = 2e-5
+LAYERS = (0, 1, 3, 5, 7, 9)      # floodplain layers whose heights are features 5-10
+RECENT_DAYS = 7
 
 
 class SurrogateError(RuntimeError):
@@ -68,8 +74,13 @@ def features(m: CMFMap, clim: Climatology) -> np.ndarray:
     X[:, 2] = np.log(m.rivwth)
     X[:, 3] = np.log(m.rivhgt)
     X[:, 4] = np.log(m.rivlen)
-    for l in range(10):
-        X[:, 5 + l] = np.log1p(fl[min(l, fl.shape[0] - 1)])
+    for k, l in enumerate(LAYERS):
+        X[:, 5 + k] = np.log1p(fl[min(l, fl.shape[0] - 1)])
+    # physical priors: steady states under the annual-mean and the recent flow
+    for k, q in enumerate((mean, accumulate(m, np.mean([clim.daily(d) for d in range(365 - RECENT_DAYS, 365)],
+                                                            axis=0)))):
+        ss = steady_state(m, q)
+        X[:, 11 + 2 * k:13 + 2 * k] = targets(m, ss["rivsto"], ss["fldsto"])
     X[:, 15] = np.log(np.maximum(mean, 1e-12))
     X[:, 16] = cv
     X[:, 17] = amp
@@ -224,14 +235,17 @@ class Surrogate:
     x_sd: np.ndarray
     y_mu: np.ndarray
     y_sd: np.ndarray
+    base: int = -1        # >= 0: the network predicts the residual over features base, base+1
 
     def save(self, path: str) -> None:
-        np.savez(path, params=self.params, x_mu=self.x_mu, x_sd=self.x_sd, y_mu=self.y_mu, y_sd=self.y_sd)
+        np.savez(path, params=self.params, x_mu=self.x_mu, x_sd=self.x_sd, y_mu=self.y_mu, y_sd=self.y_sd,
+                 base=np.int64(self.base))
 
     @staticmethod
     def load(path: str) -> "Surrogate":
         z = np.load(path, allow_pickle=False)
-        return Surrogate(z["params"], z["x_mu"], z["x_sd"], z["y_mu"], z["y_sd"])
+        return Surrogate(z["params"], z["x_mu"], z["x_sd"], z["y_mu"], z["y_sd"],
+                         int(z["base"]) if "base" in z else -1)
 
     def predict(self, X: np.ndarray, net: Optional[MLP] = None) -> np.ndarray:
         own = net is None
@@ -240,7 +254,10 @@ class Surrogate:
         y = net.forward((X - self.x_mu) / self.x_sd)
         if own:
             net.close()
-        return y * self.y_sd + self.y_mu
+        y = y * self.y_sd + self.y_mu
+        if self.base >= 0:
+            y = y + X[:, self.base:self.base + N_OUT]
+        return y
 
     def warm_start(self, m: CMFMap, clim: Climatology, net: Optional[MLP] = None):
         """Predicted equilibrium (rivsto, fldsto) of every cell."""
@@ -248,17 +265,20 @@ class Surrogate:
 
 
 def train(X: np.ndarray, Y: np.ndarray, epochs: int = 40, batch: int = 1024, lr: float = 1e-3, seed: int = 0,
-          log=None, w_over: float = 1.0):
+          log=None, w_over: float = 1.0, base: int = -1):
     """Standardise, train on the device, return (Surrogate, loss history).
     w_over > 1 weights over-predictions more: a warm start that errs low fills up
-    at the inflow rate, one that errs high in a filled depression drains slowly."""
+    at the inflow rate, one that errs high in a filled depression drains slowly.
+    base >= 0: learn the residual over the prior in features base, base+1."""
+    if base >= 0:
+        Y = Y - X[:, base:base + N_OUT]
     x_mu, x_sd = X.mean(axis=0), X.std(axis=0) + 1e-6
     y_mu, y_sd = Y.mean(axis=0), Y.std(axis=0) + 1e-6
     net = MLP(max_batch=max(batch, 1024), seed=seed)
     net.set_loss(w_over)
     hist = net.fit((X - x_mu) / x_sd, (Y - y_mu) / y_sd, epochs, batch, lr, seed, log)
     s = Surrogate(net.params(), x_mu.astype(np.float32), x_sd.astype(np.float32), y_mu.astype(np.float32),
-                  y_sd.astype(np.float32))
+                  y_sd.astype(np.float32), base)
     net.close()
     return s, hist
 
@@ -353,40 +373,61 @@ def warm_state(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray, precision: str
     return st
 
 
-def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None):
-    """Batched FP64 spin-up of every basin from its cold start (one handle, one
-    domain per basin) to an annual storage change below tol; returns the
-    feature matrix, the targets and the per-basin spin-up years."""
+def prior_state(m: CMFMap, clim: Climatology, X: Optional[np.ndarray] = None) -> dict:
+    """Warm start from the physical prior alone: the steady state under the
+    annual-mean accumulated runoff (features 11-12)."""
+    X = features(m, clim) if X is None else X
+    riv, fld = storages(m, X[:, 11:13])
+    return warm_state(m, riv, fld)
+
+
+def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None, start: str = "prior"):
+    """Batched FP64 spin-up of every basin (one handle, one domain per basin) to
+    an annual storage change below tol, from the steady-state prior (start =
+    "prior") or from the cold start ("cold"); returns the feature matrix, the
+    targets and the per-basin spin-up years. The equilibrium does not depend on
+    the start; the prior only gets there sooner."""
     from .model import CMFModel
     cm, dom_cells = maps.concat_maps([b.m for b in basins])
     offs = np.cumsum([0] + dom_cells)
+    X = np.concatenate([features(b.m, b.clim) for b in basins])
     init = {}
-    sts = [cold_state(b.m) for b in basins]
+    if start == "prior":
+        sts = [prior_state(b.m, b.clim, X[offs[d]:offs[d + 1]]) for d, b in enumerate(basins)]
+    elif start == "cold":
+        sts = [cold_state(b.m) for b in basins]
+    else:
+        raise ValueError(f"start must be 'prior' or 'cold', not {start!r}")
     for k in ("rivsto", "fldsto", "rivdph_pre"):
         init[k] = np.concatenate([s[k] for s in sts])
     model = CMFModel(cm, "fp64", members=1, dom_cells=dom_cells)
     model.set_state(init)
     res = spinup(model, [b.clim for b in basins], offs, tol=1e-3, stop_tol=tol, max_years=max_years, log=log)
     model.close()
-    X = np.concatenate([features(b.m, b.clim) for b in basins])
     Y = np.concatenate([targets(b.m, res.state["rivsto"][offs[d]:offs[d + 1]],
                                 res.state["fldsto"][offs[d]:offs[d + 1]]) for d, b in enumerate(basins)])
     return X, Y, res
 
 
 def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1e-3, log=None) -> dict:
-    """Cold-start and warm-start FP64 spin-ups of one network, run together as
-    two members of one handle (same map and forcing, own adaptive steps), until
-    both annual storage changes are below tol or `years` pass. The equilibrium
-    that both are judged against is the cold run's last year."""
+    """FP64 spin-ups of one network from the cold start, from the steady-state
+    prior alone and from the surrogate's warm start, run together as members of
+    one handle (same map and forcing, own adaptive steps; a fourth member repeats
+    the warm start to make the member count a power of two), until every annual
+    storage change is below tol or `years` pass. The equilibrium all are judged
+    against is the last year of the member whose last annual change is smallest."""
     import time
     from .model import CMFModel
     from .spinup import DAYS
     m, clim = basin.m, basin.clim
-    riv, fld = sur.warm_start(m, clim)
-    cold, warm = cold_state(m), warm_state(m, riv, fld)
-    model = CMFModel(m, "fp64", members=2, dom_cells=[m.n])
-    model.set_state({k: np.stack([cold[k], warm[k]]) for k in ("rivsto", "fldsto", "rivdph_pre")})
+    X = features(m, clim)
+    riv, fld = storages(m, sur.predict(X))
+    names = ("cold", "prior", "warm")
+    starts = [cold_state(m), prior_state(m, clim, X), warm_state(m, riv, fld)]
+    starts.append(starts[-1])
+    M = len(starts)
+    model = CMFModel(m, "fp64", members=M, dom_cells=[m.n])
+    model.set_state({k: np.stack([st[k] for st in starts]) for k in ("rivsto", "fldsto", "rivdph_pre")})
     model.physics_fldstg()
     t0 = time.time()
 
@@ -394,34 +435,35 @@ def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1
         st = model.get_state(["rivsto", "fldsto"])
         return (st["rivsto"] + st["fldsto"]).sum(axis=1)
 
-    s0 = total()
-    hist, daily = [s0], []
-    conv = [0, 0]
-    one = np.ones(2)
+    hist, daily = [total()], []
+    conv = [0] * len(names)
+    one = np.ones(M)
     for y in range(1, years + 1):
         for d in range(DAYS):
             model.set_runoff_scaled(clim.daily(d), one)
             model.ensemble_advance(86400.0)
             daily.append(total())
-        s_ = hist[-1]
         now = daily[-1]
-        ch = np.abs(now - s_) / now
+        ch = np.abs(now - hist[-1]) / now
         hist.append(now)
-        for k in range(2):
+        for k in range(len(names)):
             if conv[k] == 0 and ch[k] < tol:
                 conv[k] = y
         if log:
-            log(f"year {y}: annual change cold {ch[0]:.3e} warm {ch[1]:.3e} ({time.time() - t0:.1f}s)")
-        if conv[0] and conv[1]:
+            log(f"year {y}: annual change " + " ".join(f"{nm} {ch[k]:.3e}" for k, nm in enumerate(names)) +
+                f" ({time.time() - t0:.1f}s)")
+        if all(conv):
             break
     model.close()
     daily = np.asarray(daily)
     hist = np.asarray(hist)
-    eq = daily[-DAYS:, 0]
-    res = {}
-    for k, name in enumerate(("cold", "warm")):
+    last = np.abs(hist[-1] - hist[-2]) / hist[-1]
+    ref = int(np.argmin(last[:len(names)]))
+    eq = daily[-DAYS:, ref]
+    res = {"reference_member": names[ref]}
+    for k, name in enumerate(names):
         res[name] = dict(years=int(conv[k]), days=days_to_equilibrium(daily[:, k], eq, tol),
-                         init_rel_err=float(abs(hist[0, k] - hist[-1, 0]) / hist[-1, 0]),
+                         init_rel_err=float(abs(hist[0, k] - hist[-1, ref]) / hist[-1, ref]),
                          annual_change=[float(abs(hist[y, k] - hist[y - 1, k]) / hist[y, k])
                                         for y in range(1, hist.shape[0])])
     res["years_run"] = int(hist.shape[0] - 1)

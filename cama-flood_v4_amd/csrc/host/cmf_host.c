@@ -404,3 +404,113 @@ int cmfh_accumulate(int n, const int32_t* next, int nrec, const double* val, dou
     free(indeg); free(order);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Steady state of the local-inertial outflow under constant flows (no reference
+ * counterpart; a physical prior for the spin-up surrogate).
+ *
+ * With RIVOUT = RIVOUT_PRE and equal current/previous depths, the river update of
+ * src/cmf_calc_outflw_mod.F90:60-80 is a fixed point exactly when
+ *   Q = W h^(5/3) S^(1/2) / n        (h = flow depth, S = water-surface slope)
+ * and the floodplain update (:84-116) when
+ *   Qf = A hf^(2/3) min(S, 0.005)^(1/2) / n_fld
+ * with A = FLDSTO/RIVLEN - FLDDPH*RIVWTH and hf the depth above ELEVTN. Cells are
+ * visited downstream first; each takes the surface elevation at which its outflow
+ * equals q[i], found by bisection (the outflow grows monotonically with it), given
+ * the downstream cell's elevation (DWNELV at a mouth, slope over PDSTMTH). */
+static double steady_storage(const cmfh_channel* c, int i, double d, double* fldsto, double* flddph)
+{
+    const int n = c->n;
+    const double rivlen = c->rivlen[i], rivwth = c->rivwth[i];
+    const double rivstomax = c->rivstomax[i];
+    if (d <= c->rivhgt[i]) { *fldsto = 0.0; *flddph = 0.0; return d * rivlen * rivwth; }
+    const double f = d - c->rivhgt[i];
+    const double dwth_inc = (c->grarea[i] / rivlen) * c->dfrcinc;
+    double dsto_fil = rivstomax, dwth_fil = rivwth, ddph_fil = 0.0, sto;
+    int l = 0;
+    for (; l < c->nlfp; ++l) {
+        const double grd = c->fldgrd[(size_t)l * n + i];
+        const double top = ddph_fil + grd * dwth_inc;
+        if (f <= top && grd > 0.0) break;
+        dsto_fil = c->fldstomax[(size_t)l * n + i];
+        dwth_fil += dwth_inc;
+        ddph_fil = top;
+    }
+    if (l == c->nlfp) {
+        sto = dsto_fil + (f - ddph_fil) * dwth_fil * rivlen;
+    } else {
+        const double grd = c->fldgrd[(size_t)l * n + i];
+        const double dwth_add = (f - ddph_fil) / grd;
+        sto = dsto_fil + rivlen * grd * (dwth_fil * dwth_add + 0.5 * dwth_add * dwth_add);
+    }
+    const double rs = rivstomax + rivlen * rivwth * f;
+    *fldsto = dmax(sto - rs, 0.0);
+    *flddph = f;
+    return sto;
+}
+
+static double steady_flow(const cmfh_channel* c, int i, double s, double s_down, double* qfld)
+{
+    const double d = s - c->rivelv[i];
+    *qfld = 0.0;
+    if (d <= 0.0 || s <= s_down) return 0.0;
+    const int mouth = c->next[i] < 0;
+    const double slope = (s - s_down) / (mouth ? c->pdstmth : c->nxtdst[i]);
+    const double h = d;                          /* s > s_down: flow depth above RIVELV */
+    double q = c->rivwth[i] * pow(h, 5.0 / 3.0) * sqrt(slope) / c->rivman[i];
+    if (c->lfldout) {
+        double fsto, fdph;
+        (void)steady_storage(c, i, d, &fsto, &fdph);
+        const double a = dmax(fsto / c->rivlen[i] - fdph * c->rivwth[i], 0.0);
+        const double hf = dmax(s - c->elevtn[i], 0.0);
+        if (a > 1e-5 && hf > 1e-5) *qfld = a * pow(hf, 2.0 / 3.0) * sqrt(dmin(slope, 0.005)) / c->pmanfld;
+    }
+    return q + *qfld;
+}
+
+int cmfh_steady_state(const cmfh_channel* c, const double* q, double* rivsto, double* fldsto, double* sfcelv,
+                      double* rivout, double* fldout)
+{
+    if (!c || !q || !rivsto || !fldsto || !sfcelv || c->n <= 0 || c->nlfp <= 0) return -1;
+    const int n = c->n;
+    int32_t* indeg = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!indeg || !order) { free(indeg); free(order); return -2; }
+    for (int i = 0; i < n; ++i) {
+        if (c->next[i] >= n) { free(indeg); free(order); return -1; }
+        if (c->next[i] >= 0) indeg[c->next[i]] += 1;
+    }
+    int qh = 0, qt = 0;
+    for (int i = 0; i < n; ++i) if (indeg[i] == 0) order[qt++] = i;
+    while (qh < qt) {
+        const int i = order[qh++];
+        if (c->next[i] >= 0 && --indeg[c->next[i]] == 0) order[qt++] = c->next[i];
+    }
+    if (qt != n) { free(indeg); free(order); return -3; }
+    for (int k = n - 1; k >= 0; --k) {
+        const int i = order[k];
+        const double s_down = c->next[i] >= 0 ? sfcelv[c->next[i]] : c->dwnelv[i];
+        double lo = dmax(c->rivelv[i], s_down), s = lo, qf = 0.0;
+        if (q[i] > 0.0) {
+            double step = 1.0;
+            while (steady_flow(c, i, lo + step, s_down, &qf) < q[i] && step < 1e5) step *= 2.0;
+            double hi = lo + step;
+            for (int it = 0; it < 200 && hi - lo > 1e-12 * (1.0 + fabs(hi)); ++it) {
+                const double mid = 0.5 * (lo + hi);
+                if (mid <= lo || mid >= hi) break;
+                if (steady_flow(c, i, mid, s_down, &qf) < q[i]) lo = mid; else hi = mid;
+            }
+            s = 0.5 * (lo + hi);
+        }
+        sfcelv[i] = s;
+        const double qt = steady_flow(c, i, s, s_down, &qf);
+        if (rivout) rivout[i] = qt - qf;
+        if (fldout) fldout[i] = qf;
+        double fsto, fdph;
+        const double sto = steady_storage(c, i, s - c->rivelv[i], &fsto, &fdph);
+        fldsto[i] = fsto;
+        rivsto[i] = sto - fsto;
+    }
+    free(indeg); free(order);
+    return 0;
+}

@@ -183,7 +183,9 @@ int cmf_hip_get_tune(cmf_handle h, cmf_tune* t);
 /* Time nsub substeps of an outer step of length dt under each candidate
  * configuration, keep the fastest (also written to *best if not NULL) and
  * restore the model state. report (optional) receives 8 values per trial:
- * fused, block size, xcd, fused grid, outflow/rate/update ms, total ms.
+ * fused, block size, xcd, fused grid, outflow/rate/update ms, total ms (kernel
+ * event times; trials with fused = 2 / 3 are wall-clock times of the best
+ * configuration without / with graph replay).
  * Returns the number of trials (>= 0) or a negative CMF_E* code. */
 int cmf_hip_autotune(cmf_handle h, double dt, int32_t nsub, cmf_tune* best, double* report,
                      int32_t report_cap);

@@ -85,6 +85,25 @@ int cmfh_upstream_area(int nx, int ny, const int32_t* nextx, const int32_t* next
  * network has a cycle. */
 int cmfh_accumulate(int n, const int32_t* next, int nrec, const double* val, double* out);
 
+/* Channel description for cmfh_steady_state: per-cell arrays of n cells in any
+ * order (next[i] < 0 at mouths), floodplain tables layer-planar. */
+typedef struct {
+    int32_t n, nlfp, lfldout;
+    const int32_t* next;
+    const double *elevtn, *rivelv, *rivhgt, *rivwth, *rivlen, *rivman, *nxtdst, *grarea, *dwnelv;
+    const double *rivstomax, *fldstomax, *fldgrd;
+    double dfrcinc, pmanfld, pdstmth;
+} cmfh_channel;
+
+/* Steady state of the local-inertial river and floodplain outflow
+ * (src/cmf_calc_outflw_mod.F90:60-171 with current = previous values) that
+ * carries the constant outflow q[i] out of every cell: river and floodplain
+ * storage, surface elevation and the river / floodplain outflow split per cell
+ * (rivout, fldout may be NULL). No reference counterpart: the physical prior of
+ * the spin-up surrogate. Returns 0, -1 on bad arguments, -3 on a cycle. */
+int cmfh_steady_state(const cmfh_channel* c, const double* q, double* rivsto, double* fldsto, double* sfcelv,
+                      double* rivout, double* fldout);
+
 #ifdef __cplusplus
 }
 #endif

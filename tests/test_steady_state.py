@@ -1,0 +1,72 @@
+"""Steady state of the local-inertial outflow (cmfh_steady_state), checked against
+the oracle's restatement of the reference physics (CPU):
+
+* storage <-> depth: the storages it returns, passed through the reference's
+  stage inversion (CMF_PHYSICS_FLDSTG), give back its surface elevations;
+* fixed point: one hourly outer step of the reference physics from the steady
+  state under the runoff that produces its flows changes no storage, no outflow;
+* Manning closed form on a single in-bank mouth cell.
+"""
+import numpy as np
+import pytest
+
+from cmf import maps, spinup, synth
+
+
+def _basin(seed, mean):
+    net = synth.make_network(90, 45, seed=seed)
+    m = maps.build_map(net, maps.Options())
+    r = m.vec(synth.make_runoff(net, 1, seed, mean_mm_day=mean)[0])
+    return m, r
+
+
+@pytest.mark.parametrize("seed,mean", [(11, 1.5), (12, 40.0)])
+def test_steady_state_is_a_fixed_point_of_the_reference_physics(seed, mean):
+    from oracle.oracle import OracleModel
+    m, r = _basin(seed, mean)
+    q = spinup.accumulate(m, r)
+    ss = spinup.steady_state(m, q)
+    assert np.all(ss["rivsto"] >= 0) and np.all(ss["fldsto"] >= 0)
+    flooded = int((ss["fldsto"] > 0).sum())
+    if mean > 10:
+        assert flooded > 50                      # the wet case exercises the floodplain tables
+    st = maps.cold_start(m)
+    st["rivsto"], st["fldsto"] = ss["rivsto"].copy(), ss["fldsto"].copy()
+    om = OracleModel(m, st)
+    om.physics_fldstg()
+    # storage -> stage of the reference returns the solver's surface elevation
+    depth = ss["sfcelv"] - m.rivelv
+    assert np.max(np.abs(om.st["rivdph"] - depth) / np.maximum(depth, 1e-3)) < 1e-9
+    # previous-step values equal to the current ones: a steady flow
+    om.st["rivout_pre"][:] = ss["rivout"]
+    om.st["fldout_pre"][:] = ss["fldout"]
+    om.st["rivdph_pre"][:] = om.st["rivdph"]
+    om.st["fldsto_pre"][:] = om.st["fldsto"]
+    om.st["runoff"][:] = r
+    sto0 = om.st["rivsto"] + om.st["fldsto"]
+    om.physics_advance(3600.0)
+    sto1 = om.st["rivsto"] + om.st["fldsto"]
+    # per-cell storage change relative to the hour's throughput (q * 3600 s). Not
+    # 0: at sea-level mouths the surface slope over PDSTMTH is ~1e-9 m, so the
+    # rounding of RIVDPH = RIVSTO/RIVLEN/RIVWTH moves the outflow by ~1e-6
+    ch = np.abs(sto1 - sto0) / np.maximum(q * 3600.0, 1e-3)
+    assert ch.max() < 1e-4, ch.max()
+    assert np.median(ch) < 1e-9
+    out = om.st["rivout"] + om.st["fldout"]
+    assert np.max(np.abs(out - q) / np.maximum(q, 1e-9)) < 1e-5
+
+
+def test_manning_closed_form_single_cell():
+    """One in-bank mouth cell: Q = W h^(5/3) ((h + RIVELV - DWNELV) / PDSTMTH)^(1/2) / n."""
+    m, _ = _basin(11, 1.5)
+    i = int(np.flatnonzero(m.next < 0)[0])
+    sub = maps.subset_map(m, np.array([i]))
+    assert sub.n == 1 and sub.next[0] < 0
+    sub.dwnelv = sub.rivelv - 1.0                  # downstream water surface 1 m below the bed
+    h = 0.3 * float(sub.rivhgt[0])
+    s = float(sub.rivelv[0]) + h
+    q = float(sub.rivwth[0]) * h ** (5 / 3) * np.sqrt((s - float(sub.dwnelv[0])) / sub.opts.pdstmth) / float(sub.rivman[0])
+    ss = spinup.steady_state(sub, np.array([q]))
+    assert abs(ss["sfcelv"][0] - s) < 1e-9
+    assert abs(ss["rivsto"][0] - h * float(sub.rivlen[0] * sub.rivwth[0])) < 1e-9 * ss["rivsto"][0]
+    assert ss["fldsto"][0] == 0.0 and ss["fldout"][0] == 0.0
