@@ -353,24 +353,27 @@ def ensemble_section(a):
 
 
 def spinup_section(a):
-    """Learned spin-up warm start: labels from a batched FP64 spin-up of river
-    basins cut from glb_15min-shaped networks (own climatology per basin), training
-    by the project's BF16 matrix-core kernels, then cold-start vs warm-start FP64
-    spin-up of a held-out network (both as members of one handle)."""
+    """Learned spin-up warm start: labels from a batched FP64 spin-up of 600 river
+    basins cut from six 720x360 glb_15min-shaped networks (own climatology per
+    basin), training by the project's BF16 matrix-core kernels, then FP64
+    spin-ups of a held-out network from the cold start, the mean-flow steady
+    state, the physical prior and the learned warm start (four members of one
+    handle)."""
     from cmf import maps, synth
     from cmf import surrogate as sg
     from cmf.spinup import make_climatology
-    full = a.spinup_full
-    nx, ny = (1440, 720) if full else (720, 360)
-    nets, per = (4, 50) if full else (2, 100)
+    nx, ny = 720, 360
+    nets, per = 6, 100
     t0 = time.time()
     basins = sg.basins_from_networks(nets, per, seed=a.seed + 7, nx=nx, ny=ny)
-    X, Y, res = sg.equilibrium_labels(basins, tol=1e-4, max_years=40, log=lambda s_: log("labels " + s_))
+    X, Y, res = sg.equilibrium_labels(basins, tol=1e-5, max_years=40, log=lambda s_: log("labels " + s_))
     t_labels = time.time() - t0
     log(f"spin-up labels: {len(basins)} basins from {nets} {nx}x{ny} networks, {X.shape[0]} cells, "
-        f"cold-start years median {int(np.median(res.years))} max {int(res.years.max())}, {t_labels:.1f}s")
+        f"years to < 0.1% annual change from the prior: median {int(np.median(res.years))} "
+        f"max {int(res.years.max())}, {t_labels:.1f}s")
     t1 = time.time()
-    sur, hist = sg.train(X, Y, epochs=a.spin_epochs, batch=1024, lr=2e-3, seed=0, w_over=a.spin_w_over,
+    Xs, Ys, ws = sg.training_set(X, Y, res.state["rivsto"] + res.state["fldsto"])
+    sur, hist = sg.train(Xs, Ys, epochs=a.spin_epochs, batch=1024, lr=2e-3, seed=0, weights=ws,
                          log=lambda s_: log("train " + s_))
     t_train = time.time() - t1
     net = synth.make_network(nx, ny, seed=a.seed + 991)
@@ -389,26 +392,30 @@ def spinup_section(a):
     flops = 2.0 * big.shape[0] * (24 * 256 + 2 * 256 * 256 + 256 * 2)
     log(f"inference on 6e6 cells: {1e3 * t_inf:.1f} ms ({flops / t_inf / 1e12:.2f} TFLOP/s incl. host transfers)")
     cmp_ = sg.compare_spinup(sur, sg.Basin(net, m, clim), years=a.spin_years, log=lambda s_: log("spin-up " + s_))
-    c, w = cmp_["cold"], cmp_["warm"]
+    c, w, pr, sd = cmp_["cold"], cmp_["warm"], cmp_["prior"], cmp_["steady"]
     out = dict(
         application=f"glb_15min-shaped {nx}x{ny} held-out network, {m.n} cells, monthly gamma climatology "
                     f"repeated, DT=86400, criterion: annual storage change < 0.1%",
         cold_years=c["years"] or f">{cmp_['years_run']}", warm_years=w["years"] or f">{cmp_['years_run']}",
+        physics_prior_years=pr["years"] or f">{cmp_['years_run']}",
+        mean_flow_steady_years=sd["years"] or f">{cmp_['years_run']}",
         years_ratio=(c["years"] / w["years"]) if (c["years"] and w["years"]) else None,
-        cold_days_to_0p1pct=c["days"], warm_days_to_0p1pct=w["days"],
-        init_rel_err={"cold": c["init_rel_err"], "warm": w["init_rel_err"]},
-        first_year_change={"cold": c["annual_change"][0], "warm": w["annual_change"][0]},
+        days_to_0p1pct={k: v["days"] for k, v in (("cold", c), ("steady", sd), ("prior", pr), ("warm", w))},
+        init_rel_err={k: v["init_rel_err"] for k, v in (("cold", c), ("steady", sd), ("prior", pr), ("warm", w))},
+        annual_change={k: v["annual_change"] for k, v in (("cold", c), ("steady", sd), ("prior", pr),
+                                                           ("warm", w))},
+        equilibrium_reference=cmp_["reference_member"],
         training={"basins": len(basins), "cells": int(X.shape[0]), "epochs": a.spin_epochs,
-                  "loss_weight_over": a.spin_w_over,
+                  "corrected_cells": int(Xs.shape[0]),
                   "loss_first": hist[0], "loss_last": hist[-1], "label_seconds": round(t_labels, 1),
                   "train_seconds": round(t_train, 1)},
         inference={"cells": big.shape[0], "ms": round(1e3 * t_inf, 2),
                    "tflops_incl_transfers": round(flops / t_inf / 1e12, 3)},
         spinup_seconds=cmp_["seconds"],
     )
-    log(f"spin-up: cold {c['years']} y, warm {w['years']} y; first-year change cold {c['annual_change'][0]:.3e} "
-        f"warm {w['annual_change'][0]:.3e}; initial storage error cold {c['init_rel_err']:.3e} "
-        f"warm {w['init_rel_err']:.3e}")
+    log(f"spin-up: cold {c['years']} y, mean-flow steady {sd['years']} y, physics prior {pr['years']} y, "
+        f"warm {w['years']} y; first-year change cold {c['annual_change'][0]:.3e} steady "
+        f"{sd['annual_change'][0]:.3e} prior {pr['annual_change'][0]:.3e} warm {w['annual_change'][0]:.3e}")
     return out
 
 
@@ -456,14 +463,9 @@ def main():
                     help='launch configuration of the single-model runs: "auto" (autotuner) or a JSON object '
                          'of cmf_tune fields')
     ap.add_argument("--no-spinup", action="store_true")
-    ap.add_argument("--spinup-full", action="store_true",
-                    help="train on basins of four 1440x720 networks and apply to a held-out 1440x720 "
-                         "network (over 20 minutes: ~13 min of labels, ~60 s per simulated year)")
     ap.add_argument("--spinup-only", action="store_true")
-    ap.add_argument("--spin-epochs", type=int, default=60)
+    ap.add_argument("--spin-epochs", type=int, default=150)
     ap.add_argument("--spin-years", type=int, default=20)
-    ap.add_argument("--spin-w-over", type=float, default=16.0,
-                    help="loss weight of over-predicted storage when training the warm start")
     ap.add_argument("--precision-report", action="store_true",
                     help="only report mixed-vs-FP64 errors of the C1/C2/C3 configurations (JSON)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -76,21 +76,71 @@ def steady_state(m: CMFMap, q: np.ndarray) -> dict:
     """Steady state of the local-inertial outflow carrying the constant outflow
     q [m3/s] out of every cell (include/cmf_host.h cmfh_steady_state): rivsto,
     fldsto, sfcelv, rivout, fldout."""
-    from ._abi import cmfh_channel
     lib = _native.host()
-    f64 = lambda a: np.ascontiguousarray(a, np.float64)
-    keep = {k: f64(getattr(m, k)) for k in ("elevtn", "rivelv", "rivhgt", "rivwth", "rivlen", "rivman",
-                                              "nxtdst", "grarea", "dwnelv", "rivstomax", "fldstomax", "fldgrd")}
-    nxt = np.ascontiguousarray(np.where(m.next >= 0, m.next, -1), np.int32)
-    ch = cmfh_channel(n=m.n, nlfp=m.nlfp, lfldout=int(m.opts.lfldout), next=ptr(nxt),
-                      dfrcinc=m.dfrcinc, pmanfld=m.opts.pmanfld, pdstmth=m.opts.pdstmth,
-                      **{k: ptr(v) for k, v in keep.items()})
-    qq = f64(q)
+    ch, keep = _channel(m)
+    qq = np.ascontiguousarray(q, np.float64)
     out = {k: np.empty(m.n) for k in ("rivsto", "fldsto", "sfcelv", "rivout", "fldout")}
     rc = lib.cmfh_steady_state(C.byref(ch), ptr(qq), *(ptr(out[k]) for k in
                                                         ("rivsto", "fldsto", "sfcelv", "rivout", "fldout")))
     if rc != 0:
         raise RuntimeError(f"steady state failed ({rc})")
+    return out
+
+
+def travel_time(m: CMFMap, q: np.ndarray, tau: np.ndarray) -> np.ndarray:
+    """Runoff-weighted mean travel time from the runoff sources upstream of each
+    cell to the cell: T_i = sum_u r_u tau(u -> i) / sum_u r_u, where tau(u -> i)
+    sums the residence times tau of the cells from u down to i (u included, i
+    excluded) and r = the local runoff, q = its upstream accumulation. With
+    B_i = T_i q_i, B_i = sum over upstream neighbours c of (B_c + q_c tau_c), so
+    B = accumulate(q tau) - q tau."""
+    x = q * tau
+    b = accumulate(m, x) - x
+    return b / np.maximum(q, 1e-30)
+
+
+def _channel(m: CMFMap):
+    """cmfh_channel of a map plus the arrays it points to (kept alive by the caller)."""
+    from ._abi import cmfh_channel
+    f64 = lambda a: np.ascontiguousarray(a, np.float64)
+    keep = {k: f64(getattr(m, k)) for k in ("elevtn", "rivelv", "rivhgt", "rivwth", "rivlen", "rivman",
+                                              "nxtdst", "grarea", "dwnelv", "rivstomax", "fldstomax", "fldgrd")}
+    keep["next"] = np.ascontiguousarray(np.where(m.next >= 0, m.next, -1), np.int32)
+    ch = cmfh_channel(n=m.n, nlfp=m.nlfp, lfldout=int(m.opts.lfldout), dfrcinc=m.dfrcinc,
+                      pmanfld=m.opts.pmanfld, pdstmth=m.opts.pdstmth, **{k: ptr(v) for k, v in keep.items()})
+    return ch, keep
+
+
+def runoff_harmonics(clim: Climatology, nh: int) -> np.ndarray:
+    """Complex amplitudes r_k (k = 1..nh) of the local daily runoff over the
+    climatological year, r(t) = r_0 + 2 Re sum_k r_k exp(i w_k t), w_k = 2 pi k / 365 d."""
+    d = np.arange(DAYS)
+    R = np.stack([clim.daily(k) for k in d])                               # [365, n]
+    w = 2.0 * np.pi * np.arange(1, nh + 1) / DAYS
+    return (np.exp(-1j * np.outer(w, d + 0.5)) / DAYS) @ R
+
+
+def periodic_state(m: CMFMap, clim: Climatology, sfc_mean: np.ndarray, t_day: float = 0.0, nh: int = 6) -> dict:
+    """State at day t_day of the climatological year from the network's linear
+    periodic response around the steady surface sfc_mean (include/cmf_host.h
+    cmfh_periodic_response): surface elevation and river / floodplain storage."""
+    lib = _native.host()
+    ch, keep = _channel(m)
+    rh = runoff_harmonics(clim, nh)
+    omega = np.ascontiguousarray(2.0 * np.pi * np.arange(1, nh + 1) / (DAYS * 86400.0))
+    rr, ri = np.ascontiguousarray(rh.real), np.ascontiguousarray(rh.imag)
+    hr, hi = np.empty_like(rr), np.empty_like(ri)
+    sm = np.ascontiguousarray(sfc_mean, np.float64)
+    rc = lib.cmfh_periodic_response(C.byref(ch), ptr(sm), nh, ptr(omega), ptr(rr), ptr(ri), ptr(hr), ptr(hi))
+    if rc != 0:
+        raise RuntimeError(f"periodic response failed ({rc})")
+    w = 2.0 * np.pi * np.arange(1, nh + 1) / DAYS
+    eta = sm + 2.0 * np.sum(((hr + 1j * hi) * np.exp(1j * w * t_day)[:, None]).real, axis=0)
+    eta = np.ascontiguousarray(np.maximum(eta, m.rivelv))
+    out = {"sfcelv": eta, "rivsto": np.empty(m.n), "fldsto": np.empty(m.n)}
+    rc = lib.cmfh_storage_at_level(C.byref(ch), ptr(eta), ptr(out["rivsto"]), ptr(out["fldsto"]))
+    if rc != 0:
+        raise RuntimeError(f"storage at level failed ({rc})")
     return out
 
 
@@ -106,7 +156,7 @@ def domain_storage(st: dict, offs: Sequence[int]) -> np.ndarray:
 class SpinupResult:
     years: np.ndarray            # per domain: first year whose annual change is below tol (0 = not reached)
     storage: np.ndarray          # [years + 1, ndom] storage at every year end (row 0: start)
-    state: dict                  # final rivsto / fldsto
+    state: dict                  # final rivsto, fldsto, rivdph, rivout, fldout, outflw_avg (last outer step)
     seconds: float
     daily: Optional[np.ndarray] = None   # single-domain models: total storage at the end of every day
     substeps: int = 0
@@ -144,7 +194,7 @@ def spinup(model, clims: List[Climatology], offs: Sequence[int], tol: float = 1e
                     nsub += nt
             if not model.ensemble:
                 daily.append(glb[-1, 7] + glb[-1, 8])
-        st = model.get_state(["rivsto", "fldsto", "rivdph"])
+        st = model.get_state(["rivsto", "fldsto", "rivdph", "rivout", "fldout", "outflw_avg"])
         s = domain_storage(st, offs)
         ch = np.abs(s - hist[-1]) / np.maximum(s, 1e-30)
         hist.append(s)

@@ -1,24 +1,36 @@
-"""Learned spin-up warm start: a per-cell network that predicts the equilibrium
-river and floodplain storage of the periodic climatological regime from
-catchment attributes and the runoff climatology, trained on equilibria that the
-FP64 device physics itself produces (:func:`cmf.spinup.spinup`).
+"""Learned spin-up warm start for the periodic climatological regime, trained on
+equilibria that the FP64 device physics itself produces (:func:`cmf.spinup.spinup`).
+
+Two layers. The physical prior is the network's linear periodic response
+(:func:`cmf.spinup.periodic_state`): the steady state of the local-inertial
+outflow under the annual-mean runoff (:func:`cmf.spinup.steady_state`), plus the
+response of the mass balance, linearised in outflow and storage around it, to the
+first six harmonics of the runoff climatology, evaluated at the year start. It
+places the lakes and other slow storage (the part a cold start takes years to
+fill) within ~0.1% of their equilibrium; what it misses is the nonlinear response
+of the faster cells. The network learns that residual per cell, log(equilibrium
+storage / prior storage), from catchment attributes, the runoff climatology and
+the prior itself, on the cells that remember less than 30 days of the forcing
+(drawn in proportion to the storage they hold), and the warm start is the prior
+times the predicted factor on those cells.
 
 The network (24 -> 256 -> 256 -> 256 -> 2, ReLU) is trained and evaluated by the
 project's HIP kernels (include/cmf_mlp.h: BF16 matrix-core products, FP32
 accumulation, FP32 master weights, Adam). There is no CPU fallback: the library
 must be built. The plain cold start (STORAGE_SEA_SURFACE) stays the fallback of
-the spin-up and the judge of the warm start's drift.
+the spin-up.
 
-Features (24 per cell, all standardised with training-set statistics):
+Features (24 per cell, standardised with training-set statistics; q = upstream-
+accumulated runoff, qm = its annual mean, S = steady storage under qm, tau = S/qm):
   0 log10 upstream area [km2]       1 log10 channel slope        2 log river width
-  3 log river depth (bank height)   4 log river length
-  5-10 log1p floodplain layer heights of layers 1, 2, 4, 6, 8, 10
-  11-12 the steady state of the outflow equations under the annual-mean
-     accumulated runoff (cmf.spinup.steady_state), in target units
-  13-14 the same under the mean of the last 7 days of the year
-  15 log mean upstream-accumulated runoff   16 CV of its monthly means   17 its seasonal amplitude
-  18 log of its value on the first day of the year over its mean (the equilibrium is
-     taken at the start of the climatological year)
+  3 log river depth (bank height)   4 log river length           5 log tau [days]
+  6 log(T + tau) [days], T = runoff-weighted travel time from the upstream sources
+     (cmf.spinup.travel_time): how far back in the forcing the cell remembers
+  7-10 log(mean q over the last w days of the year / qm), w = 1, 7, 30, 90
+  11 log(prior storage / S)   12 asinh(prior surface elevation - steady elevation) [m]
+  13 log(prior storage / (RIVLEN*RIVWTH))   14 floodplain share of the prior storage
+  15 log qm   16 CV of the monthly means of q   17 seasonal amplitude of q
+  18 floodplain share of S
   19 log Manning depth at the cell's spill point of the accumulated runoff of the
      last 30 days of the year ((Q n / (W sqrt(slope)))^(3/5): the depth that carries
      the flow out of the flat reach)   20 log1p length [km] of the flat reach
@@ -26,10 +38,9 @@ Features (24 per cell, all standardised with training-set statistics):
   21 asinh of the cell's elevation above its spill point [m]
   22 log1p(top floodplain capacity / RIVSTOMAX)   23 log1p flat area [km2] upstream
   The spill point is the first cell downstream that is not flat (the cell itself
-  when it is not flat). Features 19-21 and 23 describe filled depressions (lakes):
-  their water level is set at the spill point, and their floodplain storage is
-  the slow part of a spin-up.
-Targets: log(RIVSTO / (RIVLEN*RIVWTH) + 1e-3) and log1p(FLDSTO / (RIVLEN*RIVWTH)).
+  when it is not flat). Features 19-21 and 23 describe filled depressions (lakes).
+Targets: log(equilibrium RIVSTO + FLDSTO at the year start / prior storage) (used
+for the warm start) and log(RIVOUT + FLDOUT at the year start / qm) (auxiliary).
 """
 from __future__ import annotations
 
@@ -42,21 +53,44 @@ import numpy as np
 from . import _native
 from . import maps, synth
 from .maps import CMFMap
-from .spinup import Climatology, accumulate, days_to_equilibrium, make_climatology, spinup, steady_state
+from .spinup import (Climatology, accumulate, days_to_equilibrium, make_climatology, periodic_state, spinup,
+                     steady_state, travel_time)
 
 N_IN, HIDDEN, N_OUT = 24, 256, 2
-EPS_DEPTH = 1e-3
 FLAT_SLOPE = 2e-5
-LAYERS = (0, 1, 3, 5, 7, 9)      # floodplain layers whose heights are features 5-10
-RECENT_DAYS = 7
+WINDOWS = (1, 7, 30, 90)          # days before the year start, features 7-10
+NHARM = 6                         # runoff harmonics of the linear periodic response
+Q_FLOOR = 1e-3                    # flow ratios are clipped to >= this
+STO_FLOOR = 1.0                   # m3, storages are floored at this in the log targets
+FAST_DAYS = 30.0                  # the network corrects cells that remember less of the forcing
 
 
 class SurrogateError(RuntimeError):
     pass
 
 
-def features(m: CMFMap, clim: Climatology) -> np.ndarray:
+@dataclasses.dataclass
+class Prior:
+    """Annual-mean accumulated runoff qm, the steady state under it and the linear
+    periodic response at the year start (the physical prior)."""
+    qm: np.ndarray
+    steady: dict
+    periodic: dict
+
+    @property
+    def storage(self) -> np.ndarray:
+        return np.maximum(self.periodic["rivsto"] + self.periodic["fldsto"], STO_FLOOR)
+
+
+def prior(m: CMFMap, clim: Climatology) -> Prior:
+    qm = accumulate(m, clim.monthly).mean(axis=0)
+    st = steady_state(m, qm)
+    return Prior(qm, st, periodic_state(m, clim, st["sfcelv"], 0.0, NHARM))
+
+
+def features(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None) -> np.ndarray:
     n = m.n
+    pr = pr or prior(m, clim)
     up = accumulate(m, m.grarea) * 1e-6
     nxt = m.next
     riv = nxt >= 0
@@ -64,33 +98,36 @@ def features(m: CMFMap, clim: Climatology) -> np.ndarray:
     slope = np.where(riv, (m.elevtn - m.elevtn[j]) / m.nxtdst, m.elevtn / m.opts.pdstmth)
     slope = np.maximum(slope, 1e-5)
     acc = accumulate(m, clim.monthly)                       # [12, n]
-    mean = acc.mean(axis=0)
-    cv = acc.std(axis=0) / np.maximum(mean, 1e-12)
-    amp = (acc.max(axis=0) - acc.min(axis=0)) / np.maximum(2.0 * mean, 1e-12)
-    fl = m.fldhgt                                           # [nlfp, n]
+    qm = np.maximum(pr.qm, 1e-12)
+    sto = np.maximum(pr.steady["rivsto"] + pr.steady["fldsto"], STO_FLOOR)
+    tau = sto / qm / 86400.0                                # days
+    T = travel_time(m, qm, tau)
+    daily = np.stack([clim.daily(d) for d in range(365 - max(WINDOWS), 365)])   # local runoff
+    ps = pr.storage
     X = np.empty((n, N_IN), np.float64)
     X[:, 0] = np.log10(up)
     X[:, 1] = np.log10(slope)
     X[:, 2] = np.log(m.rivwth)
     X[:, 3] = np.log(m.rivhgt)
     X[:, 4] = np.log(m.rivlen)
-    for k, l in enumerate(LAYERS):
-        X[:, 5 + k] = np.log1p(fl[min(l, fl.shape[0] - 1)])
-    # physical priors: steady states under the annual-mean and the recent flow
-    for k, q in enumerate((mean, accumulate(m, np.mean([clim.daily(d) for d in range(365 - RECENT_DAYS, 365)],
-                                                            axis=0)))):
-        ss = steady_state(m, q)
-        X[:, 11 + 2 * k:13 + 2 * k] = targets(m, ss["rivsto"], ss["fldsto"])
-    X[:, 15] = np.log(np.maximum(mean, 1e-12))
-    X[:, 16] = cv
-    X[:, 17] = amp
-    day0 = accumulate(m, clim.daily(0))
-    X[:, 18] = np.log(np.maximum(day0, 1e-12) / np.maximum(mean, 1e-12))
-    q30 = accumulate(m, np.mean([clim.daily(d) for d in range(335, 365)], axis=0))
+    X[:, 5] = np.log(np.maximum(tau, 1e-6))
+    X[:, 6] = np.log(np.maximum(T + tau, 1e-6))
+    for k, w in enumerate(WINDOWS):
+        qw = accumulate(m, daily[-w:].mean(axis=0))
+        X[:, 7 + k] = np.log(np.maximum(qw, 1e-12) / qm)
+    X[:, 11] = np.log(ps / sto)
+    X[:, 12] = np.arcsinh(pr.periodic["sfcelv"] - pr.steady["sfcelv"])
+    X[:, 13] = np.log(ps / (m.rivlen * m.rivwth))
+    X[:, 14] = pr.periodic["fldsto"] / ps
+    X[:, 15] = np.log(qm)
+    X[:, 16] = acc.std(axis=0) / qm
+    X[:, 17] = (acc.max(axis=0) - acc.min(axis=0)) / (2.0 * qm)
+    X[:, 18] = pr.steady["fldsto"] / sto
+    q30 = accumulate(m, daily[-30:].mean(axis=0))
     flat = slope < FLAT_SLOPE
     reach, spill = flat_reach_km(m, flat)
     hman = (q30[spill] * m.rivman[spill] / (m.rivwth[spill] * np.sqrt(slope[spill]))) ** 0.6
-    X[:, 19] = np.log(hman + EPS_DEPTH)
+    X[:, 19] = np.log(hman + 1e-3)
     X[:, 20] = np.log1p(reach)
     X[:, 21] = np.arcsinh(m.elevtn - m.elevtn[spill])
     X[:, 22] = np.log1p(m.fldstomax[-1] / m.rivstomax)
@@ -119,16 +156,36 @@ def flat_reach_km(m: CMFMap, flat: np.ndarray):
     return L, spill
 
 
-def targets(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray) -> np.ndarray:
-    a = m.rivlen * m.rivwth
-    return np.stack([np.log(rivsto / a + EPS_DEPTH), np.log1p(np.maximum(fldsto, 0.0) / a)], axis=1).astype(np.float32)
+def targets(pr: Prior, sto_eq: np.ndarray, q_start: np.ndarray) -> np.ndarray:
+    """Network targets from the equilibrium's storage and outflow at the year start."""
+    qm = np.maximum(pr.qm, 1e-12)
+    return np.stack([np.log(np.maximum(sto_eq, STO_FLOOR) / pr.storage),
+                     np.log(np.maximum(q_start / qm, Q_FLOOR))], axis=1).astype(np.float32)
 
 
-def storages(m: CMFMap, y: np.ndarray):
-    a = m.rivlen * m.rivwth
-    riv = np.maximum(np.exp(y[:, 0].astype(np.float64)) - EPS_DEPTH, 0.0) * a
-    fld = np.maximum(np.expm1(y[:, 1].astype(np.float64)), 0.0) * a
-    return riv, fld
+def fast_cells(X: np.ndarray) -> np.ndarray:
+    """Cells whose memory of the forcing (feature 6, travel plus residence time) is
+    shorter than FAST_DAYS: where the network's correction applies. Slower cells
+    (lakes, long flat reaches) keep the physical prior, which the linear response
+    already places within ~0.1% of their equilibrium."""
+    return X[:, 6] < np.log(FAST_DAYS)
+
+
+def storage(pr: Prior, y: np.ndarray, X: np.ndarray) -> np.ndarray:
+    """Total storage per cell [m3]: the prior times the network's correction
+    (first output) on the fast cells."""
+    corr = np.where(fast_cells(X), y[:, 0].astype(np.float64), 0.0)
+    return pr.storage * np.exp(corr)
+
+
+def training_set(X: np.ndarray, Y: np.ndarray, sto_eq: np.ndarray):
+    """(features, targets, sampling weights) of the cells the network corrects.
+    The weights (equilibrium storage over its mean, clipped to [0.05, 50]) draw
+    cells in proportion to the storage they hold: what a spin-up's annual change
+    measures is the total."""
+    sel = fast_cells(X)
+    s = np.maximum(sto_eq[sel], STO_FLOOR)
+    return X[sel], Y[sel], np.clip(s / s.mean(), 0.05, 50.0)
 
 
 def init_params(seed: int) -> np.ndarray:
@@ -197,14 +254,21 @@ class MLP:
         return float(loss.value)
 
     def fit(self, x: np.ndarray, t: np.ndarray, epochs: int, batch: int = 1024, lr: float = 1e-3,
-            seed: int = 0, log=None) -> list:
+            seed: int = 0, log=None, weights: Optional[np.ndarray] = None) -> list:
+        """Adam over the resident data set; every epoch visits N samples, a
+        permutation, or with `weights` a draw with replacement in proportion to
+        them (a weighted loss in expectation)."""
         a = np.ascontiguousarray(x, np.float32)
         b = np.ascontiguousarray(t, np.float32)
         self._check(self.lib.cmf_mlp_set_data(self.h, a.ctypes.data, b.ctypes.data, a.shape[0]))
         rng = np.random.default_rng(seed)
+        p = None if weights is None else np.asarray(weights, np.float64) / np.sum(weights)
         hist = []
         for ep in range(epochs):
-            order = rng.permutation(a.shape[0]).astype(np.int64)
+            if p is None:
+                order = rng.permutation(a.shape[0]).astype(np.int64)
+            else:
+                order = rng.choice(a.shape[0], a.shape[0], p=p).astype(np.int64)
             loss = C.c_float(0.0)
             rate = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs, 1)))     # cosine decay
             nb = self._check(self.lib.cmf_mlp_train(self.h, order.ctypes.data, order.size, batch, float(rate),
@@ -235,17 +299,14 @@ class Surrogate:
     x_sd: np.ndarray
     y_mu: np.ndarray
     y_sd: np.ndarray
-    base: int = -1        # >= 0: the network predicts the residual over features base, base+1
 
     def save(self, path: str) -> None:
-        np.savez(path, params=self.params, x_mu=self.x_mu, x_sd=self.x_sd, y_mu=self.y_mu, y_sd=self.y_sd,
-                 base=np.int64(self.base))
+        np.savez(path, params=self.params, x_mu=self.x_mu, x_sd=self.x_sd, y_mu=self.y_mu, y_sd=self.y_sd)
 
     @staticmethod
     def load(path: str) -> "Surrogate":
         z = np.load(path, allow_pickle=False)
-        return Surrogate(z["params"], z["x_mu"], z["x_sd"], z["y_mu"], z["y_sd"],
-                         int(z["base"]) if "base" in z else -1)
+        return Surrogate(z["params"], z["x_mu"], z["x_sd"], z["y_mu"], z["y_sd"])
 
     def predict(self, X: np.ndarray, net: Optional[MLP] = None) -> np.ndarray:
         own = net is None
@@ -254,31 +315,28 @@ class Surrogate:
         y = net.forward((X - self.x_mu) / self.x_sd)
         if own:
             net.close()
-        y = y * self.y_sd + self.y_mu
-        if self.base >= 0:
-            y = y + X[:, self.base:self.base + N_OUT]
-        return y
+        return y * self.y_sd + self.y_mu
 
-    def warm_start(self, m: CMFMap, clim: Climatology, net: Optional[MLP] = None):
-        """Predicted equilibrium (rivsto, fldsto) of every cell."""
-        return storages(m, self.predict(features(m, clim), net))
+    def warm_start(self, m: CMFMap, clim: Climatology, pr: Optional[Prior] = None,
+                   net: Optional[MLP] = None) -> np.ndarray:
+        """Predicted equilibrium storage (RIVSTO + FLDSTO) of every cell at the year start."""
+        pr = pr or prior(m, clim)
+        X = features(m, clim, pr)
+        return storage(pr, self.predict(X, net), X)
 
 
 def train(X: np.ndarray, Y: np.ndarray, epochs: int = 40, batch: int = 1024, lr: float = 1e-3, seed: int = 0,
-          log=None, w_over: float = 1.0, base: int = -1):
+          log=None, w_over: float = 1.0, weights: Optional[np.ndarray] = None):
     """Standardise, train on the device, return (Surrogate, loss history).
-    w_over > 1 weights over-predictions more: a warm start that errs low fills up
-    at the inflow rate, one that errs high in a filled depression drains slowly.
-    base >= 0: learn the residual over the prior in features base, base+1."""
-    if base >= 0:
-        Y = Y - X[:, base:base + N_OUT]
+    w_over > 1 weights over-predictions more than under-predictions; `weights`
+    (per sample) sets how often each sample is drawn."""
     x_mu, x_sd = X.mean(axis=0), X.std(axis=0) + 1e-6
     y_mu, y_sd = Y.mean(axis=0), Y.std(axis=0) + 1e-6
     net = MLP(max_batch=max(batch, 1024), seed=seed)
     net.set_loss(w_over)
-    hist = net.fit((X - x_mu) / x_sd, (Y - y_mu) / y_sd, epochs, batch, lr, seed, log)
+    hist = net.fit((X - x_mu) / x_sd, (Y - y_mu) / y_sd, epochs, batch, lr, seed, log, weights)
     s = Surrogate(net.params(), x_mu.astype(np.float32), x_sd.astype(np.float32), y_mu.astype(np.float32),
-                  y_sd.astype(np.float32), base)
+                  y_sd.astype(np.float32))
     net.close()
     return s, hist
 
@@ -373,12 +431,13 @@ def warm_state(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray, precision: str
     return st
 
 
-def prior_state(m: CMFMap, clim: Climatology, X: Optional[np.ndarray] = None) -> dict:
-    """Warm start from the physical prior alone: the steady state under the
-    annual-mean accumulated runoff (features 11-12)."""
-    X = features(m, clim) if X is None else X
-    riv, fld = storages(m, X[:, 11:13])
-    return warm_state(m, riv, fld)
+def prior_state(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None, which: str = "periodic") -> dict:
+    """Warm start from the physics alone: the linear periodic response at the year
+    start ("periodic", the prior the network refines) or the steady state under
+    the annual-mean runoff ("steady")."""
+    pr = pr or prior(m, clim)
+    st = pr.periodic if which == "periodic" else pr.steady
+    return warm_state(m, st["rivsto"], st["fldsto"])
 
 
 def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None, start: str = "prior"):
@@ -390,10 +449,11 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None,
     from .model import CMFModel
     cm, dom_cells = maps.concat_maps([b.m for b in basins])
     offs = np.cumsum([0] + dom_cells)
-    X = np.concatenate([features(b.m, b.clim) for b in basins])
+    priors = [prior(b.m, b.clim) for b in basins]
+    X = np.concatenate([features(b.m, b.clim, pr) for b, pr in zip(basins, priors)])
     init = {}
     if start == "prior":
-        sts = [prior_state(b.m, b.clim, X[offs[d]:offs[d + 1]]) for d, b in enumerate(basins)]
+        sts = [prior_state(b.m, b.clim, pr) for b, pr in zip(basins, priors)]
     elif start == "cold":
         sts = [cold_state(b.m) for b in basins]
     else:
@@ -404,27 +464,29 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None,
     model.set_state(init)
     res = spinup(model, [b.clim for b in basins], offs, tol=1e-3, stop_tol=tol, max_years=max_years, log=log)
     model.close()
-    Y = np.concatenate([targets(b.m, res.state["rivsto"][offs[d]:offs[d + 1]],
-                                res.state["fldsto"][offs[d]:offs[d + 1]]) for d, b in enumerate(basins)])
+    st = res.state
+    s0 = st["rivsto"] + st["fldsto"]
+    q0 = st["rivout"] + st["fldout"]
+    Y = np.concatenate([targets(pr, s0[offs[d]:offs[d + 1]], q0[offs[d]:offs[d + 1]])
+                        for d, pr in enumerate(priors)])
     return X, Y, res
 
 
 def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1e-3, log=None) -> dict:
-    """FP64 spin-ups of one network from the cold start, from the steady-state
-    prior alone and from the surrogate's warm start, run together as members of
-    one handle (same map and forcing, own adaptive steps; a fourth member repeats
-    the warm start to make the member count a power of two), until every annual
+    """FP64 spin-ups of one network from the cold start, from the steady state
+    under the annual-mean runoff, from the physical prior (linear periodic
+    response) and from the surrogate's warm start, run together as four members of
+    one handle (same map and forcing, own adaptive steps), until every annual
     storage change is below tol or `years` pass. The equilibrium all are judged
     against is the last year of the member whose last annual change is smallest."""
     import time
     from .model import CMFModel
     from .spinup import DAYS
     m, clim = basin.m, basin.clim
-    X = features(m, clim)
-    riv, fld = storages(m, sur.predict(X))
-    names = ("cold", "prior", "warm")
-    starts = [cold_state(m), prior_state(m, clim, X), warm_state(m, riv, fld)]
-    starts.append(starts[-1])
+    pr = prior(m, clim)
+    names = ("cold", "steady", "prior", "warm")
+    starts = [cold_state(m), prior_state(m, clim, pr, "steady"), prior_state(m, clim, pr),
+              warm_state(m, sur.warm_start(m, clim, pr), np.zeros(m.n))]
     M = len(starts)
     model = CMFModel(m, "fp64", members=M, dom_cells=[m.n])
     model.set_state({k: np.stack([st[k] for st in starts]) for k in ("rivsto", "fldsto", "rivdph_pre")})

@@ -514,3 +514,150 @@ int cmfh_steady_state(const cmfh_channel* c, const double* q, double* rivsto, do
     free(indeg); free(order);
     return 0;
 }
+
+int cmfh_accumulate_gain(int n, const int32_t* next, int nrec, const double* v_re, const double* v_im,
+                         const double* g_re, const double* g_im, double* o_re, double* o_im)
+{
+    int32_t* indeg = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    if (!indeg || !order) { free(indeg); free(order); return -2; }
+    for (int i = 0; i < n; ++i) {
+        if (next[i] >= n) { free(indeg); free(order); return -1; }
+        if (next[i] >= 0) indeg[next[i]] += 1;
+    }
+    int qh = 0, qt = 0;
+    for (int i = 0; i < n; ++i) if (indeg[i] == 0) order[qt++] = i;
+    while (qh < qt) {
+        const int i = order[qh++];
+        if (next[i] >= 0 && --indeg[next[i]] == 0) order[qt++] = next[i];
+    }
+    if (qt != n) { free(indeg); free(order); return -3; }
+    for (int r = 0; r < nrec; ++r) {
+        const size_t b = (size_t)r * n;
+        for (int i = 0; i < n; ++i) { o_re[b + i] = v_re[b + i]; o_im[b + i] = v_im[b + i]; }
+        for (int k = 0; k < n; ++k) {
+            const int i = order[k];
+            const int j = next[i];
+            if (j < 0) continue;
+            const double a = o_re[b + i], c = o_im[b + i], gr = g_re[b + i], gi = g_im[b + i];
+            o_re[b + j] += gr * a - gi * c;
+            o_im[b + j] += gr * c + gi * a;
+        }
+    }
+    free(indeg); free(order);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Linear periodic response of the network around a steady state (no reference
+ * counterpart; the physical prior of the spin-up surrogate).
+ *
+ * Around the steady surface elevations s (from cmfh_steady_state), the outflow of
+ * cell i varies as dO_i = a_i ds_i - b_i ds_d(i) (d = downstream cell; a mouth's
+ * downstream level is fixed) and its storage as dS_i = A_i ds_i, the three
+ * derivatives taken numerically from the steady outflow and storage functions.
+ * For one harmonic exp(i w t) of the local runoff r, mass balance
+ *   i w A_i h_i = r_i + sum_u (a_u h_u - b_u h_i) - (a_i h_i - b_i h_d(i))
+ * is a complex linear system on the river tree, solved exactly by elimination
+ * from the sources down (h_u = alpha_u + beta_u h_i for every upstream u of i)
+ * and substitution back up. */
+static void response_coefs(const cmfh_channel* c, int i, double s, double s_down, double* a, double* b, double* area)
+{
+    const double d = s - c->rivelv[i];
+    const double e = 1e-3;
+    double qf, fs, fd;
+    const double lo = s - e > s_down ? s - e : s;
+    *a = (steady_flow(c, i, s + e, s_down, &qf) - steady_flow(c, i, lo, s_down, &qf)) / (s + e - lo);
+    if (c->next[i] >= 0) {
+        const double hi = s_down + e < s ? s_down + e : s_down;
+        *b = (steady_flow(c, i, s, s_down - e, &qf) - steady_flow(c, i, s, hi, &qf)) / (hi - (s_down - e));
+    } else {
+        *b = 0.0;
+    }
+    const double dl = d - e > 0.0 ? d - e : 0.0;
+    *area = (steady_storage(c, i, d + e, &fs, &fd) - steady_storage(c, i, dl, &fs, &fd)) / (d + e - dl);
+}
+
+int cmfh_periodic_response(const cmfh_channel* c, const double* sfcelv, int nh, const double* omega,
+                           const double* r_re, const double* r_im, double* h_re, double* h_im)
+{
+    if (!c || !sfcelv || nh < 0 || (nh > 0 && (!omega || !r_re || !r_im || !h_re || !h_im)) || c->n <= 0)
+        return -1;
+    const int n = c->n;
+    int32_t* indeg = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    double* co = (double*)malloc(sizeof(double) * 3 * (size_t)n);          /* a, b, A */
+    double* acc = (double*)malloc(sizeof(double) * 5 * (size_t)n);         /* sum b, sum a*beta, sum a*alpha */
+    double* ab = (double*)malloc(sizeof(double) * 4 * (size_t)n);          /* alpha, beta (complex) */
+    if (!indeg || !order || !co || !acc || !ab) {
+        free(indeg); free(order); free(co); free(acc); free(ab); return -2;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (c->next[i] >= n) { free(indeg); free(order); free(co); free(acc); free(ab); return -1; }
+        if (c->next[i] >= 0) indeg[c->next[i]] += 1;
+    }
+    int qh = 0, qt = 0;
+    for (int i = 0; i < n; ++i) if (indeg[i] == 0) order[qt++] = i;
+    while (qh < qt) {
+        const int i = order[qh++];
+        if (c->next[i] >= 0 && --indeg[c->next[i]] == 0) order[qt++] = c->next[i];
+    }
+    if (qt != n) { free(indeg); free(order); free(co); free(acc); free(ab); return -3; }
+    for (int i = 0; i < n; ++i) {
+        const double sd = c->next[i] >= 0 ? sfcelv[c->next[i]] : c->dwnelv[i];
+        response_coefs(c, i, sfcelv[i], sd, &co[3 * i], &co[3 * i + 1], &co[3 * i + 2]);
+    }
+    for (int k = 0; k < nh; ++k) {
+        const double w = omega[k];
+        const double* rr = r_re + (size_t)k * n;
+        const double* ri = r_im + (size_t)k * n;
+        double* hr = h_re + (size_t)k * n;
+        double* hi = h_im + (size_t)k * n;
+        for (size_t t = 0; t < 5 * (size_t)n; ++t) acc[t] = 0.0;
+        for (int t = 0; t < n; ++t) {
+            const int i = order[t];
+            const double a = co[3 * i], b = co[3 * i + 1], A = co[3 * i + 2];
+            double* s = acc + 5 * (size_t)i;      /* sum b_u, sum a_u beta_u (re, im), sum a_u alpha_u (re, im) */
+            const double dr = a + s[0] - s[1], di = w * A - s[2];
+            const double nr = rr[i] + s[3], ni = ri[i] + s[4];
+            const double den = dr * dr + di * di;
+            const double al_r = (nr * dr + ni * di) / den, al_i = (ni * dr - nr * di) / den;
+            const double be_r = b * dr / den, be_i = -b * di / den;
+            ab[4 * i] = al_r; ab[4 * i + 1] = al_i; ab[4 * i + 2] = be_r; ab[4 * i + 3] = be_i;
+            const int j = c->next[i];
+            if (j >= 0) {
+                double* sj = acc + 5 * (size_t)j;
+                sj[0] += b;
+                sj[1] += a * be_r; sj[2] += a * be_i;
+                sj[3] += a * al_r; sj[4] += a * al_i;
+            }
+        }
+        for (int t = n - 1; t >= 0; --t) {
+            const int i = order[t];
+            const int j = c->next[i];
+            double vr = ab[4 * i], vi = ab[4 * i + 1];
+            if (j >= 0) {
+                const double br = ab[4 * i + 2], bi = ab[4 * i + 3];
+                vr += br * hr[j] - bi * hi[j];
+                vi += br * hi[j] + bi * hr[j];
+            }
+            hr[i] = vr; hi[i] = vi;
+        }
+    }
+    free(indeg); free(order); free(co); free(acc); free(ab);
+    return 0;
+}
+
+int cmfh_storage_at_level(const cmfh_channel* c, const double* sfcelv, double* rivsto, double* fldsto)
+{
+    if (!c || !sfcelv || !rivsto || !fldsto || c->n <= 0) return -1;
+    for (int i = 0; i < c->n; ++i) {
+        double d = sfcelv[i] - c->rivelv[i];
+        if (d < 0.0) d = 0.0;
+        double fs, fd;
+        const double sto = steady_storage(c, i, d, &fs, &fd);
+        fldsto[i] = fs;
+        rivsto[i] = sto - fs;
+    }
+    return 0;
+}

@@ -85,6 +85,15 @@ int cmfh_upstream_area(int nx, int ny, const int32_t* nextx, const int32_t* next
  * network has a cycle. */
 int cmfh_accumulate(int n, const int32_t* next, int nrec, const double* val, double* out);
 
+/* Complex accumulation with a transfer factor per cell: for each of nrec records
+ * (harmonics), out[i] = v[i] + sum over cells c draining to i of g[c] * out[c],
+ * complex numbers as separate real / imaginary arrays [nrec][n]. With g = 1 / (1 +
+ * i w tau) it propagates one harmonic of the runoff through a network of linear
+ * reservoirs (residence times tau). No reference counterpart (spin-up surrogate
+ * features). Returns 0, -1 on a bad index, -3 if the network has a cycle. */
+int cmfh_accumulate_gain(int n, const int32_t* next, int nrec, const double* v_re, const double* v_im,
+                         const double* g_re, const double* g_im, double* o_re, double* o_im);
+
 /* Channel description for cmfh_steady_state: per-cell arrays of n cells in any
  * order (next[i] < 0 at mouths), floodplain tables layer-planar. */
 typedef struct {
@@ -103,6 +112,21 @@ typedef struct {
  * the spin-up surrogate. Returns 0, -1 on bad arguments, -3 on a cycle. */
 int cmfh_steady_state(const cmfh_channel* c, const double* q, double* rivsto, double* fldsto, double* sfcelv,
                       double* rivout, double* fldout);
+
+/* Linear periodic response around the steady surface elevations sfcelv: for each
+ * of nh harmonics (angular frequency omega[k] in rad/s, complex local runoff
+ * amplitude r[k][i] in m3/s) the complex surface-elevation amplitude h[k][i] [m]
+ * of the mass balance linearised in the outflow (src/cmf_calc_outflw_mod.F90:
+ * 60-171 at steady state) and the storage (stage tables). No reference
+ * counterpart: the physical prior of the spin-up surrogate. Returns 0, -1 on bad
+ * arguments, -3 on a cycle. */
+int cmfh_periodic_response(const cmfh_channel* c, const double* sfcelv, int nh, const double* omega,
+                           const double* r_re, const double* r_im, double* h_re, double* h_im);
+
+/* River and floodplain storage of each cell at the surface elevation sfcelv
+ * (the inverse of the stage inversion of src/cmf_calc_fldstg_mod.F90:37-95 without
+ * levees). No reference counterpart (surrogate). Returns 0 or -1. */
+int cmfh_storage_at_level(const cmfh_channel* c, const double* sfcelv, double* rivsto, double* fldsto);
 
 #ifdef __cplusplus
 }

@@ -64,9 +64,39 @@ def test_surrogate_features_finite():
     net = synth.make_network(40, 20, seed=3)
     m = maps.build_map(net, maps.Options())
     clim = spinup.make_climatology(net, m, 3, 1.5, 0.5, 2.0)
-    X = surrogate.features(m, clim)
+    pr = surrogate.prior(m, clim)
+    X = surrogate.features(m, clim, pr)
     assert X.shape == (m.n, surrogate.N_IN) and np.isfinite(X).all()
-    y = surrogate.targets(m, m.rivstomax * 0.5, m.rivstomax * 0.1)
-    riv, fld = surrogate.storages(m, y)
-    assert np.allclose(riv, m.rivstomax * 0.5, rtol=1e-5) and np.allclose(fld, m.rivstomax * 0.1, rtol=1e-4)
+    # storage-ratio targets invert to storages; the flow-ratio floor applies
+    s_eq = 1.5 * pr.storage
+    y = surrogate.targets(pr, s_eq, 1e-9 * pr.qm)
+    fast = surrogate.fast_cells(X)
+    assert 0 < fast.sum() < m.n
+    got = surrogate.storage(pr, y, X)
+    assert np.allclose(got[fast], s_eq[fast], rtol=1e-6)
+    assert np.allclose(got[~fast], pr.storage[~fast], rtol=1e-12)     # slow cells keep the prior
+    Xs, Ys, ws = surrogate.training_set(X, y, s_eq)
+    assert Xs.shape[0] == fast.sum() == ws.size and ws.min() >= 0.05 and ws.max() <= 50.0
+    assert np.allclose(y[:, 1], np.log(surrogate.Q_FLOOR))
+    # window ratios of a climatology without seasonality are 1 (log 0), and its
+    # periodic prior is the steady state
+    flat = spinup.Climatology(np.repeat(clim.monthly.mean(axis=0, keepdims=True), 12, axis=0), 1.5, 0.0, 0.0)
+    pf = surrogate.prior(m, flat)
+    Xf = surrogate.features(m, flat, pf)
+    assert np.abs(Xf[:, 7:11]).max() < 1e-5 and np.abs(Xf[:, 16:18]).max() < 1e-5
+    assert np.abs(Xf[:, 11]).max() < 1e-4 and np.abs(Xf[:, 12]).max() < 1e-4
+
+
+def test_travel_time_closed_form():
+    # chain 0 -> 1 -> 2 (mouth) plus 3 -> 2; local runoff r, residence times tau
+    m = _Map([1, 2, -9, 2])
+    r = np.array([1.0, 2.0, 4.0, 8.0])
+    tau = np.array([10.0, 20.0, 30.0, 40.0])
+    q = spinup.accumulate(m, r)
+    T = spinup.travel_time(m, q, tau)
+    # cell 1: runoff of 0 travels tau0 = 10, its own 0 -> (1*10 + 2*0) / 3
+    assert np.isclose(T[1], 10.0 / 3.0)
+    # cell 2: 0 travels 10+20, 1 travels 20, 3 travels 40, own 0
+    assert np.isclose(T[2], (1 * 30 + 2 * 20 + 8 * 40) / 15.0)
+    assert T[0] == 0.0 and T[3] == 0.0
 

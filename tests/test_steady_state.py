@@ -6,6 +6,10 @@ the oracle's restatement of the reference physics (CPU):
 * fixed point: one hourly outer step of the reference physics from the steady
   state under the runoff that produces its flows changes no storage, no outflow;
 * Manning closed form on a single in-bank mouth cell.
+
+and the linear periodic response around it (cmfh_periodic_response): the closed
+form of a single linear reservoir, and at zero frequency the sensitivity of the
+steady state itself (a finite difference of two steady solves).
 """
 import numpy as np
 import pytest
@@ -70,3 +74,69 @@ def test_manning_closed_form_single_cell():
     assert abs(ss["sfcelv"][0] - s) < 1e-9
     assert abs(ss["rivsto"][0] - h * float(sub.rivlen[0] * sub.rivwth[0])) < 1e-9 * ss["rivsto"][0]
     assert ss["fldsto"][0] == 0.0 and ss["fldout"][0] == 0.0
+
+
+def _response(m, sfc, omega, r):
+    """cmfh_periodic_response for one harmonic (complex local runoff r)."""
+    import ctypes as C
+    from cmf import _native
+    from cmf._abi import ptr
+    lib = _native.host()
+    ch, keep = spinup._channel(m)
+    rr, ri = np.ascontiguousarray(r.real[None]), np.ascontiguousarray(r.imag[None])
+    hr, hi = np.empty_like(rr), np.empty_like(ri)
+    w = np.array([omega])
+    s = np.ascontiguousarray(sfc, np.float64)
+    assert lib.cmfh_periodic_response(C.byref(ch), ptr(s), 1, ptr(w), ptr(rr), ptr(ri), ptr(hr), ptr(hi)) == 0
+    return hr[0] + 1j * hi[0]
+
+
+def test_periodic_response_single_reservoir_closed_form():
+    """One in-bank mouth cell is a reservoir of area RIVLEN*RIVWTH with outflow
+    Q(s) = W h^(5/3) ((s - DWNELV) / PDSTMTH)^(1/2) / n: the response to runoff
+    amplitude r at frequency w is r / (i w A + dQ/ds)."""
+    m, _ = _basin(11, 1.5)
+    sub = maps.subset_map(m, np.array([int(np.flatnonzero(m.next < 0)[0])]))
+    sub.dwnelv = sub.rivelv - 1.0
+    W, L, n, A = float(sub.rivwth[0]), float(sub.opts.pdstmth), float(sub.rivman[0]), float(sub.rivlen[0] * sub.rivwth[0])
+    h = 0.3 * float(sub.rivhgt[0])
+    s = float(sub.rivelv[0]) + h
+    q = W * h ** (5 / 3) * np.sqrt((s - float(sub.dwnelv[0])) / L) / n
+    dq = q * (5.0 / (3.0 * h) + 0.5 / (s - float(sub.dwnelv[0])))
+    for w in (0.0, 2 * np.pi / (365 * 86400.0), 2 * np.pi / 86400.0):
+        r = np.array([0.3 * q * np.exp(0.7j)])
+        got = _response(sub, np.array([s]), w, r)[0]
+        want = r[0] / (1j * w * A + dq)
+        assert abs(got - want) < 1e-5 * abs(want), (w, got, want)
+
+
+def test_periodic_response_zero_frequency_is_the_steady_sensitivity():
+    """At w = 0 the response to a small runoff change dr is the change of the
+    steady state: steady(q + acc(dr)) - steady(q), for the whole coupled network
+    (downstream backwater included)."""
+    m, r = _basin(12, 40.0)
+    q = spinup.accumulate(m, r)
+    s0 = spinup.steady_state(m, q)
+    dr = 1e-3 * r
+    s1 = spinup.steady_state(m, q + spinup.accumulate(m, dr))
+    got = _response(m, s0["sfcelv"], 0.0, dr.astype(complex))
+    want = s1["sfcelv"] - s0["sfcelv"]
+    assert np.abs(got.imag).max() == 0.0
+    scale = np.abs(want).max()
+    assert scale > 0
+    assert np.max(np.abs(got.real - want)) < 2e-2 * scale
+    assert np.median(np.abs(got.real - want) / np.maximum(np.abs(want), 1e-3 * scale)) < 1e-2
+
+
+def test_storage_at_level_inverts_the_stage():
+    """cmfh_storage_at_level on the steady surface returns the steady storages."""
+    import ctypes as C
+    from cmf import _native
+    from cmf._abi import ptr
+    m, r = _basin(12, 40.0)
+    ss = spinup.steady_state(m, spinup.accumulate(m, r))
+    ch, keep = spinup._channel(m)
+    riv, fld = np.empty(m.n), np.empty(m.n)
+    assert _native.host().cmfh_storage_at_level(C.byref(ch), ptr(ss["sfcelv"]), ptr(riv), ptr(fld)) == 0
+    assert np.allclose(riv, ss["rivsto"], rtol=1e-12, atol=1e-6)
+    assert np.allclose(fld, ss["fldsto"], rtol=1e-12, atol=1e-6)
