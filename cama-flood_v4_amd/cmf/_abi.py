@@ -26,7 +26,7 @@ HOST_FUNCS = {
     "cmfh_accumulate": (I, [I, P, I, P, P]),
     "cmfh_steady_state": (I, [C.POINTER(cmfh_channel), P, P, P, P, P, P]),
     "cmfh_accumulate_gain": (I, [I, P, I, P, P, P, P, P, P]),
-    "cmfh_periodic_response": (I, [C.POINTER(cmfh_channel), P, I, P, P, P, P, P]),
+    "cmfh_periodic_response": (I, [C.POINTER(cmfh_channel), P, I, P, P, P, P, P, P, P]),
     "cmfh_storage_at_level": (I, [C.POINTER(cmfh_channel), P, P, P]),
 }
 

@@ -123,7 +123,8 @@ def runoff_harmonics(clim: Climatology, nh: int) -> np.ndarray:
 def periodic_state(m: CMFMap, clim: Climatology, sfc_mean: np.ndarray, t_day: float = 0.0, nh: int = 6) -> dict:
     """State at day t_day of the climatological year from the network's linear
     periodic response around the steady surface sfc_mean (include/cmf_host.h
-    cmfh_periodic_response): surface elevation and river / floodplain storage."""
+    cmfh_periodic_response): surface elevation, river / floodplain storage, and
+    the outflow's departure from the steady flow (dq)."""
     lib = _native.host()
     ch, keep = _channel(m)
     rh = runoff_harmonics(clim, nh)
@@ -131,13 +132,17 @@ def periodic_state(m: CMFMap, clim: Climatology, sfc_mean: np.ndarray, t_day: fl
     rr, ri = np.ascontiguousarray(rh.real), np.ascontiguousarray(rh.imag)
     hr, hi = np.empty_like(rr), np.empty_like(ri)
     sm = np.ascontiguousarray(sfc_mean, np.float64)
-    rc = lib.cmfh_periodic_response(C.byref(ch), ptr(sm), nh, ptr(omega), ptr(rr), ptr(ri), ptr(hr), ptr(hi))
+    orr, oi = np.empty_like(rr), np.empty_like(ri)
+    rc = lib.cmfh_periodic_response(C.byref(ch), ptr(sm), nh, ptr(omega), ptr(rr), ptr(ri), ptr(hr), ptr(hi),
+                                    ptr(orr), ptr(oi))
     if rc != 0:
         raise RuntimeError(f"periodic response failed ({rc})")
     w = 2.0 * np.pi * np.arange(1, nh + 1) / DAYS
-    eta = sm + 2.0 * np.sum(((hr + 1j * hi) * np.exp(1j * w * t_day)[:, None]).real, axis=0)
+    ph = np.exp(1j * w * t_day)[:, None]
+    eta = sm + 2.0 * np.sum(((hr + 1j * hi) * ph).real, axis=0)
     eta = np.ascontiguousarray(np.maximum(eta, m.rivelv))
-    out = {"sfcelv": eta, "rivsto": np.empty(m.n), "fldsto": np.empty(m.n)}
+    dq = 2.0 * np.sum(((orr + 1j * oi) * ph).real, axis=0)
+    out = {"sfcelv": eta, "rivsto": np.empty(m.n), "fldsto": np.empty(m.n), "dq": dq}
     rc = lib.cmfh_storage_at_level(C.byref(ch), ptr(eta), ptr(out["rivsto"]), ptr(out["fldsto"]))
     if rc != 0:
         raise RuntimeError(f"storage at level failed ({rc})")

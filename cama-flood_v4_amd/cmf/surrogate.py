@@ -1,18 +1,20 @@
 """Learned spin-up warm start for the periodic climatological regime, trained on
 equilibria that the FP64 device physics itself produces (:func:`cmf.spinup.spinup`).
 
-Two layers. The physical prior is the network's linear periodic response
+Two layers. The physical prior comes from the network's linear periodic response
 (:func:`cmf.spinup.periodic_state`): the steady state of the local-inertial
 outflow under the annual-mean runoff (:func:`cmf.spinup.steady_state`), plus the
 response of the mass balance, linearised in outflow and storage around it, to the
-first six harmonics of the runoff climatology, evaluated at the year start. It
-places the lakes and other slow storage (the part a cold start takes years to
-fill) within ~0.1% of their equilibrium; what it misses is the nonlinear response
-of the faster cells. The network learns that residual per cell, log(equilibrium
-storage / prior storage), from catchment attributes, the runoff climatology and
-the prior itself, on the cells that remember less than 30 days of the forcing
-(drawn in proportion to the storage they hold), and the warm start is the prior
-times the predicted factor on those cells.
+first six harmonics of the runoff climatology. Its outflows at the year start,
+mapped back to storage through the steady state that carries them (the
+quasi-steady map keeps the Manning nonlinearity), are the prior. It places the
+lakes and other slow storage (what a cold start takes years to fill) within
+~0.1% of their equilibrium; what it misses is the nonlinear seasonal response of
+the cells with weeks to months of memory. The network learns that residual per
+cell, log(equilibrium storage / prior storage), from catchment attributes, the
+runoff climatology and the prior itself, on the cells that remember less than a
+year of the forcing (drawn in proportion to the storage they hold), and the warm
+start is the prior times the predicted factor on those cells.
 
 The network (24 -> 256 -> 256 -> 256 -> 2, ReLU) is trained and evaluated by the
 project's HIP kernels (include/cmf_mlp.h: BF16 matrix-core products, FP32
@@ -27,8 +29,8 @@ accumulated runoff, qm = its annual mean, S = steady storage under qm, tau = S/q
   6 log(T + tau) [days], T = runoff-weighted travel time from the upstream sources
      (cmf.spinup.travel_time): how far back in the forcing the cell remembers
   7-10 log(mean q over the last w days of the year / qm), w = 1, 7, 30, 90
-  11 log(prior storage / S)   12 asinh(prior surface elevation - steady elevation) [m]
-  13 log(prior storage / (RIVLEN*RIVWTH))   14 floodplain share of the prior storage
+  11 log(prior storage / S)   12 asinh(linear-response elevation - steady elevation) [m]
+  13 log(prior storage / (RIVLEN*RIVWTH))   14 log(linear-response year-start outflow / qm)
   15 log qm   16 CV of the monthly means of q   17 seasonal amplitude of q
   18 floodplain share of S
   19 log Manning depth at the cell's spill point of the accumulated runoff of the
@@ -62,7 +64,7 @@ WINDOWS = (1, 7, 30, 90)          # days before the year start, features 7-10
 NHARM = 6                         # runoff harmonics of the linear periodic response
 Q_FLOOR = 1e-3                    # flow ratios are clipped to >= this
 STO_FLOOR = 1.0                   # m3, storages are floored at this in the log targets
-FAST_DAYS = 30.0                  # the network corrects cells that remember less of the forcing
+FAST_DAYS = 365.0                 # the network corrects cells that remember less of the forcing
 
 
 class SurrogateError(RuntimeError):
@@ -71,21 +73,27 @@ class SurrogateError(RuntimeError):
 
 @dataclasses.dataclass
 class Prior:
-    """Annual-mean accumulated runoff qm, the steady state under it and the linear
-    periodic response at the year start (the physical prior)."""
+    """Annual-mean accumulated runoff qm, the steady state under it, the linear
+    periodic response at the year start, and the physical prior: the steady
+    state of the response's year-start outflows (the quasi-steady map from flows
+    to storage keeps the Manning nonlinearity the linear levels miss)."""
     qm: np.ndarray
     steady: dict
     periodic: dict
+    q0: np.ndarray
+    state: dict
 
     @property
     def storage(self) -> np.ndarray:
-        return np.maximum(self.periodic["rivsto"] + self.periodic["fldsto"], STO_FLOOR)
+        return np.maximum(self.state["rivsto"] + self.state["fldsto"], STO_FLOOR)
 
 
 def prior(m: CMFMap, clim: Climatology) -> Prior:
     qm = accumulate(m, clim.monthly).mean(axis=0)
     st = steady_state(m, qm)
-    return Prior(qm, st, periodic_state(m, clim, st["sfcelv"], 0.0, NHARM))
+    ps = periodic_state(m, clim, st["sfcelv"], 0.0, NHARM)
+    q0 = np.maximum(qm + ps["dq"], Q_FLOOR * qm)
+    return Prior(qm, st, ps, q0, steady_state(m, q0))
 
 
 def features(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None) -> np.ndarray:
@@ -118,7 +126,7 @@ def features(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None) -> np.nda
     X[:, 11] = np.log(ps / sto)
     X[:, 12] = np.arcsinh(pr.periodic["sfcelv"] - pr.steady["sfcelv"])
     X[:, 13] = np.log(ps / (m.rivlen * m.rivwth))
-    X[:, 14] = pr.periodic["fldsto"] / ps
+    X[:, 14] = np.log(pr.q0 / qm)
     X[:, 15] = np.log(qm)
     X[:, 16] = acc.std(axis=0) / qm
     X[:, 17] = (acc.max(axis=0) - acc.min(axis=0)) / (2.0 * qm)
@@ -166,8 +174,8 @@ def targets(pr: Prior, sto_eq: np.ndarray, q_start: np.ndarray) -> np.ndarray:
 def fast_cells(X: np.ndarray) -> np.ndarray:
     """Cells whose memory of the forcing (feature 6, travel plus residence time) is
     shorter than FAST_DAYS: where the network's correction applies. Slower cells
-    (lakes, long flat reaches) keep the physical prior, which the linear response
-    already places within ~0.1% of their equilibrium."""
+    (lakes, long flat reaches) keep the physical prior, which already places them
+    within ~0.1% of their equilibrium."""
     return X[:, 6] < np.log(FAST_DAYS)
 
 
@@ -431,12 +439,11 @@ def warm_state(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray, precision: str
     return st
 
 
-def prior_state(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None, which: str = "periodic") -> dict:
-    """Warm start from the physics alone: the linear periodic response at the year
-    start ("periodic", the prior the network refines) or the steady state under
-    the annual-mean runoff ("steady")."""
+def prior_state(m: CMFMap, clim: Climatology, pr: Optional[Prior] = None, which: str = "prior") -> dict:
+    """Warm start from the physics alone: the physical prior the network refines
+    ("prior") or the steady state under the annual-mean runoff ("steady")."""
     pr = pr or prior(m, clim)
-    st = pr.periodic if which == "periodic" else pr.steady
+    st = pr.state if which == "prior" else pr.steady
     return warm_state(m, st["rivsto"], st["fldsto"])
 
 

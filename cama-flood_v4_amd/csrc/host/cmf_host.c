@@ -579,7 +579,8 @@ static void response_coefs(const cmfh_channel* c, int i, double s, double s_down
 }
 
 int cmfh_periodic_response(const cmfh_channel* c, const double* sfcelv, int nh, const double* omega,
-                           const double* r_re, const double* r_im, double* h_re, double* h_im)
+                           const double* r_re, const double* r_im, double* h_re, double* h_im,
+                           double* o_re, double* o_im)
 {
     if (!c || !sfcelv || nh < 0 || (nh > 0 && (!omega || !r_re || !r_im || !h_re || !h_im)) || c->n <= 0)
         return -1;
@@ -642,6 +643,17 @@ int cmfh_periodic_response(const cmfh_channel* c, const double* sfcelv, int nh, 
                 vi += br * hi[j] + bi * hr[j];
             }
             hr[i] = vr; hi[i] = vi;
+        }
+        if (o_re && o_im) {
+            /* outflow amplitude a_i h_i - b_i h_down (the downstream level of a mouth is fixed) */
+            double* orr = o_re + (size_t)k * n;
+            double* oi = o_im + (size_t)k * n;
+            for (int i = 0; i < n; ++i) {
+                const int j = c->next[i];
+                const double a = co[3 * i], b = co[3 * i + 1];
+                orr[i] = a * hr[i] - (j >= 0 ? b * hr[j] : 0.0);
+                oi[i] = a * hi[i] - (j >= 0 ? b * hi[j] : 0.0);
+            }
         }
     }
     free(indeg); free(order); free(co); free(acc); free(ab);

@@ -117,11 +117,13 @@ int cmfh_steady_state(const cmfh_channel* c, const double* q, double* rivsto, do
  * of nh harmonics (angular frequency omega[k] in rad/s, complex local runoff
  * amplitude r[k][i] in m3/s) the complex surface-elevation amplitude h[k][i] [m]
  * of the mass balance linearised in the outflow (src/cmf_calc_outflw_mod.F90:
- * 60-171 at steady state) and the storage (stage tables). No reference
- * counterpart: the physical prior of the spin-up surrogate. Returns 0, -1 on bad
- * arguments, -3 on a cycle. */
+ * 60-171 at steady state) and the storage (stage tables), and (o_re / o_im not
+ * NULL) the outflow amplitude o[k][i] [m3/s]. No reference counterpart: the
+ * physical prior of the spin-up surrogate. Returns 0, -1 on bad arguments, -3 on
+ * a cycle. */
 int cmfh_periodic_response(const cmfh_channel* c, const double* sfcelv, int nh, const double* omega,
-                           const double* r_re, const double* r_im, double* h_re, double* h_im);
+                           const double* r_re, const double* r_im, double* h_re, double* h_im,
+                           double* o_re, double* o_im);
 
 /* River and floodplain storage of each cell at the surface elevation sfcelv
  * (the inverse of the stage inversion of src/cmf_calc_fldstg_mod.F90:37-95 without

@@ -87,7 +87,10 @@ def _response(m, sfc, omega, r):
     hr, hi = np.empty_like(rr), np.empty_like(ri)
     w = np.array([omega])
     s = np.ascontiguousarray(sfc, np.float64)
-    assert lib.cmfh_periodic_response(C.byref(ch), ptr(s), 1, ptr(w), ptr(rr), ptr(ri), ptr(hr), ptr(hi)) == 0
+    orr, oi = np.empty_like(rr), np.empty_like(ri)
+    assert lib.cmfh_periodic_response(C.byref(ch), ptr(s), 1, ptr(w), ptr(rr), ptr(ri), ptr(hr), ptr(hi),
+                                      ptr(orr), ptr(oi)) == 0
+    _response.outflow = orr[0] + 1j * oi[0]
     return hr[0] + 1j * hi[0]
 
 
@@ -108,6 +111,10 @@ def test_periodic_response_single_reservoir_closed_form():
         got = _response(sub, np.array([s]), w, r)[0]
         want = r[0] / (1j * w * A + dq)
         assert abs(got - want) < 1e-5 * abs(want), (w, got, want)
+        # outflow amplitude dQ/ds * h, and the balance r = i w A h + o
+        o = _response.outflow[0]
+        assert abs(o - dq * want) < 1e-5 * abs(dq * want)
+        assert abs(r[0] - (1j * w * A * got + o)) < 1e-6 * abs(r[0])
 
 
 def test_periodic_response_zero_frequency_is_the_steady_sensitivity():
@@ -121,6 +128,9 @@ def test_periodic_response_zero_frequency_is_the_steady_sensitivity():
     s1 = spinup.steady_state(m, q + spinup.accumulate(m, dr))
     got = _response(m, s0["sfcelv"], 0.0, dr.astype(complex))
     want = s1["sfcelv"] - s0["sfcelv"]
+    # at zero frequency the outflow change is the accumulated runoff change
+    dq = spinup.accumulate(m, dr)
+    assert np.max(np.abs(_response.outflow.real - dq)) < 1e-6 * np.abs(dq).max()
     assert np.abs(got.imag).max() == 0.0
     scale = np.abs(want).max()
     assert scale > 0
