@@ -419,6 +419,41 @@ def spinup_section(a):
     return out
 
 
+def forcing_section(net, m, seed, res_in=0.25, calls=20):
+    """Runoff interpolation on the device (ROFF_INTERP with the input matrix) for
+    the benchmark map: input grid of res_in degrees, 2 x 2 overlaps per cell.
+    Timed with the field resident in device memory (kernel + stream sync per
+    call) and with the host-to-device copy of the field included."""
+    import torch
+    from cmf import synth
+    from cmf.model import CMFModel
+    inpx, inpy, inpa, nxin, nyin = synth.make_inpmat(net, (m.seqx, m.seqy), res_in)
+    grids = synth.make_runoff_grid(nxin, nyin, 2, seed)
+    g = CMFModel(m, "fp64")
+    g.set_inpmat(inpx, inpy, inpa, nxin, nyin)
+    dev = torch.from_numpy(grids[0]).to("cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        g.interp_runoff(dev.data_ptr())
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        g.interp_runoff(dev.data_ptr())
+    t_dev = (time.perf_counter() - t0) / calls
+    g.set_runoff_grid(grids[1])
+    t0 = time.perf_counter()
+    for k in range(calls):
+        g.set_runoff_grid(grids[k % 2])
+    t_h2d = (time.perf_counter() - t0) / calls
+    g.close()
+    inpn = inpx.shape[0]
+    nbytes = m.n * (inpn * (4 + 4 + 8) + 8) + 8 * nxin * nyin
+    out = dict(input_grid=f"{nxin}x{nyin} ({res_in} deg)", inpn=inpn, cells=m.n,
+               us_per_call_resident=round(1e6 * t_dev, 1), us_per_call_with_h2d=round(1e6 * t_h2d, 1),
+               bytes_per_call=nbytes, gbs_resident=round(nbytes / t_dev / 1e9, 1))
+    log(f"runoff interpolation: {out}")
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs of the FP64 C3 workload (scaled by the copy-kernel
@@ -463,6 +498,7 @@ def main():
                     help='launch configuration of the single-model runs: "auto" (autotuner) or a JSON object '
                          'of cmf_tune fields')
     ap.add_argument("--no-spinup", action="store_true")
+    ap.add_argument("--no-forcing", action="store_true")
     ap.add_argument("--spinup-only", action="store_true")
     ap.add_argument("--spin-epochs", type=int, default=150)
     ap.add_argument("--spin-years", type=int, default=20)
@@ -532,6 +568,13 @@ def main():
         except Exception as e:              # the headline line is still printed
             log(f"ensemble section failed: {e!r}")
             ens = {"error": repr(e)}
+    forcing = None
+    if not a.no_forcing:
+        try:
+            forcing = forcing_section(net, m, a.seed)
+        except Exception as e:
+            log(f"forcing section failed: {e!r}")
+            forcing = {"error": repr(e)}
     spin = None
     if not a.no_spinup:
         try:
@@ -570,6 +613,7 @@ def main():
         "mixed": mixed,
         "fp64_nseq_order": order_cmp,
         "ensemble": ens,
+        "forcing_interpolation": forcing,
         "spinup": spin,
     }
     print(json.dumps(out), flush=True)

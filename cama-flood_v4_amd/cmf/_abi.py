@@ -81,6 +81,9 @@ HIP_FUNCS = {
     "cmf_hip_create_ensemble": (I, [C.POINTER(cmf_dims), C.POINTER(cmf_options), C.POINTER(cmf_ensemble),
                                     C.POINTER(H)]),
     "cmf_hip_set_runoff_scaled": (I, [H, P, P]),
+    "cmf_hip_set_inpmat": (I, [H, C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_double, C.c_double]),
+    "cmf_hip_set_runoff_grid": (I, [H, P]),
+    "cmf_hip_interp_runoff": (I, [H, P]),
     "cmf_hip_ensemble_advance": (I, [H, C.c_double, C.c_int32, P, P]),
     "cmf_hip_instance_times": (I, [H, P]),
     "cmf_hip_set_tune": (I, [H, C.POINTER(cmf_tune)]),

@@ -180,6 +180,31 @@ class CMFModel:
             raise CMFError("set_runoff_scaled: base needs NSEQALL values, factor one per instance")
         self._check(self.lib.cmf_hip_set_runoff_scaled(self.h, b.ctypes.data, f.ctypes.data))
 
+    def set_inpmat(self, inpx: np.ndarray, inpy: np.ndarray, inpa: np.ndarray, nxin: int, nyin: int,
+                   drofunit: float = 86400.0 * 1000.0, rmis: float = float(np.float32(1e20))) -> None:
+        """Input matrix of the device runoff interpolation (ROFF_INTERP): INPX,
+        INPY (1-based) and INPA [m2] as [inpn, NSEQALL] arrays."""
+        ix = np.ascontiguousarray(inpx, np.int32)
+        iy = np.ascontiguousarray(inpy, np.int32)
+        a = _f64(inpa)
+        if ix.ndim != 2 or ix.shape[1] != self.map.n or iy.shape != ix.shape or a.shape != ix.shape:
+            raise CMFError("set_inpmat: inpx, inpy, inpa must be [inpn, NSEQALL]")
+        self.nxin, self.nyin = int(nxin), int(nyin)
+        self._check(self.lib.cmf_hip_set_inpmat(self.h, ix.shape[0], self.nxin, self.nyin, ix.ctypes.data,
+                                                iy.ctypes.data, a.ctypes.data, float(drofunit), float(rmis)))
+
+    def set_runoff_grid(self, grid: np.ndarray) -> None:
+        """Interpolate one input-grid runoff field [nyin, nxin] (input units,
+        e.g. mm/day) into D2RUNOFF on the device."""
+        g = _f64(grid)
+        if g.size != self.nxin * self.nyin:
+            raise CMFError("set_runoff_grid: grid must hold NXIN*NYIN values")
+        self._check(self.lib.cmf_hip_set_runoff_grid(self.h, g.ctypes.data))
+
+    def interp_runoff(self, device_ptr: int) -> None:
+        """Interpolate a runoff field that is already in device memory."""
+        self._check(self.lib.cmf_hip_interp_runoff(self.h, C.c_void_p(device_ptr)))
+
     # ------------------------------------------------------------------
     def physics_fldstg(self) -> None:
         self._check(self.lib.cmf_hip_physics_fldstg(self.h))

@@ -257,6 +257,70 @@ def make_runoff(net: SynthNetwork, nrec: int, seed: int, mean_mm_day=1.5, shape=
 
 
 # ---------------------------------------------------------------------------
+# Runoff on a coarser input grid and its input matrix (the inpmat that the
+# reference's map tools write for LINTERP, src/cmf_ctrl_forcing_mod.F90:418-458)
+
+RMIS = float(np.float32(1e20))    # the reference's default RMIS (float) widened to double
+
+
+def make_inpmat(net: SynthNetwork, cells_seq_xy, res_in: float):
+    """Input matrix of the NSEQ cells (seqx, seqy 1-based) on an input grid of
+    res_in degrees over the network's extent: for every model cell the input
+    cells it overlaps (row-major order, at most 4 when res_in >= the model
+    resolution) and the overlap areas [m2] on the sphere. Returns (inpx, inpy,
+    inpa as [inpn, n] arrays, nxin, nyin)."""
+    seqx, seqy = (np.asarray(a) for a in cells_seq_xy)
+    dx = (net.east - net.west) / net.nx
+    dy = (net.north - net.south) / net.ny
+    if res_in < max(dx, dy) * (1.0 - 1e-9):
+        raise ValueError("make_inpmat: the input grid must not be finer than the model grid")
+    nxin = int(round((net.east - net.west) / res_in))
+    nyin = int(round((net.north - net.south) / res_in))
+    lon0 = net.west + (seqx - 1) * dx
+    lat0 = net.north - (seqy - 1) * dy                       # north edge
+    n = seqx.size
+    parts = []
+    for oy in (0, 1):
+        for ox in (0, 1):
+            jx = np.floor((lon0 - net.west) / res_in + 1e-9).astype(np.int64) + ox
+            jy = np.floor((net.north - lat0) / res_in + 1e-9).astype(np.int64) + oy
+            w = np.clip(np.minimum(lon0 + dx, net.west + (jx + 1) * res_in) - np.maximum(lon0, net.west + jx * res_in),
+                        0.0, None)
+            top = np.minimum(lat0, net.north - jy * res_in)
+            bot = np.maximum(lat0 - dy, net.north - (jy + 1) * res_in)
+            h = np.clip(np.sin(np.radians(top)) - np.sin(np.radians(bot)), 0.0, None)
+            area = R_EARTH ** 2 * np.radians(w) * h
+            ok = (area > 0) & (jx < nxin) & (jy < nyin)
+            parts.append((np.where(ok, jx + 1, 0), np.where(ok, jy + 1, 0), np.where(ok, area, 0.0)))
+    inpx = np.zeros((4, n), np.int32)
+    inpy = np.zeros((4, n), np.int32)
+    inpa = np.zeros((4, n))
+    cnt = np.zeros(n, np.int64)
+    for px, py, pa in parts:                                  # compact the entries of each cell
+        has = px > 0
+        inpx[cnt[has], np.flatnonzero(has)] = px[has]
+        inpy[cnt[has], np.flatnonzero(has)] = py[has]
+        inpa[cnt[has], np.flatnonzero(has)] = pa[has]
+        cnt += has
+    inpn = int(cnt.max())
+    return inpx[:inpn].copy(), inpy[:inpn].copy(), inpa[:inpn].copy(), nxin, nyin
+
+
+def make_runoff_grid(nxin: int, nyin: int, nrec: int, seed: int, mean_mm_day: float = 1.5,
+                     missing: float = 0.02) -> np.ndarray:
+    """Runoff fields [nrec, nyin, nxin] in mm/day (gamma(0.5) marginals of a smooth
+    field) with a fraction `missing` of the cells set to RMIS."""
+    from scipy.special import gammaincinv, ndtr
+    rng = np.random.default_rng(seed + 505)
+    out = np.empty((nrec, nyin, nxin))
+    for r in range(nrec):
+        g = _fractal(nxin, nyin, 3.0, rng)
+        out[r] = gammaincinv(0.5, np.clip(ndtr(g), 1e-12, 1 - 1e-12)) * (mean_mm_day / 0.5)
+        out[r][rng.random((nyin, nxin)) < missing] = RMIS
+    return out
+
+
+# ---------------------------------------------------------------------------
 # Reference input formats
 
 def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt: float,

@@ -64,6 +64,9 @@ struct cmf_hip_ctx {
     std::vector<double> last_dt;
     InstStep* d_inst = nullptr;
     double *d_rbase = nullptr, *d_rfac = nullptr;   // staging of cmf_hip_set_runoff_scaled
+    RoffMap roff{};                                  // input matrix of cmf_hip_set_inpmat
+    double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
+    size_t grid_cap = 0;
     size_t partial_cap = 0;           // doubles in S.partials
     int hist_cap = 0;
     // device cell order: perm[p] = NSEQ index of device position p (identity when
@@ -1081,6 +1084,51 @@ int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* fa
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
+}
+
+int cmf_hip_set_inpmat(cmf_handle h, int32_t inpn, int32_t nxin, int32_t nyin, const int32_t* inpx,
+                       const int32_t* inpy, const double* inpa, double drofunit, double rmis) {
+    if (!h || !inpx || !inpy || !inpa) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->roff.inpx) return fail(CMF_ESTATE, "the input matrix is already set");
+    if (inpn <= 0 || nxin <= 0 || nyin <= 0) return fail(CMF_EINVAL, "bad INPN/NXIN/NYIN (%d, %d, %d)", inpn, nxin, nyin);
+    if (!(drofunit != 0.0)) return fail(CMF_EINVAL, "DROFUNIT must be nonzero");
+    const size_t n = (size_t)h->d.nseqall, ne = n * (size_t)inpn;
+    for (size_t e = 0; e < ne; ++e)
+        if (inpx[e] > 0 && inpy[e] < 1)
+            return fail(CMF_EINVAL, "input matrix entry %zu of cell %zu: IXIN=%d with IYIN=%d", e / n + 1, e % n + 1,
+                        inpx[e], inpy[e]);
+    std::vector<int32_t> x = to_dev(h, inpx, n, (size_t)inpn), y = to_dev(h, inpy, n, (size_t)inpn);
+    std::vector<double> a = to_dev(h, inpa, n, (size_t)inpn);
+    int32_t *dx = nullptr, *dy = nullptr;
+    double* da = nullptr;
+    TRY(put_raw(h, &dx, x.data(), ne));
+    TRY(put_raw(h, &dy, y.data(), ne));
+    TRY(put_raw(h, &da, a.data(), ne));
+    h->roff = RoffMap{inpn, nxin, nyin, (int)n, dx, dy, da, drofunit, rmis};
+    return 0;
+}
+
+int cmf_hip_interp_runoff(cmf_handle h, const double* d_grid) {
+    if (!h || !d_grid) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->roff.inpx) return fail(CMF_ESTATE, "cmf_hip_set_inpmat must come first");
+    if (h->o.precision == CMF_PREC_FP64) launch_roff_interp<double>(h->M, h->S64, h->roff, d_grid, h->stream);
+    else launch_roff_interp<float>(h->M, h->S32, h->roff, d_grid, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid) {
+    if (!h || !grid) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->roff.inpx) return fail(CMF_ESTATE, "cmf_hip_set_inpmat must come first");
+    const size_t ng = (size_t)h->roff.nxin * (size_t)h->roff.nyin;
+    if (h->grid_cap < ng) {
+        TRY(dalloc(h, &h->d_grid, ng));
+        h->grid_cap = ng;
+    }
+    HIPCHK(hipMemcpyAsync(h->d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return cmf_hip_interp_runoff(h, h->d_grid);
 }
 
 int cmf_hip_physics_fldstg(cmf_handle h) {

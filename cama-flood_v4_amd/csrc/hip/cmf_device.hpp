@@ -89,6 +89,16 @@ struct DevMap {
 
 // Per-instance step of the current outer step (written by the host after the
 // CFL bound is known): DT, PGRV*DT, NADD = sum of DT over the NT substeps.
+// input matrix of the runoff interpolation (INPX, INPY, INPA of YOS_CMF_MAP), device
+// cell order, entry-planar: entry k of cell c at k * n + c
+struct RoffMap {
+    int inpn, nxin, nyin, n;
+    const int* inpx;
+    const int* inpy;
+    const double* inpa;
+    double drofunit, rmis;
+};
+
 struct InstStep {
     double dt, gdt, nadd;
     int32_t nt;                                // substeps of this instance

@@ -864,6 +864,27 @@ __global__ void k_scale_runoff(DevMap M, DevState<R> S, const double* base, cons
     S.runoff[s] = R(base[x.c] * factor[x.inst]);
 }
 
+// ROFF_INTERP (src/cmf_ctrl_forcing_mod.F90:757-790): runoff of a cell = sum over
+// its input-matrix entries, in entry order, of grid value * INPA / DROFUNIT, for
+// entries inside the grid (IXIN > 0, IXIN <= NXIN, IYIN <= NYIN) whose value is
+// not RMIS. Every member slot of a cell receives the cell's value.
+template <class R, bool MB>
+__global__ void k_roff_interp(DevMap M, DevState<R> S, RoffMap Q, const double* grid) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M.ns) return;
+    const int c = cell_slot<MB>(M, s).c;
+    double acc = 0.0;
+    for (int k = 0; k < Q.inpn; ++k) {
+        const size_t e = (size_t)k * Q.n + c;
+        const int ix = Q.inpx[e], iy = Q.inpy[e];
+        if (ix > 0 && ix <= Q.nxin && iy >= 1 && iy <= Q.nyin) {
+            const double v = grid[(size_t)(iy - 1) * Q.nxin + (ix - 1)];
+            if (v != Q.rmis) acc = acc + (v * Q.inpa[e]) / Q.drofunit;
+        }
+    }
+    S.runoff[s] = R(acc);
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers: MB (several instances) is chosen from the map, the block
 // size from the launch configuration
@@ -954,6 +975,13 @@ void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* ba
     else hipLaunchKernelGGL((k_scale_runoff<R, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, base, factor);
 }
 
+template <class R>
+void launch_roff_interp(const DevMap& M, const DevState<R>& S, const RoffMap& Q, const double* grid,
+                        hipStream_t st) {
+    if (multi(M)) hipLaunchKernelGGL((k_roff_interp<R, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, Q, grid);
+    else hipLaunchKernelGGL((k_roff_interp<R, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, Q, grid);
+}
+
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(RBS), 0, st, partials, ntiles, glb_hist);
 }
@@ -967,7 +995,9 @@ void launch_budget(const double* partials, int ntiles, int nrun, double* glb_his
                                           hipStream_t);                                                        \
     template void launch_swapfix<R>(const DevMap&, const DevState<R>&, hipStream_t);                           \
     template void launch_scale_runoff<R>(const DevMap&, const DevState<R>&, const double*, const double*,     \
-                                         hipStream_t);
+                                         hipStream_t);                                                          \
+    template void launch_roff_interp<R>(const DevMap&, const DevState<R>&, const RoffMap&, const double*,      \
+                                        hipStream_t);
 CMF_INSTANTIATE(double)
 CMF_INSTANTIATE(float)
 

@@ -29,6 +29,9 @@ void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st);
 template <class R>
 void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* base, const double* factor,
                          hipStream_t st);
+template <class R>
+void launch_roff_interp(const DevMap& M, const DevState<R>& S, const RoffMap& Q, const double* grid,
+                        hipStream_t st);
 // reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st);
 

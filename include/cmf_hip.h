@@ -144,6 +144,22 @@ int cmf_hip_set_runoff(cmf_handle h, const double* runoff);
  * NSEQALL values, factor ndom*members). No reference counterpart. */
 int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* factor);
 
+/* Runoff interpolation on the device: ROFF_INTERP of CMF_FORCING_PUT with LINTERP
+ * (src/cmf_ctrl_forcing_mod.F90:757-790). cmf_hip_set_inpmat uploads the input
+ * matrix INPX, INPY (1-based input-grid indices, <= 0 = no entry) and INPA [m2]
+ * of YOS_CMF_MAP, each [inpn][NSEQALL] (Fortran INPX(NSEQMAX,INPN) as it lies in
+ * memory), with DROFUNIT and RMIS (the float RMIS widened to double). Entries with
+ * IXIN > 0 must have IYIN >= 1. cmf_hip_set_runoff_grid uploads one runoff field
+ * PBUFFIN(NXIN,NYIN) (x fastest) and interpolates it into D2RUNOFF;
+ * cmf_hip_interp_runoff does the same from a field already in device memory (the
+ * forcing kept resident). The sum over a cell's entries runs in entry order with
+ * the reference's operation order, so D2RUNOFF is bit-identical to the host
+ * routine's. Every member of an ensemble receives its cell's value. */
+int cmf_hip_set_inpmat(cmf_handle h, int32_t inpn, int32_t nxin, int32_t nyin, const int32_t* inpx,
+                       const int32_t* inpy, const double* inpa, double drofunit, double rmis);
+int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid);
+int cmf_hip_interp_runoff(cmf_handle h, const double* d_grid);
+
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);
 

@@ -1,6 +1,7 @@
 #!/bin/bash
 # Builds the reference CaMa-Flood Fortran (read in place from /root/reference/src,
-# unmodified) plus the test driver oracle/ref_driver.F90 into oracle/_ref/.
+# unmodified) plus the test drivers oracle/ref_driver.F90 and (serial64 only)
+# oracle/ref_forcing.F90 into oracle/_ref/.
 # oracle/_ref/ is test infrastructure: it pins the C oracle (golden dumps) and is
 # timed as the CPU baseline. Nothing in the product links or loads it.
 #
@@ -57,4 +58,10 @@ for V in $VARIANTS; do
   "$FC" $FFLAGS -module-dir "$D/obj" -I "$D/obj" -c "$HERE/ref_driver.F90" -o "$D/obj/ref_driver.o"
   "$FC" $FFLAGS -o "$D/cmf_ref" "$D/obj/ref_driver.o" $OBJS
   echo "build_ref: built $D/cmf_ref"
+  if [ $V = serial64 ]; then
+    # the reference's CMF_FORCING_PUT (runoff interpolation) on a given input matrix
+    "$FC" $FFLAGS -module-dir "$D/obj" -I "$D/obj" -c "$HERE/ref_forcing.F90" -o "$D/obj/ref_forcing.o"
+    "$FC" $FFLAGS -o "$D/cmf_ref_forcing" "$D/obj/ref_forcing.o" $OBJS
+    echo "build_ref: built $D/cmf_ref_forcing"
+  fi
 done

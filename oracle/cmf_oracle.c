@@ -782,3 +782,24 @@ int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
     cmfo_diag_getave_adpstp(m);                                             /* :147 */
     return nt;
 }
+
+/* ------------------------------------------------------------------------
+ * ROFF_INTERP  src/cmf_ctrl_forcing_mod.F90:757-790 (inside CMF_FORCING_PUT) */
+void cmfo_roff_interp(int32_t n, int32_t inpn, int32_t nxin, int32_t nyin, const int32_t* inpx,
+                      const int32_t* inpy, const double* inpa, const double* grid, double drofunit,
+                      double rmis, double* out)
+{
+    for (int32_t i = 0; i < n; ++i) {                                       /* :770 */
+        out[i] = 0.0;                                                       /* :771 */
+        for (int32_t k = 0; k < inpn; ++k) {                                /* :772 */
+            const int32_t ix = inpx[(size_t)k * n + i];                     /* :773 */
+            const int32_t iy = inpy[(size_t)k * n + i];                     /* :774 */
+            if (ix > 0) {                                                   /* :775 */
+                if (ix > nxin || iy > nyin) continue;                       /* :776-780 */
+                const double v = grid[(size_t)(iy - 1) * nxin + (ix - 1)];
+                if (v != rmis)                                              /* :782 */
+                    out[i] = out[i] + (v * inpa[(size_t)k * n + i]) / drofunit;   /* :783 */
+            }
+        }
+    }
+}

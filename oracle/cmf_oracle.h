@@ -93,6 +93,13 @@ void cmfo_diag_reset_adpstp(cmfo_model* m);
 void cmfo_diag_avemax_adpstp(cmfo_model* m, double dt);
 void cmfo_diag_getave_adpstp(cmfo_model* m);
 
+/* ROFF_INTERP (src/cmf_ctrl_forcing_mod.F90:757-790): out[i] = sum over the input
+ * matrix entries k of cell i (inpx/inpy 1-based, [inpn][n]) with IXIN > 0 inside
+ * the NXIN x NYIN grid and grid value != rmis of grid * INPA / DROFUNIT. */
+void cmfo_roff_interp(int32_t n, int32_t inpn, int32_t nxin, int32_t nyin, const int32_t* inpx,
+                      const int32_t* inpy, const double* inpa, const double* grid, double drofunit,
+                      double rmis, double* out);
+
 #ifdef __cplusplus
 }
 #endif

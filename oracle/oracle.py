@@ -67,7 +67,24 @@ def _lib():
               "cmfo_diag_avemax_adpstp"):
         getattr(lib, f).restype = None
         getattr(lib, f).argtypes = [C.POINTER(_Model), C.c_double]
+    P = C.c_void_p
+    lib.cmfo_roff_interp.restype = None
+    lib.cmfo_roff_interp.argtypes = [C.c_int32] * 4 + [P, P, P, P, C.c_double, C.c_double, P]
     return lib
+
+
+def roff_interp(inpx, inpy, inpa, nxin, nyin, grid, drofunit, rmis):
+    """ROFF_INTERP of the C oracle (src/cmf_ctrl_forcing_mod.F90:757-790):
+    inpx/inpy/inpa [inpn, n], grid [nyin, nxin] -> runoff [n]."""
+    ix = np.ascontiguousarray(inpx, np.int32)
+    iy = np.ascontiguousarray(inpy, np.int32)
+    a = np.ascontiguousarray(inpa, np.float64)
+    g = np.ascontiguousarray(grid, np.float64)
+    inpn, n = ix.shape
+    out = np.empty(n)
+    _lib().cmfo_roff_interp(n, inpn, nxin, nyin, ix.ctypes.data, iy.ctypes.data, a.ctypes.data,
+                            g.ctypes.data, float(drofunit), float(rmis), out.ctypes.data)
+    return out
 
 
 class OracleModel:
