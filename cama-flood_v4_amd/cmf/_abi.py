@@ -64,6 +64,14 @@ class cmf_state(C.Structure):
     _fields_ = [(k, P) for k in STATE_FIELDS]
 
 
+OUTPUT_FIELDS = ("rivout fldout outflw rivvel pthout runoff daminf outflw_max rivdph_max storge_max "
+                 "pthflw").split()
+
+
+class cmf_output(C.Structure):
+    _fields_ = [(k, P) for k in OUTPUT_FIELDS]
+
+
 class cmf_ensemble(C.Structure):
     _fields_ = [("ndom", C.c_int32), ("dom_cells", P), ("members", C.c_int32)]
 
@@ -84,6 +92,10 @@ HIP_FUNCS = {
     "cmf_hip_set_inpmat": (I, [H, C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_double, C.c_double]),
     "cmf_hip_set_runoff_grid": (I, [H, P]),
     "cmf_hip_interp_runoff": (I, [H, P]),
+    "cmf_hip_diag_reset_output": (I, [H]),
+    "cmf_hip_diag_avemax_output": (I, [H, C.c_double]),
+    "cmf_hip_diag_getave_output": (I, [H, C.POINTER(C.c_double)]),
+    "cmf_hip_get_output": (I, [H, C.POINTER(cmf_output)]),
     "cmf_hip_ensemble_advance": (I, [H, C.c_double, C.c_int32, P, P]),
     "cmf_hip_instance_times": (I, [H, P]),
     "cmf_hip_set_tune": (I, [H, C.POINTER(cmf_tune)]),

@@ -24,8 +24,8 @@ from typing import Iterable, Optional
 import numpy as np
 
 from . import _native
-from ._abi import (STATE_FIELDS, TUNE_FIELDS, cmf_dims, cmf_ensemble, cmf_map, cmf_options, cmf_state,
-                   cmf_tune, ptr)
+from ._abi import (OUTPUT_FIELDS, STATE_FIELDS, TUNE_FIELDS, cmf_dims, cmf_ensemble, cmf_map, cmf_options,
+                   cmf_output, cmf_state, cmf_tune, ptr)
 from .maps import CMFMap
 
 PREC_FP64 = 0
@@ -204,6 +204,32 @@ class CMFModel:
     def interp_runoff(self, device_ptr: int) -> None:
         """Interpolate a runoff field that is already in device memory."""
         self._check(self.lib.cmf_hip_interp_runoff(self.h, C.c_void_p(device_ptr)))
+
+    # output-period diagnostics (CMF_DIAG_*_OUTPUT) ----------------------
+    def diag_reset_output(self) -> None:
+        self._check(self.lib.cmf_hip_diag_reset_output(self.h))
+
+    def diag_avemax_output(self, dt: float) -> None:
+        self._check(self.lib.cmf_hip_diag_avemax_output(self.h, float(dt)))
+
+    def diag_getave_output(self) -> float:
+        """Turns the accumulated sums into averages; returns NADD_out."""
+        nadd = C.c_double(0.0)
+        self._check(self.lib.cmf_hip_diag_getave_output(self.h, C.byref(nadd)))
+        return nadd.value
+
+    def get_output(self, names: Optional[Iterable[str]] = None) -> dict:
+        m = self.map
+        names = list(names) if names is not None else [
+            k for k in OUTPUT_FIELDS if (k != "pthflw" or self.opts.lpthout) and (k != "daminf" or self.opts.ldamout)]
+        lead = (self.members,) if self.members > 1 else ()
+        out, s = {}, cmf_output()
+        for k in names:
+            a = np.zeros(lead + ((m.npthlev, m.npth) if k == "pthflw" else (m.n,)))
+            out[k] = a
+            setattr(s, k, a.ctypes.data)
+        self._check(self.lib.cmf_hip_get_output(self.h, C.byref(s)))
+        return out
 
     # ------------------------------------------------------------------
     def physics_fldstg(self) -> None:

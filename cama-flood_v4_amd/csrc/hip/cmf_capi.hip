@@ -65,6 +65,8 @@ struct cmf_hip_ctx {
     InstStep* d_inst = nullptr;
     double *d_rbase = nullptr, *d_rfac = nullptr;   // staging of cmf_hip_set_runoff_scaled
     RoffMap roff{};                                  // input matrix of cmf_hip_set_inpmat
+    OutState out{};                                  // output-period accumulators
+    double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
     size_t grid_cap = 0;
     size_t partial_cap = 0;           // doubles in S.partials
@@ -1129,6 +1131,70 @@ int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid) {
     }
     HIPCHK(hipMemcpyAsync(h->d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return cmf_hip_interp_runoff(h, h->d_grid);
+}
+
+namespace {
+int npth_slots(const cmf_hip_ctx* h) { return h->o.lpthout ? h->M.nsp * h->d.npthlev : 0; }
+
+int out_alloc(cmf_hip_ctx* h) {
+    if (h->out.rivout) return 0;
+    const size_t ns = (size_t)h->M.ns, np = (size_t)npth_slots(h);
+    double** cell[] = {&h->out.rivout, &h->out.fldout, &h->out.outflw, &h->out.rivvel, &h->out.pthout,
+                       &h->out.runoff, &h->out.daminf, &h->out.outflw_max, &h->out.rivdph_max,
+                       &h->out.storge_max};
+    for (double** p : cell) TRY(dalloc(h, p, ns));          // dalloc clears
+    TRY(dalloc(h, &h->out.pthflw, np));
+    h->nadd_out = 0.0;
+    return 0;
+}
+}  // namespace
+
+int cmf_hip_diag_reset_output(cmf_handle h) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    TRY(out_alloc(h));
+    launch_out_reset(h->out, h->M.ns, npth_slots(h), h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->nadd_out = 0.0;
+    return 0;
+}
+
+int cmf_hip_diag_avemax_output(cmf_handle h, double dt) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    TRY(out_alloc(h));
+    h->nadd_out = h->nadd_out + dt;                         // diag:277
+    if (h->o.precision == CMF_PREC_FP64)
+        launch_out_avemax<double>(h->M, h->S64, h->out, dt, h->o.ldamout, npth_slots(h), h->stream);
+    else
+        launch_out_avemax<float>(h->M, h->S32, h->out, dt, h->o.ldamout, npth_slots(h), h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int cmf_hip_diag_getave_output(cmf_handle h, double* nadd_out) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->out.rivout) return fail(CMF_ESTATE, "no output period: call cmf_hip_diag_avemax_output first");
+    launch_out_getave(h->out, h->M.ns, npth_slots(h), h->nadd_out, h->o.ldamout, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (nadd_out) *nadd_out = h->nadd_out;
+    return 0;
+}
+
+int cmf_hip_get_output(cmf_handle h, const cmf_output* o) {
+    if (!h || !o) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->out.rivout) return fail(CMF_ESTATE, "no output period: call cmf_hip_diag_avemax_output first");
+    const std::pair<double*, const double*> cell[] = {
+        {o->rivout, h->out.rivout}, {o->fldout, h->out.fldout}, {o->outflw, h->out.outflw},
+        {o->rivvel, h->out.rivvel}, {o->pthout, h->out.pthout}, {o->runoff, h->out.runoff},
+        {o->daminf, h->out.daminf}, {o->outflw_max, h->out.outflw_max}, {o->rivdph_max, h->out.rivdph_max},
+        {o->storge_max, h->out.storge_max}};
+    for (const auto& c : cell) TRY(get<double>(h, c.first, c.second, K_CELL));
+    if (o->pthflw && npth_slots(h) > 0) TRY(get<double>(h, o->pthflw, h->out.pthflw, K_PTHL));
+    return 0;
 }
 
 int cmf_hip_physics_fldstg(cmf_handle h) {

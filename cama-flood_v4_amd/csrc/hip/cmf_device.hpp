@@ -99,6 +99,14 @@ struct RoffMap {
     double drofunit, rmis;
 };
 
+// output-period accumulators (D2*_oAVG / D2*_oMAX of YOS_CMF_DIAG), FP64 in both
+// precisions, one value per state slot; pthflw [npthlev][nsp]
+struct OutState {
+    double *rivout, *fldout, *outflw, *rivvel, *pthout, *runoff, *daminf;
+    double *outflw_max, *rivdph_max, *storge_max;
+    double* pthflw;
+};
+
 struct InstStep {
     double dt, gdt, nadd;
     int32_t nt;                                // substeps of this instance

@@ -885,6 +885,54 @@ __global__ void k_roff_interp(DevMap M, DevState<R> S, RoffMap Q, const double* 
     S.runoff[s] = R(acc);
 }
 
+// CMF_DIAG_AVEMAX_OUTPUT (src/cmf_calc_diag_mod.F90:273-317): after an outer step
+// of length dt, o += aAVG * dt for the averages and o = max(o, aMAX) for the
+// maxima (|aMAX| for OUTFLW), per slot; paths per level and path slot.
+template <class R>
+__global__ void k_out_avemax(DevMap M, DevState<R> S, OutState O, double dt, int ldamout, int npth_slots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < M.ns) {
+        O.rivout[s] = O.rivout[s] + (double)S.rivout_avg[s] * dt;
+        O.fldout[s] = O.fldout[s] + (double)S.fldout_avg[s] * dt;
+        O.rivvel[s] = O.rivvel[s] + (double)S.rivvel_avg[s] * dt;
+        O.outflw[s] = O.outflw[s] + (double)S.outflw_avg[s] * dt;
+        O.pthout[s] = O.pthout[s] + (double)S.pthout_avg[s] * dt;
+        O.runoff[s] = O.runoff[s] + (double)S.runoff_avg[s] * dt;
+        O.outflw_max[s] = fmax(O.outflw_max[s], fabs((double)S.outflw_max[s]));
+        O.rivdph_max[s] = fmax(O.rivdph_max[s], (double)S.rivdph_max[s]);
+        O.storge_max[s] = fmax(O.storge_max[s], (double)S.storge_max[s]);
+        if (ldamout) O.daminf[s] = O.daminf[s] + (double)S.daminf_avg[s] * dt;
+    }
+    if (s < npth_slots) O.pthflw[s] = O.pthflw[s] + (double)S.pthflw_avg[s] * dt;
+}
+
+// CMF_DIAG_GETAVE_OUTPUT (src/cmf_calc_diag_mod.F90:323-356): averages / NADD_out
+__global__ void k_out_getave(OutState O, int ns, int npth_slots, double nadd, int ldamout) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) {
+        O.rivout[s] = O.rivout[s] / nadd;
+        O.fldout[s] = O.fldout[s] / nadd;
+        O.outflw[s] = O.outflw[s] / nadd;
+        O.rivvel[s] = O.rivvel[s] / nadd;
+        O.pthout[s] = O.pthout[s] / nadd;
+        O.runoff[s] = O.runoff[s] / nadd;
+        if (ldamout) O.daminf[s] = O.daminf[s] / nadd;
+    }
+    if (s < npth_slots) O.pthflw[s] = O.pthflw[s] / nadd;
+}
+
+// CMF_DIAG_RESET_OUTPUT (src/cmf_calc_diag_mod.F90:222-267): zero every accumulator
+// (a kernel, not a memset, so that a captured graph may hold it)
+__global__ void k_out_reset(OutState O, int ns, int npth_slots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) {
+        O.rivout[s] = 0.0; O.fldout[s] = 0.0; O.outflw[s] = 0.0; O.rivvel[s] = 0.0;
+        O.pthout[s] = 0.0; O.runoff[s] = 0.0; O.daminf[s] = 0.0;
+        O.outflw_max[s] = 0.0; O.rivdph_max[s] = 0.0; O.storge_max[s] = 0.0;
+    }
+    if (s < npth_slots) O.pthflw[s] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers: MB (several instances) is chosen from the map, the block
 // size from the launch configuration
@@ -982,6 +1030,23 @@ void launch_roff_interp(const DevMap& M, const DevState<R>& S, const RoffMap& Q,
     else hipLaunchKernelGGL((k_roff_interp<R, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, Q, grid);
 }
 
+template <class R>
+void launch_out_avemax(const DevMap& M, const DevState<R>& S, const OutState& O, double dt, int ldamout,
+                       int npth_slots, hipStream_t st) {
+    const int n = M.ns > npth_slots ? M.ns : npth_slots;
+    hipLaunchKernelGGL((k_out_avemax<R>), dim3(cdiv(n, 256)), dim3(256), 0, st, M, S, O, dt, ldamout, npth_slots);
+}
+
+void launch_out_getave(const OutState& O, int ns, int npth_slots, double nadd, int ldamout, hipStream_t st) {
+    const int n = ns > npth_slots ? ns : npth_slots;
+    hipLaunchKernelGGL(k_out_getave, dim3(cdiv(n, 256)), dim3(256), 0, st, O, ns, npth_slots, nadd, ldamout);
+}
+
+void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st) {
+    const int n = ns > npth_slots ? ns : npth_slots;
+    hipLaunchKernelGGL(k_out_reset, dim3(cdiv(n, 256)), dim3(256), 0, st, O, ns, npth_slots);
+}
+
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(RBS), 0, st, partials, ntiles, glb_hist);
 }
@@ -997,7 +1062,9 @@ void launch_budget(const double* partials, int ntiles, int nrun, double* glb_his
     template void launch_scale_runoff<R>(const DevMap&, const DevState<R>&, const double*, const double*,     \
                                          hipStream_t);                                                          \
     template void launch_roff_interp<R>(const DevMap&, const DevState<R>&, const RoffMap&, const double*,      \
-                                        hipStream_t);
+                                        hipStream_t);                                                           \
+    template void launch_out_avemax<R>(const DevMap&, const DevState<R>&, const OutState&, double, int, int,   \
+                                       hipStream_t);
 CMF_INSTANTIATE(double)
 CMF_INSTANTIATE(float)
 

@@ -32,6 +32,11 @@ void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* ba
 template <class R>
 void launch_roff_interp(const DevMap& M, const DevState<R>& S, const RoffMap& Q, const double* grid,
                         hipStream_t st);
+template <class R>
+void launch_out_avemax(const DevMap& M, const DevState<R>& S, const OutState& O, double dt, int ldamout,
+                       int npth_slots, hipStream_t st);
+void launch_out_getave(const OutState& O, int ns, int npth_slots, double nadd, int ldamout, hipStream_t st);
+void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st);
 // reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st);
 

@@ -160,6 +160,26 @@ int cmf_hip_set_inpmat(cmf_handle h, int32_t inpn, int32_t nxin, int32_t nyin, c
 int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid);
 int cmf_hip_interp_runoff(cmf_handle h, const double* d_grid);
 
+/* Output-period diagnostics on the device: CMF_DIAG_RESET_OUTPUT,
+ * CMF_DIAG_AVEMAX_OUTPUT and CMF_DIAG_GETAVE_OUTPUT (src/cmf_calc_diag_mod.F90:
+ * 222-356), called where the reference's CMF_DRV_ADVANCE calls them
+ * (src/cmf_drv_advance_mod.F90:114,123,143): avemax after every outer step with
+ * its length DT, getave at output time (returns NADD_out), reset after the
+ * output is written. The accumulators (D2*_oAVG, D2*_oMAX, D1PTHFLW_oAVG) stay
+ * in device memory, FP64 in both precisions; cmf_hip_get_output copies the
+ * non-NULL ones out (NSEQ order; members planes for ensembles), so the per-step
+ * download of the substep averages is not needed. GDWRTN, ROFSUB and WEVAPEX
+ * averages are zero with the options this library supports. */
+typedef struct {
+    double *rivout, *fldout, *outflw, *rivvel, *pthout, *runoff, *daminf;   /* D2*_oAVG */
+    double *outflw_max, *rivdph_max, *storge_max;                           /* D2*_oMAX */
+    double* pthflw;                                                          /* D1PTHFLW_oAVG */
+} cmf_output;
+int cmf_hip_diag_reset_output(cmf_handle h);
+int cmf_hip_diag_avemax_output(cmf_handle h, double dt);
+int cmf_hip_diag_getave_output(cmf_handle h, double* nadd_out);
+int cmf_hip_get_output(cmf_handle h, const cmf_output* out);
+
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);
 

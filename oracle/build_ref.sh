@@ -1,7 +1,7 @@
 #!/bin/bash
 # Builds the reference CaMa-Flood Fortran (read in place from /root/reference/src,
 # unmodified) plus the test drivers oracle/ref_driver.F90 and (serial64 only)
-# oracle/ref_forcing.F90 into oracle/_ref/.
+# oracle/ref_forcing.F90 and oracle/ref_diag.F90 into oracle/_ref/.
 # oracle/_ref/ is test infrastructure: it pins the C oracle (golden dumps) and is
 # timed as the CPU baseline. Nothing in the product links or loads it.
 #
@@ -63,5 +63,9 @@ for V in $VARIANTS; do
     "$FC" $FFLAGS -module-dir "$D/obj" -I "$D/obj" -c "$HERE/ref_forcing.F90" -o "$D/obj/ref_forcing.o"
     "$FC" $FFLAGS -o "$D/cmf_ref_forcing" "$D/obj/ref_forcing.o" $OBJS
     echo "build_ref: built $D/cmf_ref_forcing"
+    # the reference's output-period diagnostics (CMF_DIAG_*_OUTPUT)
+    "$FC" $FFLAGS -module-dir "$D/obj" -I "$D/obj" -c "$HERE/ref_diag.F90" -o "$D/obj/ref_diag.o"
+    "$FC" $FFLAGS -o "$D/cmf_ref_diag" "$D/obj/ref_diag.o" $OBJS
+    echo "build_ref: built $D/cmf_ref_diag"
   fi
 done

@@ -803,3 +803,57 @@ void cmfo_roff_interp(int32_t n, int32_t inpn, int32_t nxin, int32_t nyin, const
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * CMF_DIAG_RESET_OUTPUT  src/cmf_calc_diag_mod.F90:222-267 */
+void cmfo_diag_reset_output(cmfo_output* o, int32_t n, int32_t npthl, int32_t ldamout)
+{
+    o->nadd = 0.0;                                                          /* :229 */
+    for (int32_t i = 0; i < n; ++i) {                                       /* :231-245 */
+        o->rivout[i] = 0.0; o->fldout[i] = 0.0; o->outflw[i] = 0.0; o->rivvel[i] = 0.0;
+        o->pthout[i] = 0.0; o->runoff[i] = 0.0;
+        o->storge_max[i] = 0.0; o->outflw_max[i] = 0.0; o->rivdph_max[i] = 0.0;
+    }
+    if (ldamout)                                                            /* :248-254 */
+        for (int32_t i = 0; i < n; ++i) o->daminf[i] = 0.0;
+    for (int32_t k = 0; k < npthl; ++k) o->pthflw[k] = 0.0;                 /* :262-265 */
+}
+
+/* ------------------------------------------------------------------------
+ * CMF_DIAG_AVEMAX_OUTPUT  src/cmf_calc_diag_mod.F90:273-317 */
+void cmfo_diag_avemax_output(cmfo_output* o, const cmfo_output* a, int32_t n, int32_t npthl, int32_t ldamout,
+                             double dt)
+{
+    o->nadd = o->nadd + dt;                                                 /* :277 */
+    for (int32_t i = 0; i < n; ++i) {                                       /* :280-299 */
+        o->rivout[i] = o->rivout[i] + a->rivout[i] * dt;
+        o->fldout[i] = o->fldout[i] + a->fldout[i] * dt;
+        o->rivvel[i] = o->rivvel[i] + a->rivvel[i] * dt;
+        o->outflw[i] = o->outflw[i] + a->outflw[i] * dt;
+        o->pthout[i] = o->pthout[i] + a->pthout[i] * dt;
+        o->runoff[i] = o->runoff[i] + a->runoff[i] * dt;
+        o->outflw_max[i] = DMAX(o->outflw_max[i], fabs(a->outflw_max[i]));
+        o->rivdph_max[i] = DMAX(o->rivdph_max[i], a->rivdph_max[i]);
+        o->storge_max[i] = DMAX(o->storge_max[i], a->storge_max[i]);
+    }
+    if (ldamout)                                                            /* :303-309 */
+        for (int32_t i = 0; i < n; ++i) o->daminf[i] = o->daminf[i] + a->daminf[i] * dt;
+    for (int32_t k = 0; k < npthl; ++k) o->pthflw[k] = o->pthflw[k] + a->pthflw[k] * dt;   /* :311-315 */
+}
+
+/* ------------------------------------------------------------------------
+ * CMF_DIAG_GETAVE_OUTPUT  src/cmf_calc_diag_mod.F90:323-356 */
+void cmfo_diag_getave_output(cmfo_output* o, int32_t n, int32_t npthl, int32_t ldamout)
+{
+    const double d = o->nadd;
+    for (int32_t i = 0; i < n; ++i) {                                       /* :331-348 */
+        o->rivout[i] = o->rivout[i] / d;
+        o->fldout[i] = o->fldout[i] / d;
+        o->outflw[i] = o->outflw[i] / d;
+        o->rivvel[i] = o->rivvel[i] / d;
+        o->pthout[i] = o->pthout[i] / d;
+        o->runoff[i] = o->runoff[i] / d;
+        if (ldamout) o->daminf[i] = o->daminf[i] / d;
+    }
+    for (int32_t k = 0; k < npthl; ++k) o->pthflw[k] = o->pthflw[k] / d;   /* :351-354 */
+}

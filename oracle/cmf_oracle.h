@@ -93,6 +93,23 @@ void cmfo_diag_reset_adpstp(cmfo_model* m);
 void cmfo_diag_avemax_adpstp(cmfo_model* m, double dt);
 void cmfo_diag_getave_adpstp(cmfo_model* m);
 
+/* Output-period accumulators (D2*_oAVG, D2*_oMAX, D1PTHFLW_oAVG of YOS_CMF_DIAG),
+ * or, as the input of cmfo_diag_avemax_output, the substep averages / maxima of
+ * one outer step (D2*_aAVG, D2*_aMAX). n cells, npthl = NPTHOUT*NPTHLEV. */
+typedef struct cmfo_output {
+    double *rivout, *fldout, *outflw, *rivvel, *pthout, *runoff, *daminf;
+    double *outflw_max, *rivdph_max, *storge_max;
+    double* pthflw;
+    double nadd;
+} cmfo_output;
+/* CMF_DIAG_RESET_OUTPUT (src/cmf_calc_diag_mod.F90:222-267) */
+void cmfo_diag_reset_output(cmfo_output* o, int32_t n, int32_t npthl, int32_t ldamout);
+/* CMF_DIAG_AVEMAX_OUTPUT (src/cmf_calc_diag_mod.F90:273-317), DT = dt */
+void cmfo_diag_avemax_output(cmfo_output* o, const cmfo_output* a, int32_t n, int32_t npthl, int32_t ldamout,
+                             double dt);
+/* CMF_DIAG_GETAVE_OUTPUT (src/cmf_calc_diag_mod.F90:323-356) */
+void cmfo_diag_getave_output(cmfo_output* o, int32_t n, int32_t npthl, int32_t ldamout);
+
 /* ROFF_INTERP (src/cmf_ctrl_forcing_mod.F90:757-790): out[i] = sum over the input
  * matrix entries k of cell i (inpx/inpy 1-based, [inpn][n]) with IXIN > 0 inside
  * the NXIN x NYIN grid and grid value != rmis of grid * INPA / DROFUNIT. */

@@ -68,9 +68,45 @@ def _lib():
         getattr(lib, f).restype = None
         getattr(lib, f).argtypes = [C.POINTER(_Model), C.c_double]
     P = C.c_void_p
+    for f in ("cmfo_diag_reset_output", "cmfo_diag_getave_output"):
+        getattr(lib, f).restype = None
+        getattr(lib, f).argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
+    lib.cmfo_diag_avemax_output.restype = None
+    lib.cmfo_diag_avemax_output.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_double]
     lib.cmfo_roff_interp.restype = None
     lib.cmfo_roff_interp.argtypes = [C.c_int32] * 4 + [P, P, P, P, C.c_double, C.c_double, P]
     return lib
+
+
+OUT_FIELDS = ("rivout", "fldout", "outflw", "rivvel", "pthout", "runoff", "daminf",
+              "outflw_max", "rivdph_max", "storge_max", "pthflw")
+
+
+class _Output(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in OUT_FIELDS] + [("nadd", C.c_double)]
+
+
+class OracleOutput:
+    """Output-period accumulators of the oracle (CMF_DIAG_*_OUTPUT,
+    src/cmf_calc_diag_mod.F90:222-356) over n cells and npthl path levels."""
+
+    def __init__(self, n: int, npthl: int, ldamout: bool):
+        self.n, self.npthl, self.ldamout = n, npthl, int(ldamout)
+        self.arr = {k: np.zeros(npthl if k == "pthflw" else n) for k in OUT_FIELDS}
+        self.s = _Output(**{k: a.ctypes.data for k, a in self.arr.items()}, nadd=0.0)
+        self.lib = _lib()
+        self.lib.cmfo_diag_reset_output(C.byref(self.s), n, npthl, self.ldamout)
+
+    def avemax(self, a: dict, dt: float) -> None:
+        """a: substep averages / maxima of one outer step (keys of OUT_FIELDS)."""
+        keep = {k: np.ascontiguousarray(a[k], np.float64) for k in OUT_FIELDS}
+        src = _Output(**{k: v.ctypes.data for k, v in keep.items()}, nadd=0.0)
+        self.lib.cmfo_diag_avemax_output(C.byref(self.s), C.byref(src), self.n, self.npthl, self.ldamout,
+                                         float(dt))
+
+    def getave(self) -> float:
+        self.lib.cmfo_diag_getave_output(C.byref(self.s), self.n, self.npthl, self.ldamout)
+        return self.s.nadd
 
 
 def roff_interp(inpx, inpy, inpa, nxin, nyin, grid, drofunit, rmis):
