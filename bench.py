@@ -423,7 +423,8 @@ def forcing_section(net, m, seed, res_in=0.25, calls=20):
     """Runoff interpolation on the device (ROFF_INTERP with the input matrix) for
     the benchmark map: input grid of res_in degrees, 2 x 2 overlaps per cell.
     Timed with the field resident in device memory (kernel + stream sync per
-    call) and with the host-to-device copy of the field included."""
+    call) and with the host-to-device copy of the field included; and the
+    output-period accumulation (CMF_DIAG_AVEMAX_OUTPUT) of the same map."""
     import torch
     from cmf import synth
     from cmf.model import CMFModel
@@ -444,13 +445,25 @@ def forcing_section(net, m, seed, res_in=0.25, calls=20):
     for k in range(calls):
         g.set_runoff_grid(grids[k % 2])
     t_h2d = (time.perf_counter() - t0) / calls
+    # output-period accumulation (CMF_DIAG_AVEMAX_OUTPUT) on the same handle:
+    # 10 substep averages/maxima read, 10 accumulators read and written per cell
+    g.diag_reset_output()
+    for _ in range(3):
+        g.diag_avemax_output(3600.0)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        g.diag_avemax_output(3600.0)
+    t_out = (time.perf_counter() - t0) / calls
     g.close()
     inpn = inpx.shape[0]
     nbytes = m.n * (inpn * (4 + 4 + 8) + 8) + 8 * nxin * nyin
+    obytes = m.n * (10 * 8 + 10 * 16)
     out = dict(input_grid=f"{nxin}x{nyin} ({res_in} deg)", inpn=inpn, cells=m.n,
                us_per_call_resident=round(1e6 * t_dev, 1), us_per_call_with_h2d=round(1e6 * t_h2d, 1),
-               bytes_per_call=nbytes, gbs_resident=round(nbytes / t_dev / 1e9, 1))
-    log(f"runoff interpolation: {out}")
+               bytes_per_call=nbytes, gbs_resident=round(nbytes / t_dev / 1e9, 1),
+               output_avemax_us_per_call=round(1e6 * t_out, 1), output_avemax_bytes=obytes,
+               output_avemax_gbs=round(obytes / t_out / 1e9, 1))
+    log(f"runoff interpolation and output accumulation (host-timed calls incl. stream sync): {out}")
     return out
 
 
