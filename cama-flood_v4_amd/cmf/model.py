@@ -205,6 +205,21 @@ class CMFModel:
         """Interpolate a runoff field that is already in device memory."""
         self._check(self.lib.cmf_hip_interp_runoff(self.h, C.c_void_p(device_ptr)))
 
+    # sea-level boundary (CMF_BOUNDARY_UPDATE) ----------------------------
+    def set_dwnelv(self, dwnelv: np.ndarray) -> None:
+        a = _f64(dwnelv)
+        if a.size != self.map.n:
+            raise CMFError("set_dwnelv: NSEQALL values expected")
+        self._check(self.lib.cmf_hip_set_dwnelv(self.h, a.ctypes.data))
+
+    def update_boundary(self, meansl: Optional[np.ndarray] = None, sealev: Optional[np.ndarray] = None) -> None:
+        """D2DWNELV = D2ELEVTN (+ meansl) (+ sealev) on the device."""
+        arrs = [None if a is None else _f64(a) for a in (meansl, sealev)]
+        for a in arrs:
+            if a is not None and a.size != self.map.n:
+                raise CMFError("update_boundary: NSEQALL values expected")
+        self._check(self.lib.cmf_hip_update_boundary(self.h, *(None if a is None else a.ctypes.data for a in arrs)))
+
     # output-period diagnostics (CMF_DIAG_*_OUTPUT) ----------------------
     def diag_reset_output(self) -> None:
         self._check(self.lib.cmf_hip_diag_reset_output(self.h))

@@ -66,6 +66,8 @@ struct cmf_hip_ctx {
     double *d_rbase = nullptr, *d_rfac = nullptr;   // staging of cmf_hip_set_runoff_scaled
     RoffMap roff{};                                  // input matrix of cmf_hip_set_inpmat
     OutState out{};                                  // output-period accumulators
+    double* dwnelv_dev = nullptr;                    // D2DWNELV in the device map (boundary updates)
+    double *d_meansl = nullptr, *d_sealev = nullptr; // staging of cmf_hip_update_boundary
     double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
     size_t grid_cap = 0;
@@ -958,6 +960,7 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     {
         std::vector<double> v = to_dev(h, mp->dwnelv, (size_t)n);
         TRY(put_raw(h, &pd, v.data(), (size_t)n)); M.dwnelv = pd;
+        h->dwnelv_dev = pd;
         std::vector<int32_t> m2 = to_dev(h, mp->i2mask, (size_t)n);
         TRY(put_raw(h, &p32, m2.data(), (size_t)n)); M.i2mask = p32;
     }
@@ -1194,6 +1197,41 @@ int cmf_hip_get_output(cmf_handle h, const cmf_output* o) {
         {o->storge_max, h->out.storge_max}};
     for (const auto& c : cell) TRY(get<double>(h, c.first, c.second, K_CELL));
     if (o->pthflw && npth_slots(h) > 0) TRY(get<double>(h, o->pthflw, h->out.pthflw, K_PTHL));
+    return 0;
+}
+
+int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv) {
+    if (!h || !dwnelv) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    const size_t n = (size_t)h->d.nseqall;
+    std::vector<double> v = to_dev(h, dwnelv, n);
+    HIPCHK(hipMemcpyAsync(h->dwnelv_dev, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    const size_t n = (size_t)h->d.nseqall;
+    const double *dm = nullptr, *dsl = nullptr;
+    if (meansl) {
+        if (!h->d_meansl) TRY(dalloc(h, &h->d_meansl, n));
+        std::vector<double> v = to_dev(h, meansl, n);
+        HIPCHK(hipMemcpyAsync(h->d_meansl, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dm = h->d_meansl;
+    }
+    if (sealev) {
+        if (!h->d_sealev) TRY(dalloc(h, &h->d_sealev, n));
+        std::vector<double> v = to_dev(h, sealev, n);
+        HIPCHK(hipMemcpyAsync(h->d_sealev, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dsl = h->d_sealev;
+    }
+    launch_boundary(h->M.n, h->M.elevtn, dm, dsl, h->dwnelv_dev, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 

@@ -933,6 +933,18 @@ __global__ void k_out_reset(OutState O, int ns, int npth_slots) {
     if (s < npth_slots) O.pthflw[s] = 0.0;
 }
 
+// CMF_BOUNDARY_UPDATE (src/cmf_ctrl_boundary_mod.F90:267-305): D2DWNELV = D2ELEVTN
+// (+ D2MEANSL with LMEANSL) (+ D2SEALEV with LSEALEV), in that order; NULL = off
+__global__ void k_boundary(int n, const float* elevtn, const double* meansl, const double* sealev,
+                           double* dwnelv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = (double)elevtn[i];
+    if (meansl) d = d + meansl[i];
+    if (sealev) d = d + sealev[i];
+    dwnelv[i] = d;
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers: MB (several instances) is chosen from the map, the block
 // size from the launch configuration
@@ -1045,6 +1057,11 @@ void launch_out_getave(const OutState& O, int ns, int npth_slots, double nadd, i
 void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st) {
     const int n = ns > npth_slots ? ns : npth_slots;
     hipLaunchKernelGGL(k_out_reset, dim3(cdiv(n, 256)), dim3(256), 0, st, O, ns, npth_slots);
+}
+
+void launch_boundary(int n, const float* elevtn, const double* meansl, const double* sealev, double* dwnelv,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_boundary, dim3(cdiv(n, 256)), dim3(256), 0, st, n, elevtn, meansl, sealev, dwnelv);
 }
 
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {

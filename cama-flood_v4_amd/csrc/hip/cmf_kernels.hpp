@@ -37,6 +37,8 @@ void launch_out_avemax(const DevMap& M, const DevState<R>& S, const OutState& O,
                        int npth_slots, hipStream_t st);
 void launch_out_getave(const OutState& O, int ns, int npth_slots, double nadd, int ldamout, hipStream_t st);
 void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st);
+void launch_boundary(int n, const float* elevtn, const double* meansl, const double* sealev, double* dwnelv,
+                     hipStream_t st);
 // reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st);
 

@@ -180,6 +180,14 @@ int cmf_hip_diag_avemax_output(cmf_handle h, double dt);
 int cmf_hip_diag_getave_output(cmf_handle h, double* nadd_out);
 int cmf_hip_get_output(cmf_handle h, const cmf_output* out);
 
+/* Sea-level boundary (src/cmf_ctrl_boundary_mod.F90:267-305). cmf_hip_set_dwnelv
+ * uploads D2DWNELV as the host's CMF_BOUNDARY_UPDATE left it (call before every
+ * advance when LSEALEV is on: the map upload only carries the initial value).
+ * cmf_hip_update_boundary forms D2DWNELV = D2ELEVTN (+ D2MEANSL) (+ D2SEALEV) on
+ * the device, NULL switching a term off (LMEANSL / LSEALEV false). */
+int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv);
+int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev);
+
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);
 
