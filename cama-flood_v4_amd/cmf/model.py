@@ -205,6 +205,22 @@ class CMFModel:
         """Interpolate a runoff field that is already in device memory."""
         self._check(self.lib.cmf_hip_interp_runoff(self.h, C.c_void_p(device_ptr)))
 
+    # binary restart from / into device state (WRTE_REST_BIN / READ_REST_BIN) --
+    def _grid(self):
+        m = self.map
+        return (m.nx, m.ny, np.ascontiguousarray(m.seqx, np.int32), np.ascontiguousarray(m.seqy, np.int32))
+
+    def restart_write(self, path: str, lrestdbl: bool = True, lstoonly: bool = False, dmis: float = 1e20,
+                      rmis: float = float(np.float32(1e20))) -> None:
+        nx, ny, sx, sy = self._grid()
+        self._check(self.lib.cmf_hip_restart_write(self.h, path.encode(), nx, ny, sx.ctypes.data, sy.ctypes.data,
+                                                   int(lrestdbl), int(lstoonly), float(dmis), float(rmis)))
+
+    def restart_read(self, path: str, lrestdbl: bool = True, lstoonly: bool = False) -> None:
+        nx, ny, sx, sy = self._grid()
+        self._check(self.lib.cmf_hip_restart_read(self.h, path.encode(), nx, ny, sx.ctypes.data, sy.ctypes.data,
+                                                  int(lrestdbl), int(lstoonly)))
+
     # sea-level boundary (CMF_BOUNDARY_UPDATE) ----------------------------
     def set_dwnelv(self, dwnelv: np.ndarray) -> None:
         a = _f64(dwnelv)

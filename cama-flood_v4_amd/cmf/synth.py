@@ -326,7 +326,8 @@ def make_runoff_grid(nxin: int, nyin: int, nrec: int, seed: int, mean_mm_day: fl
 def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt: float,
                            ifrq_inp: int, ndays: int, lpthout=False,
                            llevee=False, ldamout=False, ladpstp=True, lfldout=True,
-                           pmanriv=0.03, pmanfld=0.10, pcadp=0.7, pdstmth=10000.0) -> None:
+                           pmanriv=0.03, pmanfld=0.10, pcadp=0.7, pdstmth=10000.0,
+                           restart: bool = False) -> None:
     """Write the map files, tables, runoff records and input_cmf.nam that the
     reference needs (formats of src/cmf_ctrl_maps_mod.F90 READ_MAP_BIN,
     READ_TOPO_BIN, READ_BIFPARAM; src/cmf_ctrl_damout_mod.F90:141-179;
@@ -373,6 +374,9 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
     t0 = _dt.datetime(2001, 1, 1)
     t1 = t0 + _dt.timedelta(days=ndays)
     L = lambda b: ".TRUE." if b else ".FALSE."
+    # restart at the end of the run (IFRQ_RST = 0), binary FP64, restart<YYYYMMDDHH>.bin
+    restart_nml = ('IFRQ_RST = 0\nCRESTDIR = "./"\nCVNREST = "restart"\nLRESTCDF = .FALSE.\n'
+                   'LRESTDBL = .TRUE.') if restart else "IFRQ_RST = -1"
     nml = f"""&NRUNVER
 LADPSTP  = {L(ladpstp)}
 LFLDOUT  = {L(lfldout)}
@@ -422,7 +426,7 @@ LINPCDF = .FALSE.
 LINTERP = .FALSE.
 /
 &NRESTART
-IFRQ_RST = -1
+{restart_nml}
 /
 &NDAMOUT
 CDAMFILE = "dam_params.csv"

@@ -1235,6 +1235,128 @@ int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* se
     return 0;
 }
 
+namespace {
+// the restart records of the handle's options, in file order (restart:418-447)
+std::vector<const char*> restart_fields(const cmf_hip_ctx* h, bool lstoonly) {
+    std::vector<const char*> f{"rivsto", "fldsto"};
+    if (!lstoonly) { f.push_back("rivout_pre"); f.push_back("fldout_pre"); f.push_back("rivdph_pre"); f.push_back("fldsto_pre"); }
+    if (h->o.ldamout) f.push_back("damsto");
+    if (h->o.llevee) f.push_back("levsto");
+    return f;
+}
+
+double** state_slot(cmf_state* st, const char* k) {
+    if (!std::strcmp(k, "rivsto")) return &st->rivsto;
+    if (!std::strcmp(k, "fldsto")) return &st->fldsto;
+    if (!std::strcmp(k, "rivout_pre")) return &st->rivout_pre;
+    if (!std::strcmp(k, "fldout_pre")) return &st->fldout_pre;
+    if (!std::strcmp(k, "rivdph_pre")) return &st->rivdph_pre;
+    if (!std::strcmp(k, "fldsto_pre")) return &st->fldsto_pre;
+    if (!std::strcmp(k, "damsto")) return &st->damsto;
+    return &st->levsto;
+}
+
+int check_grid(const cmf_hip_ctx* h, int32_t nx, int32_t ny, const int32_t* seqx, const int32_t* seqy) {
+    if (nx <= 0 || ny <= 0 || !seqx || !seqy) return fail(CMF_EINVAL, "bad NX/NY or NULL I1SEQX/I1SEQY");
+    for (int i = 0; i < h->d.nseqall; ++i)
+        if (seqx[i] < 1 || seqx[i] > nx || seqy[i] < 1 || seqy[i] > ny)
+            return fail(CMF_EINVAL, "cell %d at (%d, %d) outside the %d x %d grid", i + 1, seqx[i], seqy[i], nx, ny);
+    return 0;
+}
+}  // namespace
+
+int cmf_hip_restart_write(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
+                          const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly, double dmis, double rmis) {
+    if (!h || !file) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->ninst != 1) return fail(CMF_EINVAL, "restart files hold one model: not an ensemble handle");
+    TRY(check_grid(h, nx, ny, seqx, seqy));
+    const size_t n = (size_t)h->d.nseqall, ng = (size_t)nx * (size_t)ny;
+    const std::vector<const char*> fields = restart_fields(h, lstoonly != 0);
+    std::vector<std::vector<double>> vec(fields.size(), std::vector<double>(n));
+    cmf_state st{};
+    for (size_t k = 0; k < fields.size(); ++k) *state_slot(&st, fields[k]) = vec[k].data();
+    std::vector<double> pth;
+    if (h->o.lpthout) { pth.resize((size_t)h->d.npthout * h->d.npthlev); st.pthflw_pre = pth.data(); }
+    TRY(cmf_hip_get_state(h, &st));
+    FILE* f = std::fopen(file, "wb");
+    if (!f) return fail(CMF_EIO, "cannot open %s for writing", file);
+    bool ok = true;
+    std::vector<double> m8;
+    std::vector<float> m4;
+    for (size_t k = 0; k < fields.size() && ok; ++k) {                // WRTE_BIN_MAP (restart:484-517)
+        if (lrestdbl) {
+            m8.assign(ng, dmis);
+            for (size_t i = 0; i < n; ++i) m8[(size_t)(seqy[i] - 1) * nx + (seqx[i] - 1)] = vec[k][i];
+            ok = std::fwrite(m8.data(), sizeof(double), ng, f) == ng;
+        } else {
+            m4.assign(ng, (float)rmis);
+            for (size_t i = 0; i < n; ++i) m4[(size_t)(seqy[i] - 1) * nx + (seqx[i] - 1)] = (float)vec[k][i];
+            ok = std::fwrite(m4.data(), sizeof(float), ng, f) == ng;
+        }
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(CMF_EIO, "writing %s failed", file);
+    if (h->o.lpthout) {                                               // restart:450-475
+        const std::string pf = std::string(file) + ".pth";
+        FILE* g = std::fopen(pf.c_str(), "wb");
+        if (!g) return fail(CMF_EIO, "cannot open %s for writing", pf.c_str());
+        if (lrestdbl) ok = std::fwrite(pth.data(), sizeof(double), pth.size(), g) == pth.size();
+        else {
+            std::vector<float> p4(pth.begin(), pth.end());
+            ok = std::fwrite(p4.data(), sizeof(float), p4.size(), g) == p4.size();
+        }
+        ok = (std::fclose(g) == 0) && ok;
+        if (!ok) return fail(CMF_EIO, "writing %s failed", pf.c_str());
+    }
+    return 0;
+}
+
+int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
+                         const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly) {
+    if (!h || !file) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->ninst != 1) return fail(CMF_EINVAL, "restart files hold one model: not an ensemble handle");
+    TRY(check_grid(h, nx, ny, seqx, seqy));
+    const size_t n = (size_t)h->d.nseqall, ng = (size_t)nx * (size_t)ny;
+    const std::vector<const char*> fields = restart_fields(h, lstoonly != 0);
+    std::vector<std::vector<double>> vec(fields.size(), std::vector<double>(n));
+    FILE* f = std::fopen(file, "rb");
+    if (!f) return fail(CMF_EIO, "cannot open %s", file);
+    bool ok = true;
+    std::vector<double> m8(lrestdbl ? ng : 0);
+    std::vector<float> m4(lrestdbl ? 0 : ng);
+    for (size_t k = 0; k < fields.size() && ok; ++k) {
+        ok = lrestdbl ? std::fread(m8.data(), sizeof(double), ng, f) == ng
+                      : std::fread(m4.data(), sizeof(float), ng, f) == ng;
+        for (size_t i = 0; ok && i < n; ++i) {
+            const size_t g = (size_t)(seqy[i] - 1) * nx + (seqx[i] - 1);
+            vec[k][i] = lrestdbl ? m8[g] : (double)m4[g];
+        }
+    }
+    std::fclose(f);
+    if (!ok) return fail(CMF_EIO, "%s is shorter than %zu records of %d x %d", file, fields.size(), nx, ny);
+    cmf_state st{};
+    for (size_t k = 0; k < fields.size(); ++k) *state_slot(&st, fields[k]) = vec[k].data();
+    std::vector<double> pth;
+    if (h->o.lpthout) {
+        pth.resize((size_t)h->d.npthout * h->d.npthlev);
+        const std::string pf = std::string(file) + ".pth";
+        FILE* g = std::fopen(pf.c_str(), "rb");
+        if (!g) return fail(CMF_EIO, "cannot open %s", pf.c_str());
+        if (lrestdbl) ok = std::fread(pth.data(), sizeof(double), pth.size(), g) == pth.size();
+        else {
+            std::vector<float> p4(pth.size());
+            ok = std::fread(p4.data(), sizeof(float), p4.size(), g) == p4.size();
+            for (size_t k = 0; k < p4.size(); ++k) pth[k] = (double)p4[k];
+        }
+        std::fclose(g);
+        if (!ok) return fail(CMF_EIO, "%s is too short", pf.c_str());
+        st.pthflw_pre = pth.data();
+    }
+    return cmf_hip_set_state(h, &st);
+}
+
 int cmf_hip_physics_fldstg(cmf_handle h) {
     if (!h) return fail(CMF_EINVAL, "NULL handle");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");

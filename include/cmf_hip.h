@@ -34,6 +34,7 @@ extern "C" {
 #define CMF_ESTATE     -3   /* call out of order (e.g. advance before set_map) */
 #define CMF_EMAP       -4   /* map tables fail validation */
 #define CMF_ENOMEM     -5
+#define CMF_EIO        -6   /* file could not be opened, read or written */
 
 /* Device arithmetic of the flux/stage variables (JPRB on the device). Storages,
  * the storage update, the stage inversion and the budget sums are FP64 in both. */
@@ -187,6 +188,23 @@ int cmf_hip_get_output(cmf_handle h, const cmf_output* out);
  * the device, NULL switching a term off (LMEANSL / LSEALEV false). */
 int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv);
 int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev);
+
+/* Binary restart straight from / into device state: WRTE_REST_BIN and
+ * READ_REST_BIN of CMF_RESTART_WRITE / CMF_RESTART_INIT
+ * (src/cmf_ctrl_restart_mod.F90:389-482, 484-517 and its reader), single-model
+ * handles. Records of NX*NY values (float64 with lrestdbl, else float32), x
+ * fastest, cells placed by seqx/seqy (1-based I1SEQX/I1SEQY, NSEQ order) and the
+ * rest of the grid DMIS (float64) or RMIS (float32): P2RIVSTO, P2FLDSTO, then,
+ * unless lstoonly, D2RIVOUT_PRE, D2FLDOUT_PRE, D2RIVDPH_PRE, D2FLDSTO_PRE, then
+ * P2DAMSTO (LDAMOUT) and P2LEVSTO (LLEVEE); with LPTHOUT, D1PTHFLW_PRE
+ * (NPTHOUT x NPTHLEV) in the file `file`.pth. The file name is the caller's
+ * (the reference builds CRESTDIR//CVNREST//YYYYMMDDHH//CSUFBIN). The reader
+ * loads the same records into the device state; the caller then runs
+ * cmf_hip_physics_fldstg as the reference's restart initialisation does. */
+int cmf_hip_restart_write(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
+                          const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly, double dmis, double rmis);
+int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
+                         const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly);
 
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);

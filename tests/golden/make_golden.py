@@ -9,7 +9,8 @@ outputs reduced to:
   * map tables after initialisation: SHA-1 of each array (float64/int32 bytes);
   * per outer step: NT, SHA-1 of every state array, the 10 budget sums;
   * the full final state of the main prognostic/diagnostic fields.
-Usage: python tests/golden/make_golden.py [case ...]
+and, for the cases of RESTART_CASES, the reference's end-of-run binary restart file.
+Usage: python tests/golden/make_golden.py [case ... | case_restart ...]
 """
 import hashlib
 import json
@@ -132,6 +133,38 @@ def make(name):
     print(f"{name}: n={mp['n']} steps={len(states) - 1} sum NT={sum(nts)}")
 
 
+RESTART_CASES = ("g_plain", "g_allwet")
+
+
+def make_restart(name):
+    """The reference's own end-of-run binary restart (CMF_RESTART_WRITE, FP64
+    records) of a golden case, as raw bytes."""
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = GOLDEN[name]
+    net, ro = load_network(os.path.join(HERE, f"{name}_inputs.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, restart=True)
+        r = subprocess.run([REF, "runoff.bin", "NONE", "0"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        files = sorted(f for f in os.listdir(d) if f.startswith("restart") and f.endswith(".bin"))
+        if len(files) != 1:
+            raise RuntimeError(f"expected one restart file, found {files}")
+        rec = dict(name=np.frombuffer(files[0].encode(), np.uint8),
+                   bin=np.fromfile(os.path.join(d, files[0]), np.uint8))
+        if lpth:
+            rec["pth"] = np.fromfile(os.path.join(d, files[0] + ".pth"), np.uint8)
+    np.savez_compressed(os.path.join(HERE, f"{name}_restart.npz"), **rec)
+    print(f"{name}: restart {files[0]} ({rec['bin'].size} bytes)")
+
+
 if __name__ == "__main__":
-    for name in (sys.argv[1:] or list(GOLDEN)):
-        make(name)
+    names = sys.argv[1:] or list(GOLDEN)
+    for name in names:
+        if name.endswith("_restart"):
+            make_restart(name[:-len("_restart")])
+        else:
+            make(name)
+    if not sys.argv[1:]:
+        for name in RESTART_CASES:
+            make_restart(name)
