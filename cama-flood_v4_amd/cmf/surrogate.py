@@ -12,9 +12,9 @@ lakes and other slow storage (what a cold start takes years to fill) within
 ~0.1% of their equilibrium; what it misses is the nonlinear seasonal response of
 the cells with weeks to months of memory. The network learns that residual per
 cell, log(equilibrium storage / prior storage), from catchment attributes, the
-runoff climatology and the prior itself, on the cells that remember less than a
-year of the forcing (drawn in proportion to the storage they hold), and the warm
-start is the prior times the predicted factor on those cells.
+runoff climatology and the prior itself, on the cells that remember less than
+60 days of the forcing (drawn in proportion to the storage they hold), and the
+warm start is the prior times the predicted factor on those cells.
 
 The network (24 -> 256 -> 256 -> 256 -> 2, ReLU) is trained and evaluated by the
 project's HIP kernels (include/cmf_mlp.h: BF16 matrix-core products, FP32
@@ -64,7 +64,7 @@ WINDOWS = (1, 7, 30, 90)          # days before the year start, features 7-10
 NHARM = 6                         # runoff harmonics of the linear periodic response
 Q_FLOOR = 1e-3                    # flow ratios are clipped to >= this
 STO_FLOOR = 1.0                   # m3, storages are floored at this in the log targets
-FAST_DAYS = 365.0                 # the network corrects cells that remember less of the forcing
+FAST_DAYS = 60.0                  # the network corrects cells that remember less of the forcing
 
 
 class SurrogateError(RuntimeError):
