@@ -82,8 +82,11 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     const Slot x = cell_slot<MB>(M, s);
     const int i = x.c, m = x.m;
     // every load that does not depend on the downstream index is issued first,
-    // together with the instance's step (an idle instance only wastes the loads)
+    // together with the instance's step. Ensemble slots of an instance that has
+    // finished its substeps leave before any load (whole wavefronts idle once its
+    // NT is reached); a single instance is active on every substep.
     const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (MB && !I.active) return;
     const int nxt = M.next[i];
     const R rivdph = S.rivdph[s], rivdph_pre = S.rivdph_pre[s];
     const R qr_pre = S.rivout[s];                                   // D2RIVOUT_PRE
@@ -395,6 +398,7 @@ __device__ __forceinline__ void rate_cell(const DevMap& M, const DevState<R>& S,
     const Slot x = cell_slot<MB>(M, s);
     const int i = x.c, m = x.m;
     const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (MB && !I.active) return;
     const R qr = S.qr_u[s], qf = S.qf_u[s];
     const int k0 = M.up_ptr[i], k1 = M.up_ptr[i + 1];
     // RIVSTO + FLDSTO, issued ahead of the gathers. Without levees STORGE holds
@@ -446,6 +450,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const Slot x = cell_slot<MB>(M, s);
     const int i = x.c, m = x.m;
     const InstView I = inst_view(S.inst, x.inst, A.it);
+    if (MB && !I.active) return;
     const R zero = R(0);
     // every load that does not depend on a gathered index is issued first, so it
     // is in flight while the upstream and downstream gathers resolve
