@@ -228,6 +228,13 @@ class CMFModel:
             raise CMFError("set_dwnelv: NSEQALL values expected")
         self._check(self.lib.cmf_hip_set_dwnelv(self.h, a.ctypes.data))
 
+    def set_upsinf(self, upsinf: np.ndarray) -> None:
+        """LUPSINF: upstream inflow D2UPSINF [m3/s] added to the river in STONXT."""
+        a = _f64(upsinf)
+        if a.size != self.members * self.map.n:
+            raise CMFError(f"set_upsinf: {self.members * self.map.n} values expected")
+        self._check(self.lib.cmf_hip_set_upsinf(self.h, a.ctypes.data))
+
     def update_boundary(self, meansl: Optional[np.ndarray] = None, sealev: Optional[np.ndarray] = None) -> None:
         """D2DWNELV = D2ELEVTN (+ meansl) (+ sealev) on the device."""
         arrs = [None if a is None else _f64(a) for a in (meansl, sealev)]

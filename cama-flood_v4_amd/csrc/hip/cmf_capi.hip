@@ -67,6 +67,7 @@ struct cmf_hip_ctx {
     RoffMap roff{};                                  // input matrix of cmf_hip_set_inpmat
     OutState out{};                                  // output-period accumulators
     double* dwnelv_dev = nullptr;                    // D2DWNELV in the device map (boundary updates)
+    int upsinf_on = 0;                               // LUPSINF switched on by cmf_hip_set_upsinf
     double *d_meansl = nullptr, *d_sealev = nullptr; // staging of cmf_hip_update_boundary
     double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
@@ -420,6 +421,7 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.lfldout = h->o.lfldout; A.lpthout = h->o.lpthout; A.ldamout = h->o.ldamout; A.llevee = h->o.llevee;
     A.budget = h->ninst == 1 && h->nmem == 1 && h->ndom == 1;   // the kernels keep budgets only then
     A.xcd = h->tune.xcd;
+    A.lupsinf = h->upsinf_on;
     return A;
 }
 
@@ -1207,6 +1209,26 @@ int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv) {
     std::vector<double> v = to_dev(h, dwnelv, n);
     HIPCHK(hipMemcpyAsync(h->dwnelv_dev, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf) {
+    if (!h || !upsinf) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    const size_t ns = (size_t)h->M.ns;
+    if (h->o.precision == CMF_PREC_FP64) {
+        if (!h->S64.upsinf) TRY(dalloc(h, &h->S64.upsinf, ns));
+        TRY(put<double>(h, h->S64.upsinf, upsinf, K_CELL));
+    } else {
+        if (!h->S32.upsinf) TRY(dalloc(h, &h->S32.upsinf, ns));
+        TRY(put<float>(h, h->S32.upsinf, upsinf, K_CELL));
+    }
+    if (!h->upsinf_on) {
+        h->upsinf_on = 1;
+        // captured substep graphs hold the old arguments
+        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+        h->graphs.clear();
+    }
     return 0;
 }
 

@@ -125,6 +125,7 @@ struct DevState {
     double *rivinf, *fldinf, *pthout;
     R* rate;
     R* runoff;
+    R* upsinf;                                 // LUPSINF: D2UPSINF [m3/s] (NULL when off)
     R *pthflw, *pthflw_u, *pthflwsum, *pthflwsum_u;   // [npthlev][nsp], [nsp]
     R *rivout_avg, *fldout_avg, *outflw_avg, *rivvel_avg, *pthout_avg, *runoff_avg;
     R *outflw_max, *rivdph_max, *storge_max, *daminf_avg;
@@ -143,6 +144,7 @@ struct StepArgs {
     int32_t it;                                // substep index within the outer step
     int32_t budget;                            // write the per-tile water-budget sums
     int32_t xcd;                               // XCD-contiguous tile order
+    int32_t lupsinf;                           // add S.upsinf to the river in STONXT
 };
 
 // Cell / path slot -> static index, member and instance. MB = false is the

@@ -459,6 +459,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const int nxt = M.next[i];
     const int k0 = M.up_ptr[i], k1 = M.up_ptr[i + 1];
     const R runoff = S.runoff[s];
+    const R upsinf = A.lupsinf ? S.upsinf[s] : R(0);
     const double rs0 = S.rivsto[s], fs0 = S.fldsto[s];
     const R fldfrc_old = S.fldfrc[s];
     const double levsto = A.llevee ? S.levsto[s] : 0.0;
@@ -550,8 +551,9 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     if (fs < 0.0) { rs = vmax(rs + fs, 0.0); fs = 0.0; }
     g[G_STONXT] = rs + fs;
     const R outflw = qrl + qfl;
-    const R drivrof = (runoff * (R(1) - fldfrc_old)) * R(dt);
+    R drivrof = (runoff * (R(1) - fldfrc_old)) * R(dt);
     const R dfldrof = (runoff * fldfrc_old) * R(dt);
+    if (A.lupsinf) drivrof = drivrof + upsinf * R(dt);             // stonxt:81
     rs = rs + (double)drivrof;
     fs = fs + (double)dfldrof;
     double st = rs + fs;

@@ -187,6 +187,11 @@ int cmf_hip_get_output(cmf_handle h, const cmf_output* out);
  * cmf_hip_update_boundary forms D2DWNELV = D2ELEVTN (+ D2MEANSL) (+ D2SEALEV) on
  * the device, NULL switching a term off (LMEANSL / LSEALEV false). */
 int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv);
+/* LUPSINF: upstream inflow D2UPSINF [m3/s] (as CMF_UPSINF_UPDATE leaves it, per
+ * cell, members planes for ensembles), added to the river storage in the storage
+ * update (src/cmf_calc_stonxt_mod.F90:81). The first call switches it on for
+ * the handle; call again whenever D2UPSINF changes. */
+int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf);
 int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev);
 
 /* Binary restart straight from / into device state: WRTE_REST_BIN and

@@ -681,8 +681,9 @@ void cmfo_calc_stonxt(cmfo_model* m, double dt)
         if (fs < 0.0) { rs = DMAX(rs + fs, 0.0); fs = 0.0; }
         g_nxt = (g_nxt + rs) + fs;
         m->outflw[i] = m->rivout[i] + m->fldout[i];
-        const double drivrof = (m->runoff[i] * (1.0 - m->fldfrc[i])) * dt;
+        double drivrof = (m->runoff[i] * (1.0 - m->fldfrc[i])) * dt;
         const double dfldrof = (m->runoff[i] * m->fldfrc[i]) * dt;
+        if (m->upsinf) drivrof = drivrof + m->upsinf[i] * dt;               /* stonxt:81 */
         rs = rs + drivrof;
         fs = fs + dfldrof;
         m->rivsto[i] = rs;

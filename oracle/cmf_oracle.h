@@ -68,6 +68,9 @@ typedef struct cmfo_model {
     /* work arrays, each of length n */
     double *w_sfcelv_pre, *w_dwnelv_pre, *w_flddph_pre, *w_stoout, *w_rate;
     double *w_rivinf, *w_fldinf, *w_pthout;
+    /* LUPSINF: upstream inflow D2UPSINF [m3/s] added to the river in STONXT
+     * (src/cmf_calc_stonxt_mod.F90:81); NULL = off */
+    const double* upsinf;
 } cmfo_model;
 
 /* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */

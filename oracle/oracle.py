@@ -35,7 +35,7 @@ _WORK = "w_sfcelv_pre w_dwnelv_pre w_flddph_pre w_stoout w_rate w_rivinf w_fldin
 
 
 class _Model(C.Structure):
-    _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK]
+    _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK] + [("upsinf", _P)]
 
 
 MAP_KEYS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax "
@@ -168,6 +168,8 @@ class OracleModel:
                 self.st[k] = np.ascontiguousarray(state.get(k, np.zeros(m.npth)), np.float64).copy()
         for k, a in self.st.items():
             setattr(s, k, a.ctypes.data)
+        if state.get("upsinf") is not None:                 # LUPSINF (stonxt:81)
+            self._bind("upsinf", np.ascontiguousarray(state["upsinf"], np.float64))
         self.work = {w: np.zeros(n) for w in _WORK}
         for w, a in self.work.items():
             setattr(s, w, a.ctypes.data)

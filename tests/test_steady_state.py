@@ -150,3 +150,34 @@ def test_storage_at_level_inverts_the_stage():
     assert _native.host().cmfh_storage_at_level(C.byref(ch), ptr(ss["sfcelv"]), ptr(riv), ptr(fld)) == 0
     assert np.allclose(riv, ss["rivsto"], rtol=1e-12, atol=1e-6)
     assert np.allclose(fld, ss["fldsto"], rtol=1e-12, atol=1e-6)
+
+
+def test_oracle_upstream_inflow_adds_to_the_river():
+    """LUPSINF (src/cmf_calc_stonxt_mod.F90:81): over one substep the inflow
+    D2UPSINF*DT lands in the river storage and nothing else moves: both runs see
+    the same outflows (computed before STONXT), so the total storage after the
+    substep differs by exactly the added volume (to rounding)."""
+    from oracle.oracle import OracleModel
+    m, r = _basin(12, 40.0)
+    ss = spinup.steady_state(m, spinup.accumulate(m, r))
+    st = maps.cold_start(m)
+    st["rivsto"], st["fldsto"] = ss["rivsto"].copy(), ss["fldsto"].copy()
+    base = OracleModel(m, st)
+    base.physics_fldstg()
+    st1 = {k: v.copy() for k, v in base.st.items()}
+    st1["runoff"] = r.copy()
+    ups = np.zeros(m.n)
+    k = np.random.default_rng(0).choice(m.n, 20, replace=False)
+    ups[k] = 50.0
+    a = OracleModel(m, dict(st1))
+    b = OracleModel(m, dict(st1, upsinf=ups))
+    dt = 600.0
+    na = a.physics_advance(dt, nt_cap=1)
+    nb = b.physics_advance(dt, nt_cap=1)
+    assert na == nb
+    sa = a.st["rivsto"] + a.st["fldsto"]
+    sb = b.st["rivsto"] + b.st["fldsto"]
+    dts = dt / na
+    added = ups * dts
+    assert np.allclose(sb - sa, added, rtol=0, atol=1e-4)          # m3, against 3e4 m3 added
+    assert np.all(np.abs(sb - sa)[ups == 0] <= 1e-4)
