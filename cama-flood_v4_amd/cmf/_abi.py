@@ -97,6 +97,7 @@ HIP_FUNCS = {
     "cmf_hip_restart_write": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32, C.c_double,
                                   C.c_double]),
     "cmf_hip_restart_read": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32]),
+    "cmf_hip_output_write": (I, [H, C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_double]),
     "cmf_hip_update_boundary": (I, [H, P, P]),
     "cmf_hip_diag_reset_output": (I, [H]),
     "cmf_hip_diag_avemax_output": (I, [H, C.c_double]),

@@ -216,6 +216,13 @@ class CMFModel:
         self._check(self.lib.cmf_hip_restart_write(self.h, path.encode(), nx, ny, sx.ctypes.data, sy.ctypes.data,
                                                    int(lrestdbl), int(lstoonly), float(dmis), float(rmis)))
 
+    def output_write(self, var: str, path: str, loutvec: bool = False,
+                     rmis: float = float(np.float32(1e20))) -> None:
+        """Append one binary output record of `var` (CMF_OUTPUT_WRITE) to `path`."""
+        nx, ny, sx, sy = self._grid()
+        self._check(self.lib.cmf_hip_output_write(self.h, var.encode(), path.encode(), nx, ny, sx.ctypes.data,
+                                                  sy.ctypes.data, int(loutvec), float(rmis)))
+
     def restart_read(self, path: str, lrestdbl: bool = True, lstoonly: bool = False) -> None:
         nx, ny, sx, sy = self._grid()
         self._check(self.lib.cmf_hip_restart_read(self.h, path.encode(), nx, ny, sx.ctypes.data, sy.ctypes.data,

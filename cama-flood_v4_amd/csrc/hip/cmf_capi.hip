@@ -1334,6 +1334,72 @@ int cmf_hip_restart_write(cmf_handle h, const char* file, int32_t nx, int32_t ny
     return 0;
 }
 
+int cmf_hip_output_write(cmf_handle h, const char* var, const char* file, int32_t nx, int32_t ny,
+                         const int32_t* seqx, const int32_t* seqy, int32_t loutvec, double rmis) {
+    if (!h || !var || !file) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->ninst != 1) return fail(CMF_EINVAL, "output files hold one model: not an ensemble handle");
+    TRY(check_grid(h, nx, ny, seqx, seqy));
+    const std::string v(var);
+    const size_t n = (size_t)h->d.nseqall;
+    std::vector<double> vec;
+    // output_mod:489-579: which array a name writes
+    const bool avg = v == "rivout" || v == "fldout" || v == "outflw" || v == "totout" || v == "rivvel" ||
+                     v == "pthout" || v == "runoff" || v == "rofsfc" || v == "daminf" || v == "maxflw" ||
+                     v == "maxdph" || v == "maxsto" || v == "pthflw";
+    if ((v == "pthout" || v == "pthflw") && !h->o.lpthout) return 0;
+    if ((v == "damsto" || v == "daminf") && !h->o.ldamout) return 0;
+    if ((v == "levsto" || v == "levdph") && !h->o.llevee) return 0;
+    if (avg) {
+        if (!h->out.rivout) return fail(CMF_ESTATE, "%s is an output-period value: no output period yet", var);
+        cmf_output o{};
+        vec.resize(v == "pthflw" ? (size_t)h->d.npthout * h->d.npthlev : n);
+        double* p = vec.data();
+        if (v == "rivout") o.rivout = p;
+        else if (v == "fldout") o.fldout = p;
+        else if (v == "outflw" || v == "totout") o.outflw = p;
+        else if (v == "rivvel") o.rivvel = p;
+        else if (v == "pthout") o.pthout = p;
+        else if (v == "runoff" || v == "rofsfc") o.runoff = p;
+        else if (v == "daminf") o.daminf = p;
+        else if (v == "maxflw") o.outflw_max = p;
+        else if (v == "maxdph") o.rivdph_max = p;
+        else if (v == "maxsto") o.storge_max = p;
+        else o.pthflw = p;
+        TRY(cmf_hip_get_output(h, &o));
+    } else {
+        cmf_state st{};
+        vec.resize(n);
+        double* p = vec.data();
+        if (v == "rivsto") st.rivsto = p;
+        else if (v == "fldsto") st.fldsto = p;
+        else if (v == "rivdph") st.rivdph = p;
+        else if (v == "flddph") st.flddph = p;
+        else if (v == "fldfrc") st.fldfrc = p;
+        else if (v == "fldare") st.fldare = p;
+        else if (v == "sfcelv") st.sfcelv = p;
+        else if (v == "storge" || v == "totsto") st.storge = p;
+        else if (v == "damsto") st.damsto = p;
+        else if (v == "levsto") st.levsto = p;
+        else if (v == "levdph") st.levdph = p;
+        else return fail(CMF_EINVAL, "unknown output variable %s", var);
+        TRY(cmf_hip_get_state(h, &st));
+    }
+    std::vector<float> rec;
+    if (v == "pthflw" || loutvec) {                                   // WRTE_OUTPTH / WRTE_OUTVEC
+        rec.assign(vec.begin(), vec.end());
+    } else {                                                          // vecD2mapR + WRTE_OUTBIN
+        rec.assign((size_t)nx * (size_t)ny, (float)rmis);
+        for (size_t i = 0; i < n; ++i) rec[(size_t)(seqy[i] - 1) * nx + (seqx[i] - 1)] = (float)vec[i];
+    }
+    FILE* f = std::fopen(file, "ab");
+    if (!f) return fail(CMF_EIO, "cannot open %s for appending", file);
+    bool ok = std::fwrite(rec.data(), sizeof(float), rec.size(), f) == rec.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return fail(CMF_EIO, "writing %s failed", file);
+    return 0;
+}
+
 int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
                          const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly) {
     if (!h || !file) return fail(CMF_EINVAL, "NULL argument");

@@ -194,6 +194,20 @@ int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv);
 int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf);
 int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev);
 
+/* One output record of variable `var` appended to `file`, as CMF_OUTPUT_WRITE
+ * writes it in binary mode (src/cmf_ctrl_output_mod.F90:474-664): float32,
+ * either an NX*NY map with RMIS off the river cells (loutvec = 0) or the NSEQALL
+ * vector (LOUTVEC); pthflw as an NPTHOUT*NPTHLEV record. Names and sources as
+ * the reference's: rivout, fldout, outflw/totout, rivvel, pthout, pthflw,
+ * runoff/rofsfc, daminf (output-period averages, so after
+ * cmf_hip_diag_getave_output), maxflw, maxdph, maxsto (output-period maxima),
+ * rivsto, fldsto, rivdph, flddph, fldfrc, fldare, sfcelv, storge/totsto, damsto,
+ * levsto, levdph (current state). Variables of a switched-off option are skipped
+ * (return 0, nothing written), as the reference skips them; other names are
+ * CMF_EINVAL. Single-model handles. */
+int cmf_hip_output_write(cmf_handle h, const char* var, const char* file, int32_t nx, int32_t ny,
+                         const int32_t* seqx, const int32_t* seqy, int32_t loutvec, double rmis);
+
 /* Binary restart straight from / into device state: WRTE_REST_BIN and
  * READ_REST_BIN of CMF_RESTART_WRITE / CMF_RESTART_INIT
  * (src/cmf_ctrl_restart_mod.F90:389-482, 484-517 and its reader), single-model

@@ -9,8 +9,9 @@ outputs reduced to:
   * map tables after initialisation: SHA-1 of each array (float64/int32 bytes);
   * per outer step: NT, SHA-1 of every state array, the 10 budget sums;
   * the full final state of the main prognostic/diagnostic fields.
-and, for the cases of RESTART_CASES, the reference's end-of-run binary restart file.
-Usage: python tests/golden/make_golden.py [case ... | case_restart ...]
+and, for the cases of RESTART_CASES, the reference's end-of-run binary restart file
+and, for OUTPUT_CASES, its daily binary output files.
+Usage: python tests/golden/make_golden.py [case ... | case_restart ... | case_output ...]
 """
 import hashlib
 import json
@@ -158,13 +159,43 @@ def make_restart(name):
     print(f"{name}: restart {files[0]} ({rec['bin'].size} bytes)")
 
 
+OUTPUT_CASES = ("g_pth",)
+OUTPUT_VARS = ("rivout,fldout,outflw,rivvel,pthout,runoff,maxflw,maxdph,maxsto,rivsto,fldsto,rivdph,"
+               "flddph,fldfrc,fldare,sfcelv,storge,pthflw")
+
+
+def make_output(name):
+    """The reference's own daily binary output files (CMF_OUTPUT_WRITE: float32
+    NX*NY records, pthflw as NPTHOUT*NPTHLEV records) of a golden case."""
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = GOLDEN[name]
+    net, ro = load_network(os.path.join(HERE, f"{name}_inputs.npz"))
+    rec = {}
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, outputs=OUTPUT_VARS)
+        r = subprocess.run([REF, "runoff.bin", "NONE", "0"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+        for v in OUTPUT_VARS.split(","):
+            files = [f for f in os.listdir(d) if f.startswith(v + "_cmf")]
+            if len(files) != 1:
+                raise RuntimeError(f"{v}: expected one output file, found {files}")
+            rec[v] = np.fromfile(os.path.join(d, files[0]), np.uint8)
+    np.savez_compressed(os.path.join(HERE, f"{name}_output.npz"), **rec)
+    print(f"{name}: output files of {len(rec)} variables")
+
+
 if __name__ == "__main__":
     names = sys.argv[1:] or list(GOLDEN)
     for name in names:
         if name.endswith("_restart"):
             make_restart(name[:-len("_restart")])
+        elif name.endswith("_output"):
+            make_output(name[:-len("_output")])
         else:
             make(name)
     if not sys.argv[1:]:
         for name in RESTART_CASES:
             make_restart(name)
+        for name in OUTPUT_CASES:
+            make_output(name)
