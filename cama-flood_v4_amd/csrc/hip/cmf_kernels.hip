@@ -471,13 +471,21 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     // the mixed path forms OUTFLW's average from the RIVOUT and FLDOUT averages on
     // the last substep instead of accumulating it (it differs by FP32 rounding)
     constexpr bool kOutSum = sizeof(R) == sizeof(float);
-    const R p_ravg = S.rivout_avg[s], p_favg = S.fldout_avg[s];
-    const R p_oavg = kOutSum ? zero : S.outflw_avg[s];
-    const R p_omax = S.outflw_max[s], p_dmax = S.rivdph_max[s], p_smax = S.storge_max[s];
-    const R p_pavg = A.lpthout ? S.pthout_avg[s] : zero;
-    R p_davg = zero;
+    // The running averages and maxima are read up front in the single-model
+    // kernels; the ensemble kernels read them where they are used, which keeps
+    // fewer values live across the stage inversion (more waves per SIMD).
+    R p_ravg = zero, p_favg = zero, p_oavg = zero, p_omax = zero, p_dmax = zero, p_smax = zero;
+    R p_pavg = zero, p_davg = zero;
+    auto load_avg = [&]() {
+        p_ravg = S.rivout_avg[s]; p_favg = S.fldout_avg[s];
+        if (!kOutSum) p_oavg = S.outflw_avg[s];
+        p_omax = S.outflw_max[s]; p_dmax = S.rivdph_max[s]; p_smax = S.storge_max[s];
+        if (A.lpthout) p_pavg = S.pthout_avg[s];
+        if (A.ldamout) p_davg = S.daminf_avg[s];
+    };
+    if (!MB) load_avg();
     double daminf = 0.0, damsto = 0.0;
-    if (A.ldamout) { p_davg = S.daminf_avg[s]; daminf = S.daminf[s]; }
+    if (A.ldamout) daminf = S.daminf[s];
     if (!I.active) return;
     const double dt = I.dt;
     const bool f0 = I.first;
@@ -589,6 +597,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     g[G_FLDARE] = o.fldare;
     g[G_LEVSTO] = A.llevee ? o.levsto : 0.0;
     // CMF_DIAG_AVEMAX_ADPSTP, diag:108-124, and GETAVE :179-186 on the last substep
+    if (MB) load_avg();
     const R rdt = R(dt);
     R a1 = (f0 ? zero : p_ravg) + qrl * rdt;
     R a2 = (f0 ? zero : p_favg) + qfl * rdt;
