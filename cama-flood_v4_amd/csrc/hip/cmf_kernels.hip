@@ -471,19 +471,9 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     // the mixed path forms OUTFLW's average from the RIVOUT and FLDOUT averages on
     // the last substep instead of accumulating it (it differs by FP32 rounding)
     constexpr bool kOutSum = sizeof(R) == sizeof(float);
-    // The running averages and maxima are read up front in the single-model
-    // kernels; the ensemble kernels read them where they are used, which keeps
-    // fewer values live across the stage inversion (more waves per SIMD).
-    R p_ravg = zero, p_favg = zero, p_oavg = zero, p_omax = zero, p_dmax = zero, p_smax = zero;
-    R p_pavg = zero, p_davg = zero;
-    auto load_avg = [&]() {
-        p_ravg = S.rivout_avg[s]; p_favg = S.fldout_avg[s];
-        if (!kOutSum) p_oavg = S.outflw_avg[s];
-        p_omax = S.outflw_max[s]; p_dmax = S.rivdph_max[s]; p_smax = S.storge_max[s];
-        if (A.lpthout) p_pavg = S.pthout_avg[s];
-        if (A.ldamout) p_davg = S.daminf_avg[s];
-    };
-    if (!MB) load_avg();
+    // The running averages and maxima are read where they are used, not up front:
+    // fewer values live across the stage inversion, 5 waves per SIMD instead of 4
+    // for the FP64 update (measured 5% faster than hoisting them).
     double daminf = 0.0, damsto = 0.0;
     if (A.ldamout) daminf = S.daminf[s];
     if (!I.active) return;
@@ -597,7 +587,11 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     g[G_FLDARE] = o.fldare;
     g[G_LEVSTO] = A.llevee ? o.levsto : 0.0;
     // CMF_DIAG_AVEMAX_ADPSTP, diag:108-124, and GETAVE :179-186 on the last substep
-    if (MB) load_avg();
+    const R p_ravg = S.rivout_avg[s], p_favg = S.fldout_avg[s];
+    const R p_oavg = kOutSum ? zero : S.outflw_avg[s];
+    const R p_omax = S.outflw_max[s], p_dmax = S.rivdph_max[s], p_smax = S.storge_max[s];
+    const R p_pavg = A.lpthout ? S.pthout_avg[s] : zero;
+    const R p_davg = A.ldamout ? S.daminf_avg[s] : zero;
     const R rdt = R(dt);
     R a1 = (f0 ? zero : p_ravg) + qrl * rdt;
     R a2 = (f0 ? zero : p_favg) + qfl * rdt;
@@ -717,8 +711,10 @@ __global__ __launch_bounds__(BS) void k_rate(DevMap M, DevState<R> S, StepArgs A
     if (t < ntc) rate_tile<R, BS, MB>(M, S, A, t);
 }
 
+// at most 96 VGPRs (5 waves per SIMD): the FP64 single-model update needs 97
+// without the bound and spills nothing to memory with it
 template <class R, int BS, bool MB>
-__global__ __launch_bounds__(BS) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+__global__ __launch_bounds__(BS, 5) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     __shared__ double lds[NGLB * (BS / 64)];
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
     if (t < nt_all) update_tile<R, BS, MB>(M, S, A, t, ntc, lds);
