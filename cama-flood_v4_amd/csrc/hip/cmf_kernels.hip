@@ -699,8 +699,11 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
     }
 }
 
+// The mixed-precision ensemble outflow is held to 64 VGPRs (8 waves per SIMD;
+// it needs 66 without the bound, 7 waves) and spills nothing; the single-model
+// forms would spill and keep the compiler's choice.
 template <class R, int BS, bool MB>
-__global__ __launch_bounds__(BS) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+__global__ __launch_bounds__(BS, (MB && sizeof(R) == 4) ? 8 : 1) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
     if (t < nt_all) outflow_tile<R, BS, MB>(M, S, A, t, ntc);
 }
@@ -712,9 +715,11 @@ __global__ __launch_bounds__(BS) void k_rate(DevMap M, DevState<R> S, StepArgs A
 }
 
 // at most 96 VGPRs (5 waves per SIMD): the FP64 single-model update needs 97
-// without the bound and spills nothing to memory with it
+// without the bound and spills nothing to memory with it. The mixed-precision
+// ensemble update fits 80 (6 waves; 82 without the bound), again without spills;
+// the FP64 ensemble update would spill at 6.
 template <class R, int BS, bool MB>
-__global__ __launch_bounds__(BS, 5) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+__global__ __launch_bounds__(BS, (MB && sizeof(R) == 4) ? 6 : 5) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     __shared__ double lds[NGLB * (BS / 64)];
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
     if (t < nt_all) update_tile<R, BS, MB>(M, S, A, t, ntc, lds);
