@@ -315,7 +315,8 @@ def ensemble_section(a):
         f"({time.time() - t0:.1f}s)")
     res = {}
     for prec in ("mixed", "fp64"):
-        r = run_ensemble(nets, ms, prec, a.ens_days, 1, a.seed, tune="auto")
+        r = run_ensemble(nets, ms, prec, a.ens_days, 1, a.seed,
+                         tune="auto" if a.ens_tune == "auto" else json.loads(a.ens_tune))
         res[prec] = r
         steps = max(r["kt"][4], 1.0)
         r["kernels_ms"] = {k: round(r["kt"][i] / steps, 4) for i, k in enumerate(("outflow", "rate", "update"))}
@@ -520,6 +521,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy three times (HBM counter calibration)")
     ap.add_argument("--ens-days", type=int, default=3)
+    ap.add_argument("--ens-tune", type=str, default="auto",
+                    help='launch configuration of both ensemble runs: "auto" (autotuner) or a JSON object '
+                         'of cmf_tune fields')
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
