@@ -4,7 +4,9 @@
 # libcmf_hip.so: the drop-in configuration a maintainer would build. The driver
 # program is oracle/ref_driver.F90 (the same one that writes the golden dumps), so
 # the run can be compared with the reference's own dumps.
-# Output: cama-flood_v4_amd/fortran/_build/cmf_dropin (git-ignored).
+# Outputs (git-ignored): cama-flood_v4_amd/fortran/_build/cmf_dropin, and
+# cmf_dropin_devdiag, which also replaces CMF_CALC_DIAG_MOD by this package's
+# cmf_calc_diag_mod.F90 (output-period diagnostics on the device).
 set -euo pipefail
 REF=${CMF_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -15,24 +17,34 @@ FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
 FFLAGS="-O2 -cpp -ffp-contract=off"
 if [ ! -d "$REF/src" ]; then echo "build_dropin: no reference sources at $REF/src; skipping" >&2; exit 0; fi
 if [ ! -f "$PKG/cmf/libcmf_hip.so" ]; then echo "build_dropin: build libcmf_hip.so first" >&2; exit 1; fi
-mkdir -p "$OUT/obj"
 SRCS="parkind1 yos_cmf_input yos_cmf_time yos_cmf_map yos_cmf_prog yos_cmf_diag cmf_utils_mod
-cmf_calc_outflw_mod cmf_calc_pthout_mod cmf_calc_fldstg_mod cmf_calc_stonxt_mod cmf_calc_diag_mod cmf_opt_outflw_mod
+cmf_calc_outflw_mod cmf_calc_pthout_mod cmf_calc_fldstg_mod cmf_calc_stonxt_mod DIAG cmf_opt_outflw_mod
 cmf_ctrl_mpi_mod cmf_ctrl_damout_mod cmf_ctrl_tracer_mod cmf_ctrl_levee_mod cmf_ctrl_forcing_mod
-cmf_ctrl_boundary_mod cmf_ctrl_upsinf_mod cmf_ctrl_output_mod cmf_ctrl_restart_mod PHYSICS
+cmf_ctrl_boundary_mod cmf_ctrl_upsinf_mod cmf_ctrl_output_mod cmf_ctrl_restart_mod PHYSICS DEVDIAG
 cmf_ctrl_time_mod cmf_ctrl_maps_mod cmf_ctrl_vars_mod cmf_ctrl_nmlist_mod
 cmf_drv_control_mod cmf_drv_advance_mod"
-OBJS=""
-for s in $SRCS; do
-  if [ $s = PHYSICS ]; then
-    f=$HERE/cmf_ctrl_physics_mod.F90; o=$OUT/obj/cmf_ctrl_physics_mod_hip.o
-  else
-    f=$REF/src/$s.F90; o=$OUT/obj/$s.o
-  fi
-  "$FC" $FFLAGS -module-dir "$OUT/obj" -c "$f" -o "$o"
-  OBJS="$OBJS $o"
-done
-"$FC" $FFLAGS -module-dir "$OUT/obj" -I "$OUT/obj" -c "$ROOT/oracle/ref_driver.F90" -o "$OUT/obj/ref_driver.o"
-"$FC" $FFLAGS -o "$OUT/cmf_dropin" "$OUT/obj/ref_driver.o" $OBJS \
-  -L"$PKG/cmf" -lcmf_hip -Wl,-rpath,'$ORIGIN/../../cmf' -Wl,-rpath,/opt/rocm/lib
-echo "build_dropin: built $OUT/cmf_dropin"
+
+# build_variant NAME DEVICE_DIAG: DEVICE_DIAG=1 links this package's device-side
+# CMF_CALC_DIAG_MOD (compiled after the physics module it uses) in place of the
+# reference's
+build_variant() {
+  local name=$1 devdiag=$2 obj=$OUT/obj_$1 objs="" f o
+  mkdir -p "$obj"
+  for s in $SRCS; do
+    case $s in
+      PHYSICS) f=$HERE/cmf_ctrl_physics_mod.F90; o=$obj/cmf_ctrl_physics_mod_hip.o ;;
+      DIAG)    [ "$devdiag" = 1 ] && continue; f=$REF/src/cmf_calc_diag_mod.F90; o=$obj/cmf_calc_diag_mod.o ;;
+      DEVDIAG) [ "$devdiag" = 1 ] || continue; f=$HERE/cmf_calc_diag_mod.F90; o=$obj/cmf_calc_diag_mod_hip.o ;;
+      *)       f=$REF/src/$s.F90; o=$obj/$s.o ;;
+    esac
+    "$FC" $FFLAGS -module-dir "$obj" -c "$f" -o "$o"
+    objs="$objs $o"
+  done
+  "$FC" $FFLAGS -module-dir "$obj" -I "$obj" -c "$ROOT/oracle/ref_driver.F90" -o "$obj/ref_driver.o"
+  "$FC" $FFLAGS -o "$OUT/$name" "$obj/ref_driver.o" $objs \
+    -L"$PKG/cmf" -lcmf_hip -Wl,-rpath,'$ORIGIN/../../cmf' -Wl,-rpath,/opt/rocm/lib
+  echo "build_dropin: built $OUT/$name"
+}
+mkdir -p "$OUT"
+build_variant cmf_dropin 0
+build_variant cmf_dropin_devdiag 1

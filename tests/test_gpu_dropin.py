@@ -4,8 +4,14 @@ CMF_CTRL_PHYSICS_MOD replacement, which drives the HIP physics through
 include/cmf_hip.h. Run on the golden fixtures, it must reproduce the reference's
 substep counts and final state to 1e-10 relative.
 
-The binary is built by cama-flood_v4_amd/fortran/build_dropin.sh where the
-reference sources exist; the test is skipped when it has not been built."""
+The output files that the reference's own writer (CMF_OUTPUT_WRITE) produces in
+the drop-in must match the reference's (tests/golden/g_pth_output.npz) to one
+float32 ulp, both with the reference's host-side CMF_CALC_DIAG_MOD (cmf_dropin)
+and with this package's device-side one (cmf_dropin_devdiag: output-period sums
+on the device, no per-step download of the substep averages).
+
+The binaries are built by cama-flood_v4_amd/fortran/build_dropin.sh where the
+reference sources exist; the tests are skipped when they have not been built."""
 import os
 import subprocess
 
@@ -16,7 +22,10 @@ from cases import rel_err
 from golden_cases import load
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "cama-flood_v4_amd", "fortran", "_build", "cmf_dropin")
+BUILD = os.path.join(ROOT, "cama-flood_v4_amd", "fortran", "_build")
+EXE = os.path.join(BUILD, "cmf_dropin")
+OUTPUT_VARS = ("rivout,fldout,outflw,rivvel,pthout,runoff,maxflw,maxdph,maxsto,rivsto,fldsto,rivdph,"
+               "flddph,fldfrc,fldare,sfcelv,storge,pthflw")
 
 
 @pytest.mark.gpu
@@ -51,3 +60,44 @@ def test_dropin_matches_reference(name, tmp_path):
         e = rel_err(last[k], v)
         print(f"  {k}: rel err {e:.3e}", flush=True)
         assert e <= 1e-10, f"{name}: {k} rel err {e:.3e}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe", ["cmf_dropin", "cmf_dropin_devdiag"])
+def test_dropin_output_files_match_reference(exe, tmp_path):
+    """g_pth (bifurcations, 4 daily steps, daily output of 18 variables): the files
+    CMF_OUTPUT_WRITE writes in the drop-in against the reference's own files."""
+    path = os.path.join(BUILD, exe)
+    if not os.path.exists(path):
+        pytest.skip(f"{exe} not built")
+    from cmf import synth
+    from golden_cases import setup
+    print(f"drop-in output files, {exe}", flush=True)
+    net, ro, ref, fin = load("g_pth")
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
+    m = setup("g_pth")[0]
+    nsteps = int(round(days * 86400.0 / dt))
+    d = str(tmp_path)
+    synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                 lpthout=lpth, llevee=llev, ldamout=ldam, outputs=OUTPUT_VARS)
+    r = subprocess.run([path, "runoff.bin", "NONE", "0"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    z = np.load(os.path.join(ROOT, "tests", "golden", "g_pth_output.npz"), allow_pickle=False)
+    cell = (m.seqy.astype(np.int64) - 1) * m.nx + (m.seqx.astype(np.int64) - 1)
+    off = np.ones(m.nx * m.ny, bool)
+    off[cell] = False
+    for v in OUTPUT_VARS.split(","):
+        files = [f for f in os.listdir(d) if f.startswith(v + "_cmf")]
+        assert len(files) == 1, (v, files)
+        got = np.fromfile(os.path.join(d, files[0]), np.float32)
+        want = z[v].view(np.float32)
+        assert got.size == want.size, v
+        if v != "pthflw":
+            got, want = got.reshape(nsteps, -1), want.reshape(nsteps, -1)
+            assert np.all(got[:, off] == want[:, off]), v
+            got, want = got[:, cell], want[:, cell]
+        ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+        same_sign = np.sign(got) == np.sign(want)
+        ok = (got == want) | (same_sign & (ulp <= 1))
+        print(f"  {v}: {int((got != want).sum())} of {got.size} values differ", flush=True)
+        assert ok.all(), v
