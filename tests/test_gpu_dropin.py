@@ -4,6 +4,10 @@ CMF_CTRL_PHYSICS_MOD replacement, which drives the HIP physics through
 include/cmf_hip.h. Run on the golden fixtures, it must reproduce the reference's
 substep counts and final state to 1e-10 relative.
 
+Without CMF_HIP_DOWNLOAD=every the module downloads the state only on output and
+restart steps; the output-file and restart-file tests below run that way, so a
+step that the host reads but the module skipped would show as stale records.
+
 The output files that the reference's own writer (CMF_OUTPUT_WRITE) produces in
 the drop-in must match the reference's (tests/golden/g_pth_output.npz) to one
 float32 ulp, both with the reference's host-side CMF_CALC_DIAG_MOD (cmf_dropin)
@@ -41,8 +45,11 @@ def test_dropin_matches_reference(name, tmp_path):
     d = str(tmp_path)
     synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
                                  lpthout=lpth, llevee=llev, ldamout=ldam)
+    # the test driver reads the state after arbitrary steps (its dumps): whole
+    # state downloaded after every step
+    env = dict(os.environ, CMF_HIP_DOWNLOAD="every")
     r = subprocess.run([EXE, "runoff.bin", "dump.bin", "0"], cwd=d, capture_output=True, text=True,
-                       timeout=300)
+                       timeout=300, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     mp, states = refdump.read_dump(os.path.join(d, "dump.bin"), lpth, llev, ldam)
     nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
@@ -51,7 +58,7 @@ def test_dropin_matches_reference(name, tmp_path):
     # a second dump written at the end (dump_every = number of steps)
     nsteps = len(ref["nt"])
     r = subprocess.run([EXE, "runoff.bin", "dump2.bin", str(nsteps)], cwd=d, capture_output=True,
-                       text=True, timeout=300)
+                       text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     mp, states = refdump.read_dump(os.path.join(d, "dump2.bin"), lpth, llev, ldam)
     last = states[-1]
@@ -101,3 +108,47 @@ def test_dropin_output_files_match_reference(exe, tmp_path):
         ok = (got == want) | (same_sign & (ulp <= 1))
         print(f"  {v}: {int((got != want).sum())} of {got.size} values differ", flush=True)
         assert ok.all(), v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exe", ["cmf_dropin", "cmf_dropin_devdiag"])
+@pytest.mark.parametrize("name", ["g_plain", "g_allwet"])
+def test_dropin_restart_matches_reference(name, exe, tmp_path):
+    """The end-of-run restart that the reference's CMF_RESTART_WRITE writes in the
+    drop-in (the state downloaded because restart_is_write_time holds on the last
+    step) against the reference's own file: 1e-10 of each record's magnitude, the
+    bound of the device physics; g_allwet adds the path, levee and dam records."""
+    path = os.path.join(BUILD, exe)
+    if not os.path.exists(path):
+        pytest.skip(f"{exe} not built")
+    from cmf import synth
+    from golden_cases import setup
+    print(f"drop-in restart {name}, {exe}", flush=True)
+    net, ro, ref, fin = load(name)
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
+    m = setup(name)[0]
+    d = str(tmp_path)
+    synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                 lpthout=lpth, llevee=llev, ldamout=ldam, restart=True)
+    r = subprocess.run([path, "runoff.bin", "NONE", "0"], cwd=d, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    files = sorted(f for f in os.listdir(d) if f.startswith("restart") and f.endswith(".bin"))
+    assert len(files) == 1, files
+    z = np.load(os.path.join(ROOT, "tests", "golden", f"{name}_restart.npz"), allow_pickle=False)
+    assert files[0] == bytes(z["name"]).decode()
+    keys = ["rivsto", "fldsto", "rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre"]
+    keys += (["damsto"] if m.opts.ldamout else []) + (["levsto"] if m.opts.llevee else [])
+    ng = m.nx * m.ny
+    want = z["bin"].view(np.float64).reshape(len(keys), ng)
+    got = np.fromfile(os.path.join(d, files[0]), np.float64)
+    assert got.size == want.size
+    got = got.reshape(len(keys), ng)
+    for k, (g, w) in enumerate(zip(got, want)):
+        e = np.abs(g - w).max() / max(np.abs(w[w < 1e19]).max(), 1e-30)
+        print(f"  {keys[k]}: rel err {e:.3e}", flush=True)
+        assert e <= 1e-10, keys[k]
+    if lpth:
+        gp = np.fromfile(os.path.join(d, files[0] + ".pth"), np.float64)
+        wp = z["pth"].view(np.float64)
+        assert gp.size == wp.size
+        assert np.abs(gp - wp).max() <= 1e-10 * max(np.abs(wp).max(), 1e-30)
