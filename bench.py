@@ -109,50 +109,102 @@ def run_gpu(net, m, rvec, precision, steps, warmup, dt, tune="auto"):
     return sum_nt, t_total, kt, tune_used
 
 
+def _reference_case(d, net, ro, dt, steps):
+    """Reference inputs of the workload (levees, dams) in directory d, the run
+    shortened to `steps` outer steps."""
+    from cmf import synth
+    days = max(1, int(np.ceil(steps * dt / 86400.0)))
+    synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                 llevee=True, ldamout=True)
+    # end time = start + steps*dt
+    nml = open(os.path.join(d, "input_cmf.nam")).read()
+    import datetime as _dt
+    t1 = _dt.datetime(2001, 1, 1) + _dt.timedelta(seconds=steps * dt)
+    nml = nml.replace(nml[nml.index("EYEAR"):nml.index("EHOUR")],
+                      f"EYEAR = {t1.year}\nEMON  = {t1.month}\nEDAY  = {t1.day}\n")
+    nml = nml.replace("EHOUR = 0", f"EHOUR = {t1.hour}")
+    open(os.path.join(d, "input_cmf.nam"), "w").write(nml)
+
+
+def _big_stack():
+    """the reference keeps NX*NY work arrays on the stack (its run scripts: ulimit -s unlimited)"""
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def _run_host(exe, d, env):
+    """Run a reference-host binary (reference build or drop-in) in d as a child
+    process; returns (stdout, per-step seconds, substeps per step) or None."""
+    from oracle import refdump
+    r = subprocess.run([exe, "runoff.bin", "NONE", "0"], cwd=d, env=env, capture_output=True,
+                       text=True, timeout=900, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        log(f"{os.path.basename(exe)} run failed (rc={r.returncode}): {r.stdout[-800:]} {r.stderr[-800:]}")
+        try:
+            log(open(os.path.join(d, "log_CaMa.txt")).read()[-1500:])
+        except OSError:
+            pass
+        return None
+    steps = [float(l.split()[2]) for l in r.stdout.splitlines() if l.startswith("STEPTIME")]
+    return r.stdout, steps, refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+
+
 def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
     """Reference OpenMP build (omp64: FP64; omp32: the reference's default
     single-precision mode) timed on this host over a bounded sample."""
     exe = os.path.join(ROOT, "oracle", "_ref", variant, "cmf_ref")
     if not os.path.exists(exe):
         return None
-    from cmf import synth
-    from oracle import refdump
     ncores = len(os.sched_getaffinity(0))
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
-        days = max(1, int(np.ceil(sample_steps * dt / 86400.0)))
-        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
-                                     llevee=True, ldamout=True)
-        # shorten the run to the sample: end time = start + sample_steps*dt
-        nml = open(os.path.join(d, "input_cmf.nam")).read()
-        import datetime as _dt
-        t1 = _dt.datetime(2001, 1, 1) + _dt.timedelta(seconds=sample_steps * dt)
-        nml = nml.replace(nml[nml.index("EYEAR"):nml.index("EHOUR")],
-                          f"EYEAR = {t1.year}\nEMON  = {t1.month}\nEDAY  = {t1.day}\n")
-        nml = nml.replace("EHOUR = 0", f"EHOUR = {t1.hour}")
-        open(os.path.join(d, "input_cmf.nam"), "w").write(nml)
+        _reference_case(d, net, ro, dt, sample_steps)
         env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores",
                    OMP_STACKSIZE="512M")
-        import resource
-
-        def big_stack():   # the reference keeps NX*NY work arrays on the stack (run scripts: ulimit -s unlimited)
-            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
-        r = subprocess.run([exe, "runoff.bin", "NONE", "0"], cwd=d, env=env, capture_output=True,
-                           text=True, timeout=900, preexec_fn=big_stack)
-        if r.returncode != 0:
-            log(f"reference run failed (rc={r.returncode}): {r.stdout[-800:]} {r.stderr[-800:]}")
-            try:
-                log(open(os.path.join(d, "log_CaMa.txt")).read()[-1500:])
-            except OSError:
-                pass
+        res = _run_host(exe, d, env)
+        if res is None:
             return None
-        line = [l for l in r.stdout.splitlines() if l.startswith("REFTIME")][0].split()
+        out, _steps, nts = res
+        line = [l for l in out.splitlines() if l.startswith("REFTIME")][0].split()
         t, nsteps, n = float(line[1]), int(line[2]), int(line[3])
-        nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
         cs = n * sum(nts)
         return dict(value=cs / t, unit="cell-substeps/s", cores=ncores, kind="reference",
                     sample=f"reference {'FP64' if variant == 'omp64' else 'single-precision'} OpenMP "
                            f"(flang -O3 -fopenmp), {nsteps} outer steps of "
                            f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
+
+
+def dropin_section(net, ro, n_cells, dt, steps):
+    """The Fortran drop-in end to end: the reference's host (serial flang build:
+    time loop, forcing, diagnostics, log) with the device physics and the
+    device-side output diagnostics (cmf_dropin_devdiag), on the same inputs as the
+    CPU baseline, without output or restart files. Timed per outer step inside
+    CMF_DRV_ADVANCE; the first step also uploads the map and the state. Once as
+    built (downloads only when the host reads the state) and once with
+    CMF_HIP_DOWNLOAD=every (the whole state after every step over PCIe)."""
+    exe = os.path.join(ROOT, "cama-flood_v4_amd", "fortran", "_build", "cmf_dropin_devdiag")
+    if not os.path.exists(exe):
+        return None
+    res = {"binary": "cmf_dropin_devdiag", "cells": n_cells, "steps": steps, "dt": dt}
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        _reference_case(d, net, ro, dt, steps)
+        for mode in ("due", "every"):
+            env = dict(os.environ)
+            if mode == "every":
+                env["CMF_HIP_DOWNLOAD"] = "every"
+            log(f"drop-in run, downloads {mode}")
+            r = _run_host(exe, d, env)
+            if r is None:
+                return None
+            _out, st, nts = r
+            if len(st) < 3:
+                return None
+            rest = st[1:]
+            res[mode] = dict(first_step_s=round(st[0], 3), step_ms_median=round(1e3 * float(np.median(rest)), 2),
+                             cell_substeps_per_s=n_cells * sum(nts[1:]) / sum(rest),
+                             substeps=int(sum(nts[1:])))
+            log(f"drop-in ({mode}): first step {st[0]:.2f} s, then {1e3 * np.median(rest):.1f} ms per "
+                f"{dt:.0f} s step, {res[mode]['cell_substeps_per_s']:.3g} cell-substeps/s")
+    return res
 
 
 def initial_state(net, m):
@@ -513,6 +565,8 @@ def main():
                          'of cmf_tune fields')
     ap.add_argument("--no-spinup", action="store_true")
     ap.add_argument("--no-forcing", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=12)
     ap.add_argument("--spinup-only", action="store_true")
     ap.add_argument("--spin-epochs", type=int, default=150)
     ap.add_argument("--spin-years", type=int, default=20)
@@ -610,6 +664,13 @@ def main():
                 mixed["cpu_baseline"] = c32
                 if c32:
                     log(f"CPU reference single precision: {c32['value']:.4g} cell-substeps/s")
+    dropin = None
+    if not a.no_dropin:
+        try:
+            dropin = dropin_section(net, ro, n, dt, a.dropin_steps)
+        except Exception as e:
+            log(f"drop-in section failed: {e!r}")
+            dropin = {"error": repr(e)}
     roof["traffic"] = measured_traffic(roof["kernel"]) if a.nx == 3600 and a.ny == 1800 else None
     sub_ms = sum(per.values())
     roof_sub = dict(bound="hbm", kernels="k_outflow + k_rate + k_update", avg_ms=round(sub_ms, 4),
@@ -632,6 +693,7 @@ def main():
         "ensemble": ens,
         "forcing_interpolation": forcing,
         "spinup": spin,
+        "dropin": dropin,
     }
     print(json.dumps(out), flush=True)
 
