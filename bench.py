@@ -27,7 +27,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell-substep by kernel: the 508 B (FP64) and 276 B (mixed)
-# of the per-item table of the survey, each item charged to the kernel that moves it
+# of the per-item byte count of one substep, each item charged to the kernel that moves it
 BYTES_FP64 = {"outflow": 180.0, "rate": 44.0, "update": 284.0}
 BYTES_MIXED = {"outflow": 96.0, "rate": 24.0, "update": 156.0}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_c3_fp64.json")

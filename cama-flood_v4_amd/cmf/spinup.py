@@ -6,7 +6,7 @@ times (gosh/test1-glb_15min.sh:311-345: NSP loops, no convergence test). Here th
 year is repeated until the total storage of a domain changes by less than a
 tolerance from one year end to the next (0.1 % by default), and the number of
 simulated years needed is the figure of merit that the learned warm start
-(:mod:`cmf.surrogate`) is judged by.
+(:mod:`cmf.surrogate`) is measured by.
 
 Climatology: twelve monthly runoff fields per cell (m3/s), a spatially correlated
 gamma field scaled by a seasonal cycle and by random month-to-month factors;

@@ -484,7 +484,7 @@ def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1
     under the annual-mean runoff, from the physical prior (linear periodic
     response) and from the surrogate's warm start, run together as four members of
     one handle (same map and forcing, own adaptive steps), until every annual
-    storage change is below tol or `years` pass. The equilibrium all are judged
+    storage change is below tol or `years` pass. The equilibrium all are measured
     against is the last year of the member whose last annual change is smallest."""
     import time
     from .model import CMFModel

@@ -6,7 +6,7 @@
  * The product (cama-flood_v4_amd/) never links, loads or calls it.
  *
  * It restates, function by function, the reference Fortran built in FP64 mode
- * (no -DSinglePrec_CMF), with the default namelist options that the north-star
+ * (no -DSinglePrec_CMF), with the default namelist options that the device
  * path uses (LSPAMAT, local-inertial outflow, optional LFLDOUT/LPTHOUT/LDAMOUT/
  * LLEVEE/LADPSTP). Every function cites the reference file:line it follows.
  * Arithmetic keeps the reference's operation order and constant kinds: default-kind

@@ -1,6 +1,6 @@
 """HIP physics (FP64, through the C-ABI) against the reference Fortran's golden
 fixtures: identical substep counts NT at every outer step and final fields within
-1e-10 relative of each field's magnitude (the north-star bound on storage per
+1e-10 relative of each field's magnitude (the required bound on storage per
 simulated day, applied here to the whole run)."""
 import numpy as np
 import pytest
