@@ -132,10 +132,24 @@ def _big_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
+def _substeps(log_path):
+    """NT per outer step from a CaMa-Flood log (LOGNAM): one water-budget line
+    (' flx: ') per substep, steps closed by the 'DRV_ADVANCE END' line
+    (src/cmf_ctrl_physics_mod.F90:241-246, src/cmf_drv_advance_mod.F90:165)."""
+    nts, cur = [], 0
+    with open(log_path, errors="replace") as f:
+        for line in f:
+            if " flx: " in line:
+                cur += 1
+            elif "DRV_ADVANCE END" in line:
+                nts.append(cur)
+                cur = 0
+    return nts
+
+
 def _run_host(exe, d, env):
     """Run a reference-host binary (reference build or drop-in) in d as a child
     process; returns (stdout, per-step seconds, substeps per step) or None."""
-    from oracle import refdump
     r = subprocess.run([exe, "runoff.bin", "NONE", "0"], cwd=d, env=env, capture_output=True,
                        text=True, timeout=900, preexec_fn=_big_stack)
     if r.returncode != 0:
@@ -146,7 +160,7 @@ def _run_host(exe, d, env):
             pass
         return None
     steps = [float(l.split()[2]) for l in r.stdout.splitlines() if l.startswith("STEPTIME")]
-    return r.stdout, steps, refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+    return r.stdout, steps, _substeps(os.path.join(d, "log_CaMa.txt"))
 
 
 def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
@@ -177,7 +191,8 @@ def dropin_section(net, ro, n_cells, dt, steps):
     """The Fortran drop-in end to end: the reference's host (serial flang build:
     time loop, forcing, diagnostics, log) with the device physics and the
     device-side output diagnostics (cmf_dropin_devdiag), on the same inputs as the
-    CPU baseline, without output or restart files. Timed per outer step inside
+    CPU baseline, without output or restart files, driven by the package's
+    host program (fortran/cmf_dropin_main.F90). Timed per outer step inside
     CMF_DRV_ADVANCE; the first step also uploads the map and the state. Once as
     built (downloads only when the host reads the state) and once with
     CMF_HIP_DOWNLOAD=every (the whole state after every step over PCIe)."""

@@ -1,12 +1,13 @@
 #!/bin/bash
 # Links the unmodified reference Fortran host (read in place from
 # /root/reference/src) with this package's CMF_CTRL_PHYSICS_MOD replacement and
-# libcmf_hip.so: the drop-in configuration a maintainer would build. The driver
-# program is oracle/ref_driver.F90 (the same one that writes the golden dumps), so
-# the run can be compared with the reference's own dumps.
-# Outputs (git-ignored): cama-flood_v4_amd/fortran/_build/cmf_dropin, and
-# cmf_dropin_devdiag, which also replaces CMF_CALC_DIAG_MOD by this package's
-# cmf_calc_diag_mod.F90 (output-period diagnostics on the device).
+# libcmf_hip.so: the drop-in configuration a maintainer would build.
+# Outputs (git-ignored), in cama-flood_v4_amd/fortran/_build/:
+#   cmf_dropin          the reference's CMF_CALC_DIAG_MOD; main program: the test
+#                       driver oracle/ref_driver.F90 (writes the state dumps the
+#                       tests compare with the reference's own)
+#   cmf_dropin_devdiag  this package's device-side cmf_calc_diag_mod.F90; main
+#                       program: this package's cmf_dropin_main.F90
 set -euo pipefail
 REF=${CMF_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -24,11 +25,11 @@ cmf_ctrl_boundary_mod cmf_ctrl_upsinf_mod cmf_ctrl_output_mod cmf_ctrl_restart_m
 cmf_ctrl_time_mod cmf_ctrl_maps_mod cmf_ctrl_vars_mod cmf_ctrl_nmlist_mod
 cmf_drv_control_mod cmf_drv_advance_mod"
 
-# build_variant NAME DEVICE_DIAG: DEVICE_DIAG=1 links this package's device-side
-# CMF_CALC_DIAG_MOD (compiled after the physics module it uses) in place of the
-# reference's
+# build_variant NAME DEVICE_DIAG MAIN: DEVICE_DIAG=1 links this package's
+# device-side CMF_CALC_DIAG_MOD (compiled after the physics module it uses) in
+# place of the reference's; MAIN is the main program's source
 build_variant() {
-  local name=$1 devdiag=$2 obj=$OUT/obj_$1 objs="" f o
+  local name=$1 devdiag=$2 main=$3 obj=$OUT/obj_$1 objs="" f o
   mkdir -p "$obj"
   for s in $SRCS; do
     case $s in
@@ -40,11 +41,11 @@ build_variant() {
     "$FC" $FFLAGS -module-dir "$obj" -c "$f" -o "$o"
     objs="$objs $o"
   done
-  "$FC" $FFLAGS -module-dir "$obj" -I "$obj" -c "$ROOT/oracle/ref_driver.F90" -o "$obj/ref_driver.o"
-  "$FC" $FFLAGS -o "$OUT/$name" "$obj/ref_driver.o" $objs \
+  "$FC" $FFLAGS -module-dir "$obj" -I "$obj" -c "$main" -o "$obj/main.o"
+  "$FC" $FFLAGS -o "$OUT/$name" "$obj/main.o" $objs \
     -L"$PKG/cmf" -lcmf_hip -Wl,-rpath,'$ORIGIN/../../cmf' -Wl,-rpath,/opt/rocm/lib
   echo "build_dropin: built $OUT/$name"
 }
 mkdir -p "$OUT"
-build_variant cmf_dropin 0
-build_variant cmf_dropin_devdiag 1
+build_variant cmf_dropin 0 "$ROOT/oracle/ref_driver.F90"
+build_variant cmf_dropin_devdiag 1 "$HERE/cmf_dropin_main.F90"

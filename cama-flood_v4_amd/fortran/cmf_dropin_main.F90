@@ -1,0 +1,64 @@
+!==========================================================
+! Stand-alone host program of the drop-in build (cmf_dropin_devdiag): the
+! reference's initialisation and time loop (CMF_DRV_INPUT / CMF_DRV_INIT /
+! CMF_DRV_ADVANCE / CMF_DRV_END, as src/MAIN_cmf.F90 drives them) with the
+! device physics and the device-side output diagnostics linked in.
+!
+! usage: cmf_dropin_devdiag <runoff.bin> [ignored arguments]
+!   runoff.bin : records of NX*NY float64 runoff [m3/s], one per forcing
+!                interval DTIN, repeated cyclically; assigned to D2RUNOFF of each
+!                cell directly (the file-based CMF_FORCING_GET/PUT and its
+!                interpolation are not part of this program)
+!   stdout     : "STEPTIME <step> <seconds>" per outer step (time inside
+!                CMF_DRV_ADVANCE), then "RUNTIME <seconds> <nsteps> <nseqall>"
+! Output and restart files are written by the reference's own writers as the
+! namelist (input_cmf.nam) asks.
+!==========================================================
+PROGRAM CMF_DROPIN_MAIN
+USE PARKIND1,                ONLY: JPIM, JPRD
+USE YOS_CMF_INPUT,           ONLY: NX, NY, DT, DTIN, LOUTPUT, IFRQ_OUT
+USE YOS_CMF_TIME,            ONLY: NSTEPS
+USE YOS_CMF_MAP,             ONLY: NSEQALL, I1SEQX, I1SEQY
+USE YOS_CMF_PROG,            ONLY: D2RUNOFF
+USE CMF_DRV_CONTROL_MOD,     ONLY: CMF_DRV_INPUT, CMF_DRV_INIT, CMF_DRV_END
+USE CMF_DRV_ADVANCE_MOD,     ONLY: CMF_DRV_ADVANCE
+IMPLICIT NONE
+CHARACTER(LEN=512)           :: CROF
+INTEGER(KIND=JPIM)           :: ISTEP, NPERREC, IREC, NREC, ISEQ, IU
+REAL(KIND=JPRD), ALLOCATABLE :: ROF(:,:)
+INTEGER(KIND=8)              :: T0, T1, RATE, TSUM, NBYTES
+
+CALL GET_COMMAND_ARGUMENT(1, CROF)
+CALL CMF_DRV_INPUT
+!! the output predicate of CMF_DRV_ADVANCE (drv_advance:121) takes MOD(JHOUR,
+!! IFRQ_OUT) even with LOUTPUT off, when the output namelist left IFRQ_OUT at 0
+IF( .NOT. LOUTPUT .AND. IFRQ_OUT <= 0 ) IFRQ_OUT = 24
+CALL CMF_DRV_INIT
+
+ALLOCATE(ROF(NX,NY))
+OPEN(NEWUNIT=IU, FILE=TRIM(CROF), FORM='UNFORMATTED', ACCESS='STREAM', STATUS='OLD')
+INQUIRE(UNIT=IU, SIZE=NBYTES)
+NREC = INT(NBYTES / (8_8*NX*NY), JPIM)
+IF( NREC < 1 ) STOP 'cmf_dropin_main: runoff file holds no NX*NY float64 record'
+NPERREC = MAX(1, INT(DTIN/DT, JPIM))
+
+TSUM = 0
+IREC = 0
+DO ISTEP=1, NSTEPS
+  IF( MOD(ISTEP-1, NPERREC) == 0 )THEN
+    IREC = MOD(IREC, NREC) + 1
+    READ(IU, POS=1_8 + INT(IREC-1,8)*8_8*NX*NY) ROF
+    DO ISEQ=1, NSEQALL
+      D2RUNOFF(ISEQ,1) = ROF(I1SEQX(ISEQ), I1SEQY(ISEQ))
+    END DO
+  ENDIF
+  CALL SYSTEM_CLOCK(T0, RATE)
+  CALL CMF_DRV_ADVANCE(1)
+  CALL SYSTEM_CLOCK(T1)
+  TSUM = TSUM + (T1 - T0)
+  WRITE(*,'(A,1X,I10,1X,ES16.8)') 'STEPTIME', ISTEP, REAL(T1-T0,8)/REAL(RATE,8)
+END DO
+WRITE(*,'(A,1X,ES16.8,1X,I10,1X,I10)') 'RUNTIME', REAL(TSUM,8)/REAL(RATE,8), NSTEPS, NSEQALL
+CLOSE(IU)
+CALL CMF_DRV_END
+END PROGRAM CMF_DROPIN_MAIN

@@ -1,0 +1,60 @@
+"""Host-side parts of bench.py that need no GPU: the reference CPU baseline run
+(the reference's OpenMP build, oracle/_ref/omp64, on a small 1-degree network
+with levees and dams) and the parsing of the test driver's per-step times and
+substep counts that the CPU baseline and the drop-in timing share; the HBM
+traffic figure read from the committed rocprofv3 summary."""
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "omp64", "cmf_ref")
+
+
+def _bench():
+    sys.path.insert(0, ROOT)
+    import bench
+    return bench
+
+
+def _net():
+    from cmf import synth
+    net = synth.make_network(360, 180, seed=5)
+    synth.add_levees(net, 0.03, 5, hmin=0.5, hmax=3.0, fmin=0.05, fmax=0.5)
+    synth.add_dams(net, 20, 5)
+    return net, synth.make_runoff(net, 2, 5, mean_mm_day=1.5)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference OpenMP build not built")
+def test_host_run_parsing():
+    bench = _bench()
+    net, ro = _net()
+    steps = 5
+    with tempfile.TemporaryDirectory() as d:
+        bench._reference_case(d, net, ro, 3600.0, steps)
+        out, times, nts = bench._run_host(REF, d, dict(os.environ, OMP_NUM_THREADS="2"))
+    assert len(times) == steps and all(t >= 0.0 for t in times)
+    assert len(nts) == steps and all(k >= 1 for k in nts)
+    line = [l for l in out.splitlines() if l.startswith("REFTIME")][0].split()
+    assert int(line[2]) == steps
+    assert abs(float(line[1]) - sum(times)) <= 1e-3 + 1e-6 * sum(times)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference OpenMP build not built")
+def test_cpu_baseline_fields():
+    bench = _bench()
+    net, ro = _net()
+    r = bench.cpu_baseline(net, ro, net.ncells, 3600.0, 4)
+    assert r is not None and r["value"] > 0 and r["kind"] == "reference"
+    assert r["unit"] == "cell-substeps/s" and r["cores"] >= 1
+    assert "4 outer steps of 3600 s" in r["sample"]
+
+
+def test_traffic_profile_is_read():
+    bench = _bench()
+    t = bench.measured_traffic("k_update")
+    assert t is not None and t > 284 * 1500120 * 0.5          # bytes per launch of the 1.5M-cell network
+    assert np.isfinite(t)
