@@ -94,6 +94,7 @@ HIP_FUNCS = {
     "cmf_hip_interp_runoff": (I, [H, P]),
     "cmf_hip_set_dwnelv": (I, [H, P]),
     "cmf_hip_set_upsinf": (I, [H, P]),
+    "cmf_hip_set_kinematic": (I, [H, C.c_int32]),
     "cmf_hip_restart_write": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32, C.c_double,
                                   C.c_double]),
     "cmf_hip_restart_read": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32]),

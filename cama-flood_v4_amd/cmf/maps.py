@@ -29,6 +29,7 @@ class Options:
     lpthout: bool = False
     ldamout: bool = False
     llevee: bool = False
+    lkine: bool = False           # kinematic-wave outflow (CMF_CALC_OUTFLW_KINE)
     pmanriv: float = 0.03
     pmanfld: float = 0.10
     pgrv: float = 9.8

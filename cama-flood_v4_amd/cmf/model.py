@@ -74,6 +74,8 @@ class CMFModel:
         self.h = h
         self._keep = []
         self._set_map(m)
+        if getattr(m.opts, "lkine", False):
+            self.set_kinematic(True)
 
     # ------------------------------------------------------------------
     def _check(self, rc):
@@ -241,6 +243,11 @@ class CMFModel:
         if a.size != self.members * self.map.n:
             raise CMFError(f"set_upsinf: {self.members * self.map.n} values expected")
         self._check(self.lib.cmf_hip_set_upsinf(self.h, a.ctypes.data))
+
+    def set_kinematic(self, on: bool) -> None:
+        """LKINE: kinematic-wave outflow (CMF_CALC_OUTFLW_KINE) instead of the
+        local inertial one; applied from maps.Options.lkine at construction."""
+        self._check(self.lib.cmf_hip_set_kinematic(self.h, int(bool(on))))
 
     def update_boundary(self, meansl: Optional[np.ndarray] = None, sealev: Optional[np.ndarray] = None) -> None:
         """D2DWNELV = D2ELEVTN (+ meansl) (+ sealev) on the device."""

@@ -327,7 +327,8 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
                            ifrq_inp: int, ndays: int, lpthout=False,
                            llevee=False, ldamout=False, ladpstp=True, lfldout=True,
                            pmanriv=0.03, pmanfld=0.10, pcadp=0.7, pdstmth=10000.0,
-                           restart: bool = False, outputs: Optional[str] = None) -> None:
+                           restart: bool = False, outputs: Optional[str] = None,
+                           lkine: bool = False) -> None:
     """Write the map files, tables, runoff records and input_cmf.nam that the
     reference needs (formats of src/cmf_ctrl_maps_mod.F90 READ_MAP_BIN,
     READ_TOPO_BIN, READ_BIFPARAM; src/cmf_ctrl_damout_mod.F90:141-179;
@@ -383,6 +384,7 @@ LFLDOUT  = {L(lfldout)}
 LPTHOUT  = {L(lpthout)}
 LDAMOUT  = {L(ldamout)}
 LLEVEE   = {L(llevee)}
+LKINE    = {L(lkine)}
 LRESTART = .FALSE.
 LOUTPUT  = {L(outputs is not None)}
 LSPAMAT  = .TRUE.

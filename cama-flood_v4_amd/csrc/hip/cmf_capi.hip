@@ -68,6 +68,7 @@ struct cmf_hip_ctx {
     OutState out{};                                  // output-period accumulators
     double* dwnelv_dev = nullptr;                    // D2DWNELV in the device map (boundary updates)
     int upsinf_on = 0;                               // LUPSINF switched on by cmf_hip_set_upsinf
+    int lkine = 0;                                   // LKINE set by cmf_hip_set_kinematic
     double *d_meansl = nullptr, *d_sealev = nullptr; // staging of cmf_hip_update_boundary
     double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
@@ -422,6 +423,7 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.budget = h->ninst == 1 && h->nmem == 1 && h->ndom == 1;   // the kernels keep budgets only then
     A.xcd = h->tune.xcd;
     A.lupsinf = h->upsinf_on;
+    A.lkine = h->lkine;
     return A;
 }
 
@@ -1225,6 +1227,18 @@ int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf) {
     }
     if (!h->upsinf_on) {
         h->upsinf_on = 1;
+        // captured substep graphs hold the old arguments
+        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+        h->graphs.clear();
+    }
+    return 0;
+}
+
+int cmf_hip_set_kinematic(cmf_handle h, int32_t lkine) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (lkine != 0 && lkine != 1) return fail(CMF_EINVAL, "lkine=%d must be 0 or 1", (int)lkine);
+    if (h->lkine != lkine) {
+        h->lkine = lkine;
         // captured substep graphs hold the old arguments
         for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
         h->graphs.clear();

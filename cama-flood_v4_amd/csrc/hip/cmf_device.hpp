@@ -145,6 +145,7 @@ struct StepArgs {
     int32_t budget;                            // write the per-tile water-budget sums
     int32_t xcd;                               // XCD-contiguous tile order
     int32_t lupsinf;                           // add S.upsinf to the river in STONXT
+    int32_t lkine;                             // kinematic-wave outflow (LKINE)
 };
 
 // Cell / path slot -> static index, member and instance. MB = false is the

@@ -76,6 +76,39 @@ __device__ __forceinline__ InstView inst_view(const InstStep* tab, int inst, int
 }
 
 // ---------------------------------------------------------------------------
+// Kinematic wave, CMF_CALC_OUTFLW_KINE (src/cmf_opt_outflw_mod.F90:27-127):
+// slope from the bed elevations over NXTDST, at least PMINSLP (mouths: PMINSLP,
+// the LSLOPEMOUTH = .FALSE. branch :86-91); velocity (1/n) S^(1/2) h^(2/3);
+// outflow capped by storage / DT; RIVVEL is the kinematic velocity. In R (FP64:
+// the reference's statements in order; mixed: FP32 flux arithmetic, the slope
+// formed in FP64 and the storage caps divided in FP64, each rounded once).
+template <class R>
+__device__ __forceinline__ void kine_outflow(const DevMap& M, const DevState<R>& S, const StepArgs& A, int i,
+                                             int nxt, int s, double dt, R& q_r, R& q_f, R& vel) {
+    const R one = R(1), zero = R(0);
+    double dsl = A.pminslp;
+    if (nxt >= 0) {                                                   // :57-58
+        dsl = ((double)M.elevtn[i] - (double)M.elevtn[nxt]) * (1.0 / (double)M.nxtdst[i]);
+        dsl = vmax(dsl, A.pminslp);
+    }
+    const R dslope = R(dsl);
+    const R rd = S.rivdph[s];
+    const R dvel = ((one / R(M.rivman[i])) * vsqrt(dslope)) * vpow(rd, R(K_2O3F));   // :59
+    const R darea = R(M.rivwth[i]) * rd;                             // :60
+    vel = dvel;
+    q_r = vmin(darea * dvel, R(S.rivsto[s] / dt));                   // :63-64
+    q_f = zero;
+    if (A.lfldout) {                                                  // :66-75
+        const R dslope_f = vmin(R(0.005), dslope);
+        const R fd = S.flddph[s];
+        const R dvel_f = ((one / R(A.pmanfld)) * vsqrt(dslope_f)) * vpow(fd, R(K_2O3F));
+        R dare_f = R(S.fldsto[s]) * (one / R(M.rivlen[i]));
+        dare_f = vmax(dare_f - fd * R(M.rivwth[i]), zero);
+        q_f = vmin(dare_f * dvel_f, R(S.fldsto[s] / dt));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // phase 1a: river / mouth / floodplain outflow and dam operation, per cell slot
 template <class R, bool MB>
 __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
@@ -109,123 +142,129 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     const R flddph_pre = vmax(rivdph_pre - R(f_rivhgt), zero);
     const R rivwth = R(f_rivwth), rivman = R(f_rivman);
     const bool river = nxt >= 0;                                     // mouths carry a negative code
-    // Water-surface terms of outflw:45-58. With R = double they are evaluated as
-    // the reference writes them (SFCELV = RIVELV + RIVDPH, slopes and depths from
-    // SFCELV). With R = float the same quantities are formed from depths and the
-    // exact FP64 elevation difference, rounded once: a ~250 m surface elevation
-    // keeps only ~1.5e-5 m in FP32, which the slope (a difference of a few cm)
-    // would otherwise inherit.
-    //   hs  = SFCELV - DWNELV          (slope numerator)
-    //   hr  = max(SFCELV, DWNELV) - RIVELV,  hrp the same with the _PRE values
-    //   (mouths: hr = RIVDPH, hrp = RIVDPH_PRE)
-    //   fa  = surface above ELEVTN for the floodplain (outflw:88, :147), fap the _PRE one
-    R hs, hr, hrp, fa, fap, rdst;
-    if (sizeof(R) == sizeof(double)) {
-        const R rivelv = R((double)f_elevtn - (double)f_rivhgt);    // maps:722
-        const R elevtn = R(f_elevtn);
-        const R sfcelv = rivelv + rivdph;                           // outflw:45-47
-        const R sfcelv_pre = rivelv + rivdph_pre;
-        if (river) {                                                 // outflw:52-57
-            const int j = nxt, sj = sidx<MB>(M, j, m);
-            const R rivelv_j = R(rivelv_of(M, j));
-            const R dwn = rivelv_j + S.rivdph[sj];
-            const R dwn_pre = rivelv_j + S.rivdph_pre[sj];
-            const R dsfc = vmax(sfcelv, dwn), dsfc_pr = vmax(sfcelv_pre, dwn_pre);
-            hs = sfcelv - dwn;
-            hr = dsfc - rivelv;
-            hrp = dsfc_pr - rivelv;
-            fa = dsfc - elevtn;
-            fap = dsfc_pr - elevtn;
-            rdst = one / R(f_nxtdst);
-        } else {
-            const R dwn = R(M.dwnelv[i]);
-            hs = sfcelv - dwn;
-            hr = rivdph;
-            hrp = rivdph_pre;
-            fa = sfcelv - elevtn;
-            fap = sfcelv_pre - elevtn;
-            rdst = one / R(A.pdstmth);
-        }
-    } else {
-        const double e_i = (double)f_elevtn - (double)f_rivhgt;
-        if (river) {
-            const int j = nxt, sj = sidx<MB>(M, j, m);
-            const R dz = R(rivelv_of(M, j) - e_i);                   // RIVELV(j) - RIVELV(i)
-            const R dj = S.rivdph[sj], dj_pre = S.rivdph_pre[sj];
-            hs = (rivdph - dj) - dz;
-            hr = vmax(rivdph, dz + dj);
-            hrp = vmax(rivdph_pre, dz + dj_pre);
-            rdst = one / R(f_nxtdst);
-        } else {
-            hs = rivdph - R(M.dwnelv[i] - e_i);
-            hr = rivdph;
-            hrp = rivdph_pre;
-            rdst = one / R(A.pdstmth);
-        }
-        const R hgt = R(f_rivhgt);                                   // ELEVTN = RIVELV + RIVHGT
-        fa = hr - hgt;
-        fap = hrp - hgt;
-    }
-    // ---- river channel: outflw:60-80 (river), :119-139 (mouth)
     R q_r, q_f = zero, vel;
-    {
-        const R dslp = hs * rdst;
-        const R dflw = hr, dflw_pr = hrp;
-        R dare = rivwth * dflw;
-        dare = vmax(dare, R(1.e-10));
-        R dflw_im = vsqrt(dflw * dflw_pr);
-        dflw_im = vmax(dflw_im, R(1.e-6));
-        const R dout_pr = qr_pre * (one / rivwth);
-        const R dout = (rivwth * (dout_pr + (gdt * dflw_im) * dslp)) *
-                       (one / (one + ((gdt * (rivman * rivman)) * vabs(dout_pr)) * vpow(dflw_im, R(K_P7O3))));
-        const R dvel = qr_old * (one / dare);
-        const bool mask = (dflw_im > R(K_1EM5F)) && (dare > R(K_1EM5F));
-        q_r = mask ? dout : zero;
-        vel = mask ? dvel : zero;
-    }
-    // ---- floodplain: outflw:84-116 (river), :143-171 (mouth). Surface heights
-    // above ELEVTN = RIVELV + RIVHGT: max(SFCELV, DWNELV) - ELEVTN = hr - RIVHGT.
-    if (A.lfldout) {
-        const R gdtn2 = gdt * (R(A.pmanfld) * R(A.pmanfld));
-        const R dfsto = R(fldsto_d);
-        const R rrl = one / R(f_rivlen);
-        R dslp = hs * rdst;
-        dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
-        R dflw, dflw_pr, dare, dare_pr;
-        if (river) {
-            dflw = vmax(fa, zero);
-            dare = dfsto * rrl;
-            dare = vmax(dare - flddph * rivwth, zero);
-            dflw_pr = fap;
-            dare_pr = fldsto_pre * rrl;
-            dare_pr = vmax(dare_pr - flddph_pre * rivwth, R(1.e-6));
+    if (A.lkine) {
+        // LKINE: CMF_CALC_OUTFLW_KINE (opt_outflw:27-127) in place of the local
+        // inertial equations and their reverse-flow limiter
+        kine_outflow<R>(M, S, A, i, nxt, s, I.dt, q_r, q_f, vel);
+    } else {
+        // Water-surface terms of outflw:45-58. With R = double they are evaluated as
+        // the reference writes them (SFCELV = RIVELV + RIVDPH, slopes and depths from
+        // SFCELV). With R = float the same quantities are formed from depths and the
+        // exact FP64 elevation difference, rounded once: a ~250 m surface elevation
+        // keeps only ~1.5e-5 m in FP32, which the slope (a difference of a few cm)
+        // would otherwise inherit.
+        //   hs  = SFCELV - DWNELV          (slope numerator)
+        //   hr  = max(SFCELV, DWNELV) - RIVELV,  hrp the same with the _PRE values
+        //   (mouths: hr = RIVDPH, hrp = RIVDPH_PRE)
+        //   fa  = surface above ELEVTN for the floodplain (outflw:88, :147), fap the _PRE one
+        R hs, hr, hrp, fa, fap, rdst;
+        if (sizeof(R) == sizeof(double)) {
+            const R rivelv = R((double)f_elevtn - (double)f_rivhgt);    // maps:722
+            const R elevtn = R(f_elevtn);
+            const R sfcelv = rivelv + rivdph;                           // outflw:45-47
+            const R sfcelv_pre = rivelv + rivdph_pre;
+            if (river) {                                                 // outflw:52-57
+                const int j = nxt, sj = sidx<MB>(M, j, m);
+                const R rivelv_j = R(rivelv_of(M, j));
+                const R dwn = rivelv_j + S.rivdph[sj];
+                const R dwn_pre = rivelv_j + S.rivdph_pre[sj];
+                const R dsfc = vmax(sfcelv, dwn), dsfc_pr = vmax(sfcelv_pre, dwn_pre);
+                hs = sfcelv - dwn;
+                hr = dsfc - rivelv;
+                hrp = dsfc_pr - rivelv;
+                fa = dsfc - elevtn;
+                fap = dsfc_pr - elevtn;
+                rdst = one / R(f_nxtdst);
+            } else {
+                const R dwn = R(M.dwnelv[i]);
+                hs = sfcelv - dwn;
+                hr = rivdph;
+                hrp = rivdph_pre;
+                fa = sfcelv - elevtn;
+                fap = sfcelv_pre - elevtn;
+                rdst = one / R(A.pdstmth);
+            }
         } else {
-            dflw = fa;
-            dare = vmax(dfsto * rrl - flddph * rivwth, zero);
-            dflw_pr = fap;
-            dare_pr = vmax(fldsto_pre * rrl - flddph_pre * rivwth, R(1.e-6));
+            const double e_i = (double)f_elevtn - (double)f_rivhgt;
+            if (river) {
+                const int j = nxt, sj = sidx<MB>(M, j, m);
+                const R dz = R(rivelv_of(M, j) - e_i);                   // RIVELV(j) - RIVELV(i)
+                const R dj = S.rivdph[sj], dj_pre = S.rivdph_pre[sj];
+                hs = (rivdph - dj) - dz;
+                hr = vmax(rivdph, dz + dj);
+                hrp = vmax(rivdph_pre, dz + dj_pre);
+                rdst = one / R(f_nxtdst);
+            } else {
+                hs = rivdph - R(M.dwnelv[i] - e_i);
+                hr = rivdph;
+                hrp = rivdph_pre;
+                rdst = one / R(A.pdstmth);
+            }
+            const R hgt = R(f_rivhgt);                                   // ELEVTN = RIVELV + RIVHGT
+            fa = hr - hgt;
+            fap = hrp - hgt;
         }
-        const R t = dflw * dflw_pr;
-        R dflw_im = vsqrt(vmax(t, zero));
-        dflw_im = vmax(dflw_im, R(1.e-6));
-        R dare_im = vsqrt(dare * dare_pr);
-        dare_im = vmax(dare_im, R(1.e-6));
-        const R dout_pr = fldout_pre;
-        const R dout = (dout_pr + (gdt * dare_im) * dslp) *
-                       (one / (one + ((gdtn2 * vabs(dout_pr)) * vpow(dflw_im, R(K_P4O3))) * (one / dare_im)));
-        const R thr = river ? R(1.e-5) : R(K_1EM5F);
-        const bool mask = (dflw_im > thr) && (dare > thr);
-        const R f = mask ? dout : zero;
-        q_f = (f * q_r > zero) ? f : zero;
-    }
-    // ---- reverse-flow storage limiter, river cells: outflw:175-182
-    if (river) {
-        R dout = ((-q_r) - q_f) * R(I.dt);
-        dout = vmax(dout, R(K_1EM10F));
-        R rate = (R(0.05) * storge) / dout;
-        rate = vmin(rate, one);
-        q_r = q_r * rate;
-        q_f = q_f * rate;
+        // ---- river channel: outflw:60-80 (river), :119-139 (mouth)
+        {
+            const R dslp = hs * rdst;
+            const R dflw = hr, dflw_pr = hrp;
+            R dare = rivwth * dflw;
+            dare = vmax(dare, R(1.e-10));
+            R dflw_im = vsqrt(dflw * dflw_pr);
+            dflw_im = vmax(dflw_im, R(1.e-6));
+            const R dout_pr = qr_pre * (one / rivwth);
+            const R dout = (rivwth * (dout_pr + (gdt * dflw_im) * dslp)) *
+                           (one / (one + ((gdt * (rivman * rivman)) * vabs(dout_pr)) * vpow(dflw_im, R(K_P7O3))));
+            const R dvel = qr_old * (one / dare);
+            const bool mask = (dflw_im > R(K_1EM5F)) && (dare > R(K_1EM5F));
+            q_r = mask ? dout : zero;
+            vel = mask ? dvel : zero;
+        }
+        // ---- floodplain: outflw:84-116 (river), :143-171 (mouth). Surface heights
+        // above ELEVTN = RIVELV + RIVHGT: max(SFCELV, DWNELV) - ELEVTN = hr - RIVHGT.
+        if (A.lfldout) {
+            const R gdtn2 = gdt * (R(A.pmanfld) * R(A.pmanfld));
+            const R dfsto = R(fldsto_d);
+            const R rrl = one / R(f_rivlen);
+            R dslp = hs * rdst;
+            dslp = vmax(R(-0.005), vmin(R(0.005), dslp));
+            R dflw, dflw_pr, dare, dare_pr;
+            if (river) {
+                dflw = vmax(fa, zero);
+                dare = dfsto * rrl;
+                dare = vmax(dare - flddph * rivwth, zero);
+                dflw_pr = fap;
+                dare_pr = fldsto_pre * rrl;
+                dare_pr = vmax(dare_pr - flddph_pre * rivwth, R(1.e-6));
+            } else {
+                dflw = fa;
+                dare = vmax(dfsto * rrl - flddph * rivwth, zero);
+                dflw_pr = fap;
+                dare_pr = vmax(fldsto_pre * rrl - flddph_pre * rivwth, R(1.e-6));
+            }
+            const R t = dflw * dflw_pr;
+            R dflw_im = vsqrt(vmax(t, zero));
+            dflw_im = vmax(dflw_im, R(1.e-6));
+            R dare_im = vsqrt(dare * dare_pr);
+            dare_im = vmax(dare_im, R(1.e-6));
+            const R dout_pr = fldout_pre;
+            const R dout = (dout_pr + (gdt * dare_im) * dslp) *
+                           (one / (one + ((gdtn2 * vabs(dout_pr)) * vpow(dflw_im, R(K_P4O3))) * (one / dare_im)));
+            const R thr = river ? R(1.e-5) : R(K_1EM5F);
+            const bool mask = (dflw_im > thr) && (dare > thr);
+            const R f = mask ? dout : zero;
+            q_f = (f * q_r > zero) ? f : zero;
+        }
+        // ---- reverse-flow storage limiter, river cells: outflw:175-182
+        if (river) {
+            R dout = ((-q_r) - q_f) * R(I.dt);
+            dout = vmax(dout, R(K_1EM10F));
+            R rate = (R(0.05) * storge) / dout;
+            rate = vmin(rate, one);
+            q_r = q_r * rate;
+            q_f = q_f * rate;
+        }
     }
     // ---- reservoir operation: damout:316-498 (Yamazaki & Funato rule)
     if (A.ldamout) {

@@ -193,6 +193,14 @@ int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv);
  * the handle; call again whenever D2UPSINF changes. */
 int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf);
 int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev);
+/* LKINE (namelist NRUNVER): 1 replaces the local inertial river and floodplain
+ * outflow and its reverse-flow limiter by the kinematic wave of
+ * CMF_CALC_OUTFLW_KINE (src/cmf_opt_outflw_mod.F90:27-127, selected at
+ * src/cmf_ctrl_physics_mod.F90:66-72): bed slope ELEVTN(i)-ELEVTN(next) over
+ * NXTDST, at least PMINSLP (mouths PMINSLP: LSLOPEMOUTH off), outflow capped by
+ * storage / DT. 0 (the default) restores the local inertial scheme. Dams, paths,
+ * the limiter of the inflow and the storage update are unchanged. */
+int cmf_hip_set_kinematic(cmf_handle h, int32_t lkine);
 
 /* One output record of variable `var` appended to `file`, as CMF_OUTPUT_WRITE
  * writes it in binary mode (src/cmf_ctrl_output_mod.F90:474-664): float32,

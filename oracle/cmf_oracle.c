@@ -754,6 +754,40 @@ void cmfo_diag_getave_adpstp(cmfo_model* m)
 }
 
 /* ------------------------------------------------------------------------
+ * CMF_CALC_OUTFLW_KINE  src/cmf_opt_outflw_mod.F90:27-127 (LKINE): kinematic
+ * wave with the bed slope (ELEVTN difference over NXTDST, at least PMINSLP;
+ * mouths PMINSLP, the LSLOPEMOUTH = .FALSE. branch :86-91), outflow capped by
+ * storage / DT, no reverse-flow limiter. RIVVEL is the kinematic velocity. */
+void cmfo_calc_outflw_kine(cmfo_model* m, double dt)
+{
+    const int n = m->n, nr = m->nseqriv;
+    for (int i = 0; i < n; ++i) m->sfcelv[i] = m->rivelv[i] + m->rivdph[i];   /* :44-48 */
+    for (int i = 0; i < n; ++i) {
+        double dslope;
+        if (i < nr) {                                                       /* :53-78 */
+            const int j = m->next[i];
+            dslope = (m->elevtn[i] - m->elevtn[j]) * (1.0 / m->nxtdst[i]);   /* :57 */
+            dslope = DMAX(dslope, m->pminslp);                              /* :58 */
+        } else {                                                            /* :85-123 */
+            dslope = m->pminslp;
+        }
+        const double dvel = ((1.0 / m->rivman[i]) * sqrt(dslope)) * pow(m->rivdph[i], F2O3);
+        const double darea = m->rivwth[i] * m->rivdph[i];
+        m->rivvel[i] = dvel;
+        const double q = darea * dvel;
+        m->rivout[i] = DMIN(q, m->rivsto[i] / dt);
+        if (m->lfldout) {                                                   /* :66-75, :105-114 */
+            const double dslope_f = DMIN(0.005, dslope);
+            const double dvel_f = ((1.0 / m->pmanfld) * sqrt(dslope_f)) * pow(m->flddph[i], F2O3);
+            double dare_f = m->fldsto[i] * (1.0 / m->rivlen[i]);
+            dare_f = DMAX(dare_f - m->flddph[i] * m->rivwth[i], 0.0);
+            const double qf = dare_f * dvel_f;
+            m->fldout[i] = DMIN(qf, m->fldsto[i] / dt);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
  * CMF_PHYSICS_ADVANCE  src/cmf_ctrl_physics_mod.F90:21-155 */
 int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
 {
@@ -764,7 +798,8 @@ int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
     cmfo_diag_reset_adpstp(m);                                              /* :61 */
     const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
     for (int32_t it = 0; it < nrun; ++it) {                                 /* :64-144 */
-        cmfo_calc_outflw(m, *dt);
+        if (m->lkine) cmfo_calc_outflw_kine(m, *dt);                       /* :66-72 */
+        else cmfo_calc_outflw(m, *dt);
         if (!m->lfldout)
             for (int i = 0; i < m->n; ++i) { m->fldout[i] = 0.0; m->fldout_pre[i] = 0.0; }
         if (m->ldamout) cmfo_damout_calc(m, *dt);

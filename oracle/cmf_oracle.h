@@ -71,6 +71,9 @@ typedef struct cmfo_model {
     /* LUPSINF: upstream inflow D2UPSINF [m3/s] added to the river in STONXT
      * (src/cmf_calc_stonxt_mod.F90:81); NULL = off */
     const double* upsinf;
+    /* LKINE: kinematic-wave outflow (CMF_CALC_OUTFLW_KINE) in place of the local
+     * inertial one; 0 = off */
+    int32_t lkine;
 } cmfo_model;
 
 /* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */
@@ -85,6 +88,7 @@ void cmfo_calc_fldstg_def(cmfo_model* m);
 void cmfo_levee_fldstg(cmfo_model* m);
 int32_t cmfo_calc_adpstp(cmfo_model* m, double dt_def, double* dt);
 void cmfo_calc_outflw(cmfo_model* m, double dt);
+void cmfo_calc_outflw_kine(cmfo_model* m, double dt);
 void cmfo_calc_pthout(cmfo_model* m, double dt);
 void cmfo_levee_opt_pthout(cmfo_model* m, double dt);
 void cmfo_damout_calc(cmfo_model* m, double dt);

@@ -42,7 +42,10 @@ GOLDEN = {
     "g_allwet": (90, 45, 15, 4, 86400.0, True, True, True, 60.0, 0.5),
     "g_hourly": (90, 45, 16, 1, 3600.0, True, True, True, 20.0, 0.3),
     "c1_1deg":  (360, 180, 2026, 30, 86400.0, False, False, False, 1.5, 0.0),
+    "g_kine":   (90, 45, 17, 4, 86400.0, False, False, True, 20.0, 0.0),
 }
+# cases run with LKINE = .TRUE. (kinematic-wave outflow, src/cmf_opt_outflw_mod.F90:27-127)
+KINE_CASES = ("g_kine",)
 FINAL_KEYS = ["rivsto", "fldsto", "rivout", "fldout", "outflw", "rivvel"]
 
 
@@ -107,7 +110,7 @@ def make(name):
     with tempfile.TemporaryDirectory() as d:
         ifrq = int(dt // 3600)
         synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=ifrq, ndays=days,
-                                     lpthout=lpth, llevee=llev, ldamout=ldam)
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, lkine=name in KINE_CASES)
         r = subprocess.run([REF, "runoff.bin", "dump.bin", "1"], cwd=d, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
@@ -120,7 +123,7 @@ def make(name):
         keys_map += list(refdump.LEV_D)
     if ldam:
         keys_map += ["i1dam"]
-    out = dict(case=name, spec=GOLDEN[name], n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
+    out = dict(case=name, spec=GOLDEN[name], lkine=name in KINE_CASES, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
                map_sha={k: sha(mp[k]) for k in keys_map}, steps=[])
     skeys = refdump.STATE_P + refdump.STATE_D + (["levsto", "levdph"] if llev else []) + \
         (["damsto", "daminf"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])

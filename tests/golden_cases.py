@@ -13,7 +13,7 @@ from cmf import maps
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GDIR = os.path.join(HERE, "golden")
-NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg"]
+NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg", "g_kine"]
 
 
 def sha(a):
@@ -41,7 +41,7 @@ def setup(name):
     from oracle.oracle import OracleModel
     net, ro, ref, fin = load(name)
     nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
-    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam)
+    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)))
     m = maps.build_map(net, opts)
     st = maps.cold_start(m)
     om = OracleModel(m, st)

@@ -33,7 +33,7 @@ OUTPUT_VARS = ("rivout,fldout,outflw,rivvel,pthout,runoff,maxflw,maxdph,maxsto,r
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["g_allwet", "g_hourly", "c1_1deg"])
+@pytest.mark.parametrize("name", ["g_allwet", "g_hourly", "c1_1deg", "g_kine"])
 def test_dropin_matches_reference(name, tmp_path):
     if not os.path.exists(EXE):
         pytest.skip("drop-in binary not built")
@@ -44,7 +44,7 @@ def test_dropin_matches_reference(name, tmp_path):
     nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
     d = str(tmp_path)
     synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
-                                 lpthout=lpth, llevee=llev, ldamout=ldam)
+                                 lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)))
     # the test driver reads the state after arbitrary steps (its dumps): whole
     # state downloaded after every step
     env = dict(os.environ, CMF_HIP_DOWNLOAD="every")
