@@ -110,7 +110,11 @@ __device__ __forceinline__ void kine_outflow(const DevMap& M, const DevState<R>&
 
 // ---------------------------------------------------------------------------
 // phase 1a: river / mouth / floodplain outflow and dam operation, per cell slot
-template <class R, bool MB>
+// KINE selects the kinematic wave at compile time: k_outflow is instantiated per
+// scheme and the launch picks the one StepArgs::lkine asks for, so the default
+// (local inertial) kernel is compiled as if the option did not exist; the
+// persistent k_substep carries both and branches once per tile.
+template <class R, bool MB, bool KINE>
 __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
     const Slot x = cell_slot<MB>(M, s);
     const int i = x.c, m = x.m;
@@ -143,7 +147,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     const R rivwth = R(f_rivwth), rivman = R(f_rivman);
     const bool river = nxt >= 0;                                     // mouths carry a negative code
     R q_r, q_f = zero, vel;
-    if (A.lkine) {
+    if constexpr (KINE) {
         // LKINE: CMF_CALC_OUTFLW_KINE (opt_outflw:27-127) in place of the local
         // inertial equations and their reverse-flow limiter
         kine_outflow<R>(M, S, A, i, nxt, s, I.dt, q_r, q_f, vel);
@@ -696,12 +700,12 @@ __device__ __forceinline__ void update_path(const DevMap& M, const DevState<R>& 
 
 // ---------------------------------------------------------------------------
 // tiles: a tile is BS consecutive slots; cell tiles first, then path tiles
-template <class R, int BS, bool MB>
+template <class R, int BS, bool MB, bool KINE>
 __device__ __forceinline__ void outflow_tile(const DevMap& M, const DevState<R>& S, const StepArgs& A, int t,
                                              int ntc) {
     if (t < ntc) {
         const int s = t * BS + threadIdx.x;
-        if (s < M.ns) outflow_cell<R, MB>(M, S, A, s);
+        if (s < M.ns) outflow_cell<R, MB, KINE>(M, S, A, s);
     } else {
         const int q = (t - ntc) * BS + threadIdx.x;
         if (q < M.nsp) pthflow_path<R, MB>(M, S, A, q);
@@ -741,10 +745,10 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
 // The mixed-precision ensemble outflow is held to 64 VGPRs (8 waves per SIMD;
 // it needs 66 without the bound, 7 waves) and spills nothing; the single-model
 // forms would spill and keep the compiler's choice.
-template <class R, int BS, bool MB>
+template <class R, int BS, bool MB, bool KINE>
 __global__ __launch_bounds__(BS, (MB && sizeof(R) == 4) ? 8 : 1) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
-    if (t < nt_all) outflow_tile<R, BS, MB>(M, S, A, t, ntc);
+    if (t < nt_all) outflow_tile<R, BS, MB, KINE>(M, S, A, t, ntc);
 }
 
 template <class R, int BS, bool MB>
@@ -812,7 +816,10 @@ __global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArg
     __shared__ double lds[NGLB * (BS / 64)];
     int t0, dt, te;
     tile_range(blockIdx.x, gridDim.x, nt_o, A.xcd, t0, dt, te);
-    for (int t = t0; t < te; t += dt) outflow_tile<R, BS, MB>(M, S, A, t, ntc_o);
+    for (int t = t0; t < te; t += dt) {
+        if (A.lkine) outflow_tile<R, BS, MB, true>(M, S, A, t, ntc_o);
+        else outflow_tile<R, BS, MB, false>(M, S, A, t, ntc_o);
+    }
     grid_barrier(S.bar, S.err);
     tile_range(blockIdx.x, gridDim.x, ntc_r, A.xcd, t0, dt, te);
     for (int t = t0; t < te; t += dt) rate_tile<R, BS, MB>(M, S, A, t);
@@ -1023,8 +1030,12 @@ template <class R, int BS, bool MB>
 static void launch_phases_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, hipStream_t st) {
     const int ntc = tiles_cells(M, BS), ntp = tiles_paths(M, A, BS);
     if (phase == 0)
-        hipLaunchKernelGGL((k_outflow<R, BS, MB>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S, A, ntc,
-                           ntc + ntp);
+        if (A.lkine)
+            hipLaunchKernelGGL((k_outflow<R, BS, MB, true>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S,
+                               A, ntc, ntc + ntp);
+        else
+            hipLaunchKernelGGL((k_outflow<R, BS, MB, false>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S,
+                               A, ntc, ntc + ntp);
     else if (phase == 1)
         hipLaunchKernelGGL((k_rate<R, BS, MB>), dim3(pad8(ntc, A.xcd)), dim3(BS), 0, st, M, S, A, ntc);
     else
