@@ -58,3 +58,12 @@ def test_traffic_profile_is_read():
     t = bench.measured_traffic("k_update")
     assert t is not None and t > 284 * 1500120 * 0.5          # bytes per launch of the 1.5M-cell network
     assert np.isfinite(t)
+
+
+def test_more_than_one_gpu_is_refused():
+    """bench.py measures one GPU; --gpus N > 1 ends with a message and status 2."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
