@@ -8,7 +8,17 @@
  * It restates, function by function, the reference Fortran built in FP64 mode
  * (no -DSinglePrec_CMF), with the default namelist options that the device
  * path uses (LSPAMAT, local-inertial outflow, optional LFLDOUT/LPTHOUT/LDAMOUT/
- * LLEVEE/LADPSTP). Every function cites the reference file:line it follows.
+ * LLEVEE/LADPSTP/LKINE/LUPSINF). Every function cites the reference file:line it
+ * follows.
+ *
+ * Pinning: the physics (every function called by cmfo_physics_advance and
+ * cmfo_physics_fldstg, the kinematic wave included) is bit-identical to the
+ * reference's own run on the golden fixtures (every state array of every outer
+ * step, NT, budget sums; tests/test_oracle_golden.py); cmfo_roff_interp and the
+ * output diagnostics are bit-identical to the reference routines compiled from
+ * their sources (tests/test_forcing.py, tests/test_output_diag.py); the upstream
+ * inflow term is pinned by the one-substep storage difference it must produce
+ * (tests/test_steady_state.py).
  * Arithmetic keeps the reference's operation order and constant kinds: default-kind
  * literals such as 1.E-5 are the float value widened to double; x**(-1.) is 1/x,
  * x**0.5 is sqrt(x), x**(-0.5) is 1/sqrt(x), x**2 is x*x and x**3 is x*(x*x) (as
