@@ -109,13 +109,13 @@ def run_gpu(net, m, rvec, precision, steps, warmup, dt, tune="auto"):
     return sum_nt, t_total, kt, tune_used
 
 
-def _reference_case(d, net, ro, dt, steps):
-    """Reference inputs of the workload (levees, dams) in directory d, the run
-    shortened to `steps` outer steps."""
+def _reference_case(d, net, ro, dt, steps, llevee=True, ldamout=True):
+    """Reference inputs of the workload (levees, dams unless switched off) in
+    directory d, the run shortened to `steps` outer steps."""
     from cmf import synth
     days = max(1, int(np.ceil(steps * dt / 86400.0)))
     synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
-                                 llevee=True, ldamout=True)
+                                 llevee=llevee, ldamout=ldamout)
     # end time = start + steps*dt
     nml = open(os.path.join(d, "input_cmf.nam")).read()
     import datetime as _dt
@@ -185,6 +185,33 @@ def cpu_baseline(net, ro, n_cells, dt, sample_steps, variant="omp64"):
                     sample=f"reference {'FP64' if variant == 'omp64' else 'single-precision'} OpenMP "
                            f"(flang -O3 -fopenmp), {nsteps} outer steps of "
                            f"{dt:.0f} s, {sum(nts)} substeps over {n} cells, {t:.1f} s")
+
+
+def ensemble_cpu_baseline(nets, seed, steps=2):
+    """The reference's OpenMP build on the largest ensemble domain for `steps`
+    daily steps. The reference has no batched mode: an ensemble runs its
+    instances one after another, so its member-cell-substeps/s is its
+    cell-substeps/s on a domain."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "omp64", "cmf_ref")
+    if not os.path.exists(exe):
+        return None
+    from cmf import synth
+    net = max(nets, key=lambda x: x.ncells)
+    ro = synth.make_runoff(net, steps, seed)
+    ncores = len(os.sched_getaffinity(0))
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        _reference_case(d, net, ro, 86400.0, steps, llevee=False, ldamout=False)
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores",
+                   OMP_STACKSIZE="512M")
+        res = _run_host(exe, d, env)
+        if res is None:
+            return None
+        _out, times, nts = res
+    cs = net.ncells * sum(nts)
+    return dict(value=cs / sum(times), unit="member-cell-substeps/s", cores=ncores, kind="reference",
+                sample=f"reference FP64 OpenMP (flang -O3 -fopenmp) on the largest domain, {net.ncells} cells, "
+                       f"{len(times)} daily steps, {sum(nts)} substeps, {sum(times):.1f} s; instances run one "
+                       f"after another in the reference")
 
 
 def dropin_section(net, ro, n_cells, dt, steps):
@@ -404,6 +431,15 @@ def ensemble_section(a):
             rel_max.append(float(np.max(np.abs(a_ - b_)) / np.max(np.abs(b_))))
     r = res["mixed"]
     rd = res["fp64"]
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = ensemble_cpu_baseline(nets, a.seed)
+            if cpu:
+                log(f"ensemble CPU reference: {cpu['value']:.4g} member-cell-substeps/s on {cpu['cores']} cores; "
+                    f"mixed ensemble / CPU = {r['work'] / r['t'] / cpu['value']:.0f}x")
+        except Exception as e:          # a reported baseline only: never fails the run
+            log(f"ensemble CPU baseline failed: {e!r}")
     return dict(
         metric="member-cell-substeps/s", value=r["work"] / r["t"], unit="member-cell-substeps/s",
         precision="mixed", domains=len(ms), members_per_domain=ENS_MEMBERS, instances=r["ninst"],
@@ -417,6 +453,7 @@ def ensemble_section(a):
               "latency_ms": {"p50": float(np.percentile(rd["lat"], 50)),
                              "p99": float(np.percentile(rd["lat"], 99))}, "tune": rd["tune"]},
         mixed_vs_fp64_storage={"total_rel_max": float(max(rel_tot)), "cell_rel_max": float(max(rel_max))},
+        cpu_baseline=cpu,
     )
 
 

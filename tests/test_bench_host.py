@@ -67,3 +67,20 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference OpenMP build not built")
+def test_ensemble_cpu_baseline_on_regional_domains():
+    """The ensemble's CPU baseline runs the reference on the largest regional
+    domain (the form bench.py's ensemble domains take) and reports its rate."""
+    bench = _bench()
+    from cmf import synth
+    nets = []
+    for d, (nx, ny) in enumerate([(60, 40), (120, 80)]):
+        north = 70.0 - 8.0 * d
+        nets.append(synth.make_network(nx, ny, seed=2026 + 101 * d, land_frac=0.6, west=-180.0 + 20.0 * d,
+                                       east=-180.0 + 20.0 * d + 0.1 * nx, north=north, south=north - 0.1 * ny,
+                                       regional=True))
+    r = bench.ensemble_cpu_baseline(nets, 2026)
+    assert r is not None and r["value"] > 0 and r["unit"] == "member-cell-substeps/s"
+    assert f"{max(n.ncells for n in nets)} cells" in r["sample"]
