@@ -214,6 +214,31 @@ def ensemble_cpu_baseline(nets, seed, steps=2):
                        f"after another in the reference")
 
 
+def spinup_cpu_baseline(net, m, clim, days=30):
+    """The reference's OpenMP build on the spin-up network under the first
+    month's climatological runoff for `days` daily steps, from its cold start;
+    reported as seconds per simulated year (the sample scaled by 365 / days)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "omp64", "cmf_ref")
+    if not os.path.exists(exe):
+        return None
+    grid = np.zeros((1, net.ny, net.nx))
+    grid[0, m.seqy - 1, m.seqx - 1] = clim.monthly[0]
+    ncores = len(os.sched_getaffinity(0))
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        _reference_case(d, net, grid, 86400.0, days, llevee=False, ldamout=False)
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="spread", OMP_PLACES="cores",
+                   OMP_STACKSIZE="512M")
+        res = _run_host(exe, d, env)
+        if res is None:
+            return None
+        _out, times, nts = res
+    return dict(value=sum(times) * 365.0 / len(times), unit="s per simulated year", cores=ncores,
+                kind="reference",
+                sample=f"reference FP64 OpenMP (flang -O3 -fopenmp), {m.n} cells, {len(times)} daily steps "
+                       f"({sum(nts)} substeps) from the cold start under the first month's runoff, "
+                       f"{sum(times):.1f} s, scaled to 365 days")
+
+
 def dropin_section(net, ro, n_cells, dt, steps):
     """The Fortran drop-in end to end: the reference's host (serial flang build:
     time loop, forcing, diagnostics, log) with the device physics and the
@@ -517,7 +542,18 @@ def spinup_section(a):
         inference={"cells": big.shape[0], "ms": round(1e3 * t_inf, 2),
                    "tflops_incl_transfers": round(flops / t_inf / 1e12, 3)},
         spinup_seconds=cmp_["seconds"],
+        member_years=4 * cmp_["years_run"],
     )
+    if not a.no_cpu:
+        try:
+            cpu = spinup_cpu_baseline(net, m, clim)
+            out["cpu_baseline"] = cpu
+            if cpu:
+                gpu_y = cmp_["seconds"] / (4 * cmp_["years_run"])
+                log(f"spin-up CPU reference: {cpu['value']:.1f} s per simulated year on {cpu['cores']} cores; "
+                    f"device {gpu_y:.2f} s per member-year in the 4-member run")
+        except Exception as e:          # a reported baseline only: never fails the run
+            log(f"spin-up CPU baseline failed: {e!r}")
     log(f"spin-up: cold {c['years']} y, mean-flow steady {sd['years']} y, physics prior {pr['years']} y, "
         f"warm {w['years']} y; first-year change cold {c['annual_change'][0]:.3e} steady "
         f"{sd['annual_change'][0]:.3e} prior {pr['annual_change'][0]:.3e} warm {w['annual_change'][0]:.3e}")

@@ -84,3 +84,18 @@ def test_ensemble_cpu_baseline_on_regional_domains():
     r = bench.ensemble_cpu_baseline(nets, 2026)
     assert r is not None and r["value"] > 0 and r["unit"] == "member-cell-substeps/s"
     assert f"{max(n.ncells for n in nets)} cells" in r["sample"]
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference OpenMP build not built")
+def test_spinup_cpu_baseline():
+    """The spin-up form's CPU baseline: the reference under the first month's
+    climatological runoff, reported per simulated year."""
+    bench = _bench()
+    from cmf import maps, synth
+    from cmf.spinup import make_climatology
+    net = synth.make_network(180, 90, seed=991)
+    m = maps.build_map(net, maps.Options())
+    clim = make_climatology(net, m, 991, mean_mm_day=1.5, amp=0.5, phase=3.0)
+    r = bench.spinup_cpu_baseline(net, m, clim, days=5)
+    assert r is not None and r["value"] > 0 and r["unit"] == "s per simulated year"
+    assert "5 daily steps" in r["sample"]
