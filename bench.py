@@ -348,46 +348,38 @@ def precision_report(seed):
 
 
 ENS_DOMAINS = 16
-ENS_MEMBERS = 4           # members per domain: 64 member instances
+ENS_MEMBERS = 64          # runoff realisations of every domain: 16 x 64 = 1024 instances
 ENS_SIGMA = 0.3           # multiplicative lognormal runoff perturbation
+ENS_RECORDS = 3           # daily runoff records per domain, cycled
 
 
 def ensemble_domains(seed, ndom=ENS_DOMAINS):
-    """Regional 0.1-degree domains of uneven size, 2e4 .. 4e5 river cells
-    (geometric-like spread, about 1.5e6 cells in total)."""
+    """Regional 0.1-degree domains of uneven size, 2e4 .. 4e5 river cells, most near
+    4e5 (about 5.7e6 cells in total)."""
     from cmf import maps, synth
-    sizes = 2e4 * 20.0 ** ((np.arange(ndom) / (ndom - 1)) ** 2.0)
-    nets, ms = [], []
-    for d, target in enumerate(sizes):
-        side = int(np.sqrt(target / 0.6 / 1.5))
-        nx, ny = int(side * 1.5), side
-        north = 70.0 - 8.0 * (d % 8)
-        net = synth.make_network(nx, ny, seed=seed + 101 * d, land_frac=0.6, west=-180.0 + 20.0 * d,
-                                 east=-180.0 + 20.0 * d + 0.1 * nx, north=north, south=north - 0.1 * ny,
-                                 regional=True)
-        nets.append(net)
-        ms.append(maps.build_map(net, maps.Options()))
-    return nets, ms
+    nets = synth.regional_networks(synth.ensemble_domain_sizes(ndom), seed)
+    return nets, [maps.build_map(net, maps.Options()) for net in nets]
 
 
-def run_ensemble(nets, ms, precision, days, warmup, seed, tune=None, timing_days=2):
-    """Batched ensemble on one handle: every domain x ENS_MEMBERS members, daily
-    DT with a per-instance adaptive substep, forcing = domain runoff x
-    member factor (lognormal, sigma ENS_SIGMA, redrawn each day)."""
+def run_ensemble(nets, ms, precision, days, warmup, seed, tune=None, timing_days=2, members=ENS_MEMBERS):
+    """Batched ensemble on one handle: every domain x `members` members, daily DT
+    with a per-instance adaptive substep, forcing = domain runoff x member factor
+    (lognormal, sigma ENS_SIGMA, redrawn each day); every member starts from the
+    domain's cold start."""
     from cmf import maps, synth
     from cmf.model import CMFModel
     cm, dom_cells = maps.concat_maps(ms)
-    nm = ENS_MEMBERS
+    nm = members
     ninst = len(ms) * nm
     st0 = [maps.cold_start(m) for m in ms]
-    init = {k: np.stack([np.concatenate([s[k] for s in st0])] * nm)
+    init = {k: np.concatenate([s_[k] for s_ in st0])
             for k in ("rivsto", "fldsto", "rivout", "fldout", "rivdph_pre", "fldsto_pre")}
     em = CMFModel(cm, precision, members=nm, dom_cells=dom_cells)
-    em.set_state(init)
+    em.set_state(init, shared=True)
     em.physics_fldstg()
     ndays = warmup + days + timing_days
-    ro = [synth.make_runoff(net, ndays, seed + 7 * d) for d, net in enumerate(nets)]
-    base = [np.concatenate([m.vec(ro[d][k]) for d, m in enumerate(ms)]) for k in range(ndays)]
+    ro = [synth.make_runoff(net, ENS_RECORDS, seed + 7 * d) for d, net in enumerate(nets)]
+    base = [np.concatenate([m.vec(ro[d][k]) for d, m in enumerate(ms)]) for k in range(ENS_RECORDS)]
     rng = np.random.default_rng(seed)
     fac = np.exp(rng.normal(-0.5 * ENS_SIGMA ** 2, ENS_SIGMA, size=(ndays, ninst)))
     cells_inst = np.repeat(np.asarray(dom_cells, np.float64), nm)
@@ -401,12 +393,12 @@ def run_ensemble(nets, ms, precision, days, warmup, seed, tune=None, timing_days
         tune_used, trials = em.get_tune(), []
     day = 0
     for _ in range(warmup):
-        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.set_runoff_scaled(base[day % ENS_RECORDS], fac[day]); day += 1
         em.ensemble_advance(86400.0)
     em.synchronize()
     work, t_total = 0.0, 0.0
     for _ in range(days):
-        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.set_runoff_scaled(base[day % ENS_RECORDS], fac[day]); day += 1
         em.synchronize()
         t0 = time.perf_counter()
         nt, _dt = em.ensemble_advance(86400.0)
@@ -417,14 +409,14 @@ def run_ensemble(nets, ms, precision, days, warmup, seed, tune=None, timing_days
     lat = []
     em.kernel_times(True)
     for _ in range(timing_days):
-        em.set_runoff_scaled(base[day], fac[day]); day += 1
+        em.set_runoff_scaled(base[day % ENS_RECORDS], fac[day]); day += 1
         em.ensemble_advance(86400.0)
         lat.extend(em.instance_times().tolist())
     kt = em.kernel_times(False)
-    st = em.get_state(["rivsto", "fldsto", "outflw", "rivdph"])
+    g = em.get_state(["rivsto", "fldsto"])
     em.close()
     return dict(work=work, t=t_total, lat=np.asarray(lat), kt=kt, tune=tune_used, trials=trials,
-                state=st, cells=int(sum(dom_cells)), ninst=ninst, dom_cells=dom_cells)
+                storage=g["rivsto"] + g["fldsto"], cells=int(sum(dom_cells)), ninst=ninst, dom_cells=dom_cells)
 
 
 def ensemble_section(a):
@@ -443,9 +435,7 @@ def ensemble_section(a):
         log(f"ensemble {prec}: {r['work'] / r['t']:.4g} member-cell-substeps/s; tune {r['tune']}; "
             f"latency p50 {np.percentile(r['lat'], 50):.1f} ms p99 {np.percentile(r['lat'], 99):.1f} ms")
     # mixed vs FP64 storage error per instance after the same days
-    sm, sd = res["mixed"]["state"], res["fp64"]["state"]
-    tot_m = sm["rivsto"] + sm["fldsto"]
-    tot_d = sd["rivsto"] + sd["fldsto"]
+    tot_m, tot_d = res["mixed"].pop("storage"), res["fp64"].pop("storage")
     offs = np.cumsum([0] + res["fp64"]["dom_cells"])
     rel_tot, rel_max = [], []
     for mi in range(ENS_MEMBERS):
@@ -477,18 +467,20 @@ def ensemble_section(a):
               "kernels_ms": rd["kernels_ms"],
               "latency_ms": {"p50": float(np.percentile(rd["lat"], 50)),
                              "p99": float(np.percentile(rd["lat"], 99))}, "tune": rd["tune"]},
-        mixed_vs_fp64_storage={"total_rel_max": float(max(rel_tot)), "cell_rel_max": float(max(rel_max))},
+        mixed_vs_fp64_ensemble=(r["work"] / r["t"]) / (rd["work"] / rd["t"]),
+        mixed_vs_fp64_storage={"total_rel_max": float(max(rel_tot)), "cell_rel_max": float(max(rel_max)),
+                               "instances": len(rel_tot)},
         cpu_baseline=cpu,
     )
 
 
 def spinup_section(a):
-    """Learned spin-up warm start: labels from a batched FP64 spin-up of 600 river
-    basins cut from six 720x360 glb_15min-shaped networks (own climatology per
-    basin), training by the project's BF16 matrix-core kernels, then FP64
-    spin-ups of a held-out network from the cold start, the mean-flow steady
-    state, the physical prior and the learned warm start (four members of one
-    handle)."""
+    """Learned spin-up warm start at 30 arc-minutes: labels from a batched FP64
+    spin-up of 600 river basins cut from six 720x360 (0.5-degree) networks (own
+    climatology per basin), training by the project's BF16 matrix-core kernels,
+    then FP64 spin-ups of a held-out 720x360 network from the cold start, the
+    mean-flow steady state, the physical prior and the learned warm start (four
+    members of one handle). The glb_03min-sized run is spinup_full_section."""
     from cmf import maps, synth
     from cmf import surrogate as sg
     from cmf.spinup import make_climatology
@@ -524,7 +516,7 @@ def spinup_section(a):
     cmp_ = sg.compare_spinup(sur, sg.Basin(net, m, clim), years=a.spin_years, log=lambda s_: log("spin-up " + s_))
     c, w, pr, sd = cmp_["cold"], cmp_["warm"], cmp_["prior"], cmp_["steady"]
     out = dict(
-        application=f"glb_15min-shaped {nx}x{ny} held-out network, {m.n} cells, monthly gamma climatology "
+        application=f"30-arcmin {nx}x{ny} held-out network, {m.n} cells, monthly gamma climatology "
                     f"repeated, DT=86400, criterion: annual storage change < 0.1%",
         cold_years=c["years"] or f">{cmp_['years_run']}", warm_years=w["years"] or f">{cmp_['years_run']}",
         physics_prior_years=pr["years"] or f">{cmp_['years_run']}",
@@ -541,15 +533,15 @@ def spinup_section(a):
                   "train_seconds": round(t_train, 1)},
         inference={"cells": big.shape[0], "ms": round(1e3 * t_inf, 2),
                    "tflops_incl_transfers": round(flops / t_inf / 1e12, 3)},
-        spinup_seconds=cmp_["seconds"],
-        member_years=4 * cmp_["years_run"],
+        spinup_seconds=cmp_["spinup_seconds"],
+        member_years=cmp_["member_years"],
     )
     if not a.no_cpu:
         try:
             cpu = spinup_cpu_baseline(net, m, clim)
             out["cpu_baseline"] = cpu
             if cpu:
-                gpu_y = cmp_["seconds"] / (4 * cmp_["years_run"])
+                gpu_y = cmp_["spinup_seconds"] / cmp_["member_years"]
                 log(f"spin-up CPU reference: {cpu['value']:.1f} s per simulated year on {cpu['cores']} cores; "
                     f"device {gpu_y:.2f} s per member-year in the 4-member run")
         except Exception as e:          # a reported baseline only: never fails the run
@@ -558,6 +550,117 @@ def spinup_section(a):
         f"warm {w['years']} y; first-year change cold {c['annual_change'][0]:.3e} steady "
         f"{sd['annual_change'][0]:.3e} prior {pr['annual_change'][0]:.3e} warm {w['annual_change'][0]:.3e}")
     return out
+
+
+C5_NX, C5_NY = 7200, 3600        # glb_03min-shaped: 0.05-degree global grid
+C5_MAX_LAT = 70.0                # no land poleward of +-70 degrees (cells there are metres wide)
+C5_TRAIN = (2, 100)              # 200 basins: the 100 largest of two 1440x720 (glb_15min) networks
+
+
+def c5_network(seed):
+    from cmf import maps, synth
+    from cmf.spinup import make_climatology
+    net = synth.make_network(C5_NX, C5_NY, seed=seed + 991, max_lat=C5_MAX_LAT)
+    m = maps.build_map(net, maps.Options())
+    clim = make_climatology(net, m, seed + 991, mean_mm_day=1.5, amp=0.5, phase=3.0)
+    return net, m, clim
+
+
+def spinup_full_section(a):
+    """The learned warm start at its specified size, in phases that exchange files
+    through a.spin_dir (each phase fits one bounded run; a spin-up phase resumes
+    from its saved state when run again):
+      train  labels from a batched FP64 spin-up of 200 basins cut from two
+             glb_15min-shaped 1440x720 networks, training on the device
+      cold   FP64 spin-up of the held-out glb_03min-shaped 7200x3600 network
+             (~6e6 cells) from the cold start (STORAGE_SEA_SURFACE)
+      warm   the same from the physical prior and from the learned warm start
+             (two members of one handle)
+      report years and days to an annual storage change below 0.1% of every
+             start, against the equilibrium of the start that changes least"""
+    from cmf import surrogate as sg
+    d = a.spin_dir
+    os.makedirs(d, exist_ok=True)
+    ph = a.spinup_full
+    t0 = time.time()
+    if ph == "train":
+        nets, per = C5_TRAIN
+        basins = sg.basins_from_networks(nets, per, seed=a.seed + 7, nx=1440, ny=720, max_lat=C5_MAX_LAT)
+        X, Y, res = sg.equilibrium_labels(basins, tol=1e-5, max_years=40, log=lambda s_: log("labels " + s_))
+        t_labels = time.time() - t0
+        Xs, Ys, ws = sg.training_set(X, Y, res.state["rivsto"] + res.state["fldsto"])
+        t1 = time.time()
+        sur, hist = sg.train(Xs, Ys, epochs=a.spin_epochs, batch=1024, lr=2e-3, seed=0, weights=ws,
+                             log=lambda s_: log("train " + s_))
+        sur.save(os.path.join(d, "surrogate.npz"))
+        info = {"basins": len(basins), "networks": f"{nets} x 1440x720 (glb_15min-shaped, land within +-70 deg)",
+                "cells": int(X.shape[0]), "corrected_cells": int(Xs.shape[0]), "epochs": a.spin_epochs,
+                "label_years_median": int(np.median(res.years)), "label_years_max": int(res.years.max()),
+                "loss_first": hist[0], "loss_last": hist[-1], "label_seconds": round(t_labels, 1),
+                "train_seconds": round(time.time() - t1, 1)}
+        with open(os.path.join(d, "train.json"), "w") as f:
+            json.dump(info, f)
+        return {"phase": "train", **info}
+    if ph in ("cold", "warm"):
+        net, m, clim = c5_network(a.seed)
+        log(f"glb_03min-shaped network: {m.n} cells ({time.time() - t0:.1f}s)")
+        state_file = os.path.join(d, f"{ph}_state.npz")
+        resume = None
+        if os.path.exists(state_file):
+            z = np.load(state_file, allow_pickle=False)
+            resume = {k: z[k] for k in z.files}
+            starts = None
+            log(f"{ph}: resuming after {resume['hist'].shape[0] - 1} years")
+        elif ph == "cold":
+            starts = {"cold": sg.cold_state(m)}
+        else:
+            sur = sg.Surrogate.load(os.path.join(d, "surrogate.npz"))
+            tp = time.time()
+            pr = sg.prior(m, clim)
+            t_prior = time.time() - tp
+            ti = time.time()
+            ws_ = sur.warm_start(m, clim, pr)
+            t_inf = time.time() - ti
+            starts = {"prior": sg.prior_state(m, clim, pr), "warm": sg.warm_state(m, ws_, np.zeros(m.n))}
+            with open(os.path.join(d, "warm_setup.json"), "w") as f:
+                json.dump({"prior_seconds": round(t_prior, 1), "features_and_inference_seconds": round(t_inf, 1)}, f)
+        names = ["cold"] if ph == "cold" else ["prior", "warm"]
+        r = sg.spinup_from_starts(m, clim, starts, years=a.spin_years, names=names, resume=resume,
+                                  time_budget=a.spin_seconds, log=lambda s_: log(f"{ph} " + s_))
+        np.savez(state_file, **r["resume"])
+        done = all(r["conv"])
+        log(f"{ph}: years run {r['hist'].shape[0] - 1}, converged {r['conv']}, {r['seconds']} s this run")
+        return {"phase": ph, "cells": m.n, "years_run": int(r["hist"].shape[0] - 1), "converged": done,
+                "conv": r["conv"], "seconds": r["seconds"]}
+    if ph == "report":
+        runs = []
+        for nm in ("cold", "warm"):
+            z = np.load(os.path.join(d, f"{nm}_state.npz"), allow_pickle=False)
+            runs.append(dict(names=["cold"] if nm == "cold" else ["prior", "warm"], hist=z["hist"], daily=z["daily"],
+                             conv=[int(c) for c in z["conv"]], seconds=float(z["seconds"]),
+                             member_substeps=int(z["member_substeps"]), cells=int(z["cells"])))
+        res = sg.summarize_spinups(runs)
+        with open(os.path.join(d, "train.json")) as f:
+            train = json.load(f)
+        setup = json.load(open(os.path.join(d, "warm_setup.json")))
+        c, p_, w = res["cold"], res["prior"], res["warm"]
+        ys = lambda v: v["years"] or f">{v['years_run']}"
+        member_years = sum((r["hist"].shape[0] - 1) * len(r["names"]) for r in runs)
+        secs = sum(r["seconds"] for r in runs)
+        out = dict(
+            application=f"glb_03min-shaped {C5_NX}x{C5_NY} held-out network (land within +-{C5_MAX_LAT:g} deg), "
+                        f"{runs[0]['cells']} cells, monthly gamma climatology repeated, DT=86400, FP64, "
+                        f"criterion: annual storage change < 0.1%",
+            cold_years=ys(c), physics_prior_years=ys(p_), warm_years=ys(w),
+            years_ratio=(c["years"] / w["years"]) if (c["years"] and w["years"]) else None,
+            days_to_0p1pct={k: res[k]["days"] for k in ("cold", "prior", "warm")},
+            init_rel_err={k: res[k]["init_rel_err"] for k in ("cold", "prior", "warm")},
+            annual_change={k: res[k]["annual_change"] for k in ("cold", "prior", "warm")},
+            equilibrium_reference=res["reference_member"], training=train, warm_setup=setup,
+            member_years=member_years, spinup_seconds=round(secs, 1),
+            member_cell_substeps_per_s=sum(r["member_substeps"] for r in runs) * runs[0]["cells"] / secs)
+        return out
+    raise SystemExit(f"unknown spin-up phase {ph!r}")
 
 
 def forcing_section(net, m, seed, res_in=0.25, calls=20):
@@ -658,6 +761,12 @@ def main():
     ap.add_argument("--spinup-only", action="store_true")
     ap.add_argument("--spin-epochs", type=int, default=150)
     ap.add_argument("--spin-years", type=int, default=20)
+    ap.add_argument("--spinup-full", choices=("train", "cold", "warm", "report"),
+                    help="one phase of the glb_03min-sized learned warm start (JSON of the phase on stdout)")
+    ap.add_argument("--spin-dir", default=os.path.join(ROOT, "gpurun_out", "spinup_full"),
+                    help="directory the --spinup-full phases exchange files through")
+    ap.add_argument("--spin-seconds", type=float, default=900.0,
+                    help="a spin-up phase stops after the simulated year that passes this many seconds")
     ap.add_argument("--precision-report", action="store_true",
                     help="only report mixed-vs-FP64 errors of the C1/C2/C3 configurations (JSON)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -669,6 +778,9 @@ def main():
     ap.add_argument("--compare-order", action="store_true",
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
+    if a.spinup_full:
+        print(json.dumps({"spinup_full": spinup_full_section(a)}), flush=True)
+        return
     if a.spinup_only:
         print(json.dumps({"spinup": spinup_section(a)}), flush=True)
         return

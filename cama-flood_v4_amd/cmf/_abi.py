@@ -38,7 +38,8 @@ class cmf_dims(C.Structure):
 class cmf_options(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("ladpstp", "lfplain", "lfldout", "lpthout", "ldamout",
                                            "llevee", "precision")] + \
-               [(k, C.c_double) for k in ("pgrv", "pmanfld", "pdstmth", "pcadp", "pminslp")]
+               [(k, C.c_double) for k in ("pgrv", "pmanfld", "pdstmth", "pcadp", "pminslp")] + \
+               [("jprb", C.c_int32)]
 
 
 MAP_FIELDS = [("i1next", P)] + [(k, P) for k in (
@@ -106,6 +107,8 @@ HIP_FUNCS = {
     "cmf_hip_get_output": (I, [H, C.POINTER(cmf_output)]),
     "cmf_hip_ensemble_advance": (I, [H, C.c_double, C.c_int32, P, P]),
     "cmf_hip_instance_times": (I, [H, P]),
+    "cmf_hip_get_dt_min": (I, [H, P]),
+    "cmf_hip_instance_storage": (I, [H, P]),
     "cmf_hip_set_tune": (I, [H, C.POINTER(cmf_tune)]),
     "cmf_hip_get_tune": (I, [H, C.POINTER(cmf_tune)]),
     "cmf_hip_autotune": (I, [H, C.c_double, C.c_int32, C.POINTER(cmf_tune), P, C.c_int32]),
@@ -113,9 +116,11 @@ HIP_FUNCS = {
     "cmf_hip_last_error": (C.c_char_p, []),
     "cmf_hip_set_map": (I, [H, C.POINTER(cmf_map)]),
     "cmf_hip_set_state": (I, [H, C.POINTER(cmf_state)]),
+    "cmf_hip_set_state_shared": (I, [H, C.POINTER(cmf_state)]),
     "cmf_hip_get_state": (I, [H, C.POINTER(cmf_state)]),
     "cmf_hip_set_runoff": (I, [H, P]),
     "cmf_hip_physics_fldstg": (I, [H]),
+    "cmf_hip_calc_outpre": (I, [H]),
     "cmf_hip_physics_advance": (I, [H, C.POINTER(C.c_double), C.POINTER(C.c_int32), P, C.c_int32]),
     "cmf_hip_physics_advance_capped": (I, [H, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32),
                                            P, C.c_int32]),

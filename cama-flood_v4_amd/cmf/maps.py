@@ -30,6 +30,7 @@ class Options:
     ldamout: bool = False
     llevee: bool = False
     lkine: bool = False           # kinematic-wave outflow (CMF_CALC_OUTFLW_KINE)
+    ldamh22: bool = False         # Hanazaki-2022 dam rule (NDAMOUT LDAMH22)
     pmanriv: float = 0.03
     pmanfld: float = 0.10
     pgrv: float = 9.8
@@ -98,6 +99,8 @@ class CMFMap:
     dam_qa: Optional[np.ndarray] = None
     dam_qn: Optional[np.ndarray] = None
     dam_qf: Optional[np.ndarray] = None
+    dam_norvol: Optional[np.ndarray] = None     # Hanazaki-2022 rule: NorVol
+    dam_r_volupa: Optional[np.ndarray] = None   # and FldVol / upstream area
 
     def vec(self, map2d: np.ndarray) -> np.ndarray:
         """mapR2vecD (src/cmf_utils_mod.F90:282-299): float32 map -> float64 vector."""
@@ -235,10 +238,15 @@ def cold_start(m: CMFMap) -> dict:
     return st
 
 
-def init_dams(m: CMFMap, net, st: dict) -> None:
-    """CMF_DAMOUT_INIT (src/cmf_ctrl_damout_mod.F90:123-308), Yamazaki & Funato rule,
-    no year-by-year activation, no restart. Must run after the initial flood-stage
-    call, as in src/cmf_drv_control_mod.F90:243-252; updates st in place."""
+def init_dams(m: CMFMap, net, st: dict, restart: bool = False, syear: int = 2001) -> None:
+    """CMF_DAMOUT_INIT (src/cmf_ctrl_damout_mod.F90:123-308): the Yamazaki & Funato
+    rule, or with m.opts.ldamh22 the Hanazaki-2022 rule (NorVol and the ratio of
+    flood-control volume to upstream area, damout:202-205, stored per dam: the
+    reference writes every dam's ratio to R_VolUpa(NDAM)); year-by-year activation
+    (LDAMYBY) when the dam rows carry a "year", against the start year syear
+    (ISYYYY, damout:216-225). Must run after the initial flood-stage call, as in
+    src/cmf_drv_control_mod.F90:243-252; updates st in place. With restart (LRESTART)
+    the dam storage read from the restart file is kept (damout:256-285)."""
     f32 = lambda x: float(np.float32(x))
     vecidx = np.zeros((m.ny, m.nx), np.int32)
     vecidx[m.seqy - 1, m.seqx - 1] = np.arange(1, m.n + 1)
@@ -249,6 +257,9 @@ def init_dams(m: CMFMap, net, st: dict) -> None:
     damstat = np.full(nd, -9999, np.int32)
     convol, emevol, adjvol = np.zeros(nd), np.zeros(nd), np.zeros(nd)
     qn, qf, qa = np.zeros(nd), np.zeros(nd), np.zeros(nd)
+    norvol, r_volupa = np.zeros(nd), np.zeros(nd)
+    h22 = bool(m.opts.ldamh22)
+    yby = bool(rows) and "year" in rows[0]
     for d, r in enumerate(rows):
         fldvol = r["fldvol"] * 1.0e6
         convol[d] = r["convol"] * 1.0e6
@@ -264,11 +275,22 @@ def init_dams(m: CMFMap, net, st: dict) -> None:
         damseq[d], damstat[d] = s, 2
         i1dam[s] = 1
         m.i2mask[s] = 2
-        vyr = qn[d] * f32(365. * 24. * 60. * 60.)
-        qsto = (convol[d] * f32(0.7) + vyr / 4.0) / f32(180. * 24. * 60. * 60.)
-        qn[d] = min(qn[d], qsto) * 1.5
-        adjvol[d] = convol[d] + fldvol * f32(0.1)
-        qa[d] = (qn[d] + qf[d]) * 0.5
+        if h22:                                             # damout:202-205
+            norvol[d] = convol[d] * 0.5
+            r_volupa[d] = fldvol * f32(1.e-6) / r["upreal"]
+        else:                                               # damout:207-212
+            vyr = qn[d] * f32(365. * 24. * 60. * 60.)
+            qsto = (convol[d] * f32(0.7) + vyr / 4.0) / f32(180. * 24. * 60. * 60.)
+            qn[d] = min(qn[d], qsto) * 1.5
+            adjvol[d] = convol[d] + fldvol * f32(0.1)
+            qa[d] = (qn[d] + qf[d]) * 0.5
+        if yby:                                             # damout:216-225
+            if syear == r["year"]:
+                damstat[d] = 1
+            elif syear < r["year"] and r["year"] > 0:
+                damstat[d] = -1
+                i1dam[s] = -1
+                convol[d] = 0.0
     if not (damstat == 2).any():
         raise RuntimeError("no dam allocated in the domain")
     for i in range(m.n):                                   # damout:238-254, ascending
@@ -281,13 +303,14 @@ def init_dams(m: CMFMap, net, st: dict) -> None:
             if i1dam[j] in (1, 11):
                 i1dam[i] = 11
                 m.i2mask[i] = 2
-    st["damsto"] = np.zeros(m.n)
+    if not restart:
+        st["damsto"] = np.zeros(m.n)
     for d in range(nd):                                    # damout:257-272
-        if damstat[d] == -9999:
+        if restart or damstat[d] == -9999:
             continue
         s = damseq[d]
         st["damsto"][s] = st["rivsto"][s] + st["fldsto"][s]
-        if st["damsto"][s] < convol[d]:
+        if damstat[d] != -1 and st["damsto"][s] < convol[d]:
             st["damsto"][s] = convol[d]
             st["rivsto"][s] = convol[d]
             st["fldsto"][s] = 0.0
@@ -303,12 +326,14 @@ def init_dams(m: CMFMap, net, st: dict) -> None:
     m.i1dam, m.damseq, m.damstat = i1dam, damseq, damstat
     m.dam_convol, m.dam_emevol, m.dam_adjvol = convol, emevol, adjvol
     m.dam_qn, m.dam_qf, m.dam_qa = qn, qf, qa
+    m.dam_norvol, m.dam_r_volupa = norvol, r_volupa
 
 
 _CELL = ("grarea", "elevtn", "nxtdst", "rivlen", "rivwth", "rivhgt", "rivman", "rivstomax", "rivelv",
          "dwnelv", "i2mask", "seqx", "seqy")
 _LEVEE = ("levhgt", "levfrc", "bashgt", "levdst", "levbassto", "levtopsto", "levfilsto")
-_DAM = ("damstat", "dam_convol", "dam_emevol", "dam_adjvol", "dam_qa", "dam_qn", "dam_qf")
+_DAM = ("damstat", "dam_convol", "dam_emevol", "dam_adjvol", "dam_qa", "dam_qn", "dam_qf", "dam_norvol",
+        "dam_r_volupa")
 
 
 def concat_maps(ms) -> tuple:

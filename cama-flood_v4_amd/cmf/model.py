@@ -63,7 +63,7 @@ class CMFModel:
                      m.npthlev if o.lpthout else 0, m.ndam if o.ldamout else 0)
         op = cmf_options(int(o.ladpstp), int(o.lfplain), int(o.lfldout), int(o.lpthout),
                          int(o.ldamout), int(o.llevee), self.prec,
-                         o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp)
+                         o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp, 8)
         h = C.c_void_p()
         if self.ensemble:
             dc = np.ascontiguousarray(self.dom_cells, np.int32)
@@ -115,9 +115,11 @@ class CMFModel:
         self._keep.clear()
 
     # ------------------------------------------------------------------
-    def set_state(self, st: dict) -> None:
-        """Host -> device copy of the given arrays (names of cmf_state)."""
+    def set_state(self, st: dict, shared: bool = False) -> None:
+        """Host -> device copy of the given arrays (names of cmf_state). shared:
+        each array holds one plane that every member of an ensemble receives."""
         s = cmf_state()
+        nplanes = 1 if shared else self.members
         keep = []
         m = self.map
         for k, v in st.items():
@@ -131,11 +133,14 @@ class CMFModel:
                     want = m.npth
                 else:
                     want = m.n
-                if a.size != want * self.members:
-                    raise CMFError(f"set_state: {k} has {a.size} values, expected {want * self.members}")
+                if a.size != want * nplanes:
+                    raise CMFError(f"set_state: {k} has {a.size} values, expected {want * nplanes}")
                 keep.append(a)
                 setattr(s, k, a.ctypes.data)
-        self._check(self.lib.cmf_hip_set_state(self.h, C.byref(s)))
+        if shared:
+            self._check(self.lib.cmf_hip_set_state_shared(self.h, C.byref(s)))
+        else:
+            self._check(self.lib.cmf_hip_set_state(self.h, C.byref(s)))
 
     def get_state(self, names: Optional[Iterable[str]] = None) -> dict:
         m, o = self.map, self.opts
@@ -287,6 +292,11 @@ class CMFModel:
     def physics_fldstg(self) -> None:
         self._check(self.lib.cmf_hip_physics_fldstg(self.h))
 
+    def calc_outpre(self) -> None:
+        """CMF_CALC_OUTPRE (src/cmf_opt_outflw_mod.F90:308-421): previous-step flows
+        of a storage-only restart from the current stage."""
+        self._check(self.lib.cmf_hip_calc_outpre(self.h))
+
     def physics_advance(self, dt: float, nt_cap: int = 0, budget: bool = False):
         """One outer step of length dt [s]. Returns NT, or (NT, budget array
         [NT, 11]) when budget=True."""
@@ -311,6 +321,19 @@ class CMFModel:
         self._check(self.lib.cmf_hip_ensemble_advance(self.h, float(dt), int(nt_cap), nt.ctypes.data,
                                                       dto.ctypes.data))
         return nt, dto
+
+    def instance_storage(self) -> np.ndarray:
+        """Total storage (P2RIVSTO + P2FLDSTO (+ P2LEVSTO)) of every instance,
+        summed on the device."""
+        out = np.zeros(self.ninst)
+        self._check(self.lib.cmf_hip_instance_storage(self.h, out.ctypes.data))
+        return out
+
+    def dt_min(self) -> np.ndarray:
+        """DT_MIN of CALC_ADPSTP of the last advance, per instance (0 without LADPSTP)."""
+        out = np.zeros(self.ninst)
+        self._check(self.lib.cmf_hip_get_dt_min(self.h, out.ctypes.data))
+        return out
 
     def instance_times(self) -> np.ndarray:
         """Per-instance completion time [ms] of the last timed advance."""

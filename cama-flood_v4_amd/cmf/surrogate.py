@@ -360,7 +360,7 @@ class Basin:
 
 
 def basins_from_networks(nets: int, per_net: int, seed: int, nx: int = 1440, ny: int = 720,
-                         min_cells: int = 30):
+                         min_cells: int = 30, max_lat: Optional[float] = None):
     """River basins cut from glb_15min-shaped global networks: the `per_net`
     largest basins of each, so the set spans small coastal basins to continental
     ones with filled depressions. Every basin gets its own runoff climatology on
@@ -370,7 +370,7 @@ def basins_from_networks(nets: int, per_net: int, seed: int, nx: int = 1440, ny:
     out = []
     for k in range(nets):
         s_ = seed * 100 + k
-        net = synth.make_network(nx, ny, seed=s_)
+        net = synth.make_network(nx, ny, seed=s_, max_lat=max_lat)
         m = maps.build_map(net, maps.Options())
         base = m.vec(synth.make_runoff(net, 1, s_, mean_mm_day=1.0)[0])        # m3/s at 1 mm/day mean
         b = maps.basin_of(m)
@@ -479,39 +479,47 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None,
     return X, Y, res
 
 
-def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1e-3, log=None) -> dict:
-    """FP64 spin-ups of one network from the cold start, from the steady state
-    under the annual-mean runoff, from the physical prior (linear periodic
-    response) and from the surrogate's warm start, run together as four members of
-    one handle (same map and forcing, own adaptive steps), until every annual
-    storage change is below tol or `years` pass. The equilibrium all are measured
-    against is the last year of the member whose last annual change is smallest."""
+RESUME_KEYS = ("rivsto", "fldsto", "rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre", "rivout")
+
+
+def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], years: int = 20, tol: float = 1e-3,
+                       log=None, names=None, resume: Optional[dict] = None,
+                       time_budget: Optional[float] = None) -> dict:
+    """FP64 spin-up of one network from several starts, run as the members of one
+    handle (same map and forcing, own adaptive steps; the member count is padded
+    to a power of two by repeating the last start), until every start's annual
+    storage change is below tol, `years` pass, or (time_budget) the year that
+    passes time_budget seconds ends. The total storage of every member is summed
+    on the device at the end of every day. The result's "resume" dict (the
+    restart state of every member plus the history) continues the run exactly
+    when passed back as `resume`."""
     import time
     from .model import CMFModel
     from .spinup import DAYS
-    m, clim = basin.m, basin.clim
-    pr = prior(m, clim)
-    names = ("cold", "steady", "prior", "warm")
-    starts = [cold_state(m), prior_state(m, clim, pr, "steady"), prior_state(m, clim, pr),
-              warm_state(m, sur.warm_start(m, clim, pr), np.zeros(m.n))]
-    M = len(starts)
+    names = list(names if names is not None else starts)
+    M = 1 << max(len(names) - 1, 0).bit_length()
     model = CMFModel(m, "fp64", members=M, dom_cells=[m.n])
-    model.set_state({k: np.stack([st[k] for st in starts]) for k in ("rivsto", "fldsto", "rivdph_pre")})
-    model.physics_fldstg()
+    if resume is None:
+        sts = [starts[nm] for nm in names] + [starts[names[-1]]] * (M - len(names))
+        model.set_state({k: np.stack([st[k] for st in sts]) for k in ("rivsto", "fldsto", "rivdph_pre")})
+        model.physics_fldstg()
+        hist, daily = [model.instance_storage()[:len(names)]], []
+        conv = [0] * len(names)
+        substeps, spent = 0, 0.0
+    else:
+        model.set_state({k: resume[k] for k in RESUME_KEYS})
+        hist, daily = list(resume["hist"]), list(resume["daily"])
+        conv = [int(c) for c in resume["conv"]]
+        substeps, spent = int(resume["member_substeps"]), float(resume["seconds"])
     t0 = time.time()
-
-    def total():
-        st = model.get_state(["rivsto", "fldsto"])
-        return (st["rivsto"] + st["fldsto"]).sum(axis=1)
-
-    hist, daily = [total()], []
-    conv = [0] * len(names)
     one = np.ones(M)
-    for y in range(1, years + 1):
+    y0 = len(hist) - 1
+    for y in range(y0 + 1, years + 1):
         for d in range(DAYS):
             model.set_runoff_scaled(clim.daily(d), one)
-            model.ensemble_advance(86400.0)
-            daily.append(total())
+            nt, _ = model.ensemble_advance(86400.0)
+            substeps += int(nt[:len(names)].sum())
+            daily.append(model.instance_storage()[:len(names)])
         now = daily[-1]
         ch = np.abs(now - hist[-1]) / now
         hist.append(now)
@@ -521,20 +529,63 @@ def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1
         if log:
             log(f"year {y}: annual change " + " ".join(f"{nm} {ch[k]:.3e}" for k, nm in enumerate(names)) +
                 f" ({time.time() - t0:.1f}s)")
-        if all(conv):
+        if all(conv) or (time_budget is not None and time.time() - t0 > time_budget):
             break
+    seconds = spent + (time.time() - t0)
+    st = model.get_state(list(RESUME_KEYS))
     model.close()
-    daily = np.asarray(daily)
-    hist = np.asarray(hist)
-    last = np.abs(hist[-1] - hist[-2]) / hist[-1]
-    ref = int(np.argmin(last[:len(names)]))
-    eq = daily[-DAYS:, ref]
-    res = {"reference_member": names[ref]}
-    for k, name in enumerate(names):
-        res[name] = dict(years=int(conv[k]), days=days_to_equilibrium(daily[:, k], eq, tol),
-                         init_rel_err=float(abs(hist[0, k] - hist[-1, ref]) / hist[-1, ref]),
-                         annual_change=[float(abs(hist[y, k] - hist[y - 1, k]) / hist[y, k])
-                                        for y in range(1, hist.shape[0])])
-    res["years_run"] = int(hist.shape[0] - 1)
+    hist_a, daily_a = np.asarray(hist), np.asarray(daily)
+    res = dict(names=names, hist=hist_a, daily=daily_a, conv=conv, seconds=round(seconds, 1),
+               member_substeps=substeps, cells=m.n)
+    res["resume"] = dict(hist=hist_a, daily=daily_a, conv=np.asarray(conv), seconds=seconds,
+                         member_substeps=substeps, cells=m.n, **st)
+    return res
+
+
+def summarize_spinups(runs, tol: float = 1e-3) -> dict:
+    """Years (first year with an annual change below tol) and days to the
+    periodic equilibrium of every start of one or more spinup_from_starts runs
+    on the same network. The equilibrium all starts are measured against is the
+    last year of the start whose last annual change is smallest."""
+    from .spinup import DAYS
+    best = None
+    for r in runs:
+        last = np.abs(r["hist"][-1] - r["hist"][-2]) / r["hist"][-1]
+        for k, nm in enumerate(r["names"]):
+            if best is None or last[k] < best[0]:
+                best = (float(last[k]), nm, r["daily"][-DAYS:, k], float(r["hist"][-1, k]))
+    _, ref_name, eq, ref_total = best
+    res = {"reference_member": ref_name}
+    for r in runs:
+        for k, nm in enumerate(r["names"]):
+            h = r["hist"][:, k]
+            res[nm] = dict(years=int(r["conv"][k]), days=days_to_equilibrium(r["daily"][:, k], eq, tol),
+                           init_rel_err=float(abs(h[0] - ref_total) / ref_total),
+                           annual_change=[float(abs(h[y] - h[y - 1]) / h[y]) for y in range(1, h.size)],
+                           years_run=int(h.size - 1))
+    return res
+
+
+def compare_spinup(sur: Surrogate, basin: Basin, years: int = 20, tol: float = 1e-3, log=None,
+                   groups=(("cold", "steady", "prior", "warm"),)) -> dict:
+    """FP64 spin-ups of one network from the cold start, from the steady state
+    under the annual-mean runoff, from the physical prior (linear periodic
+    response) and from the surrogate's warm start. Each group of starts runs as
+    the members of one handle (spinup_from_starts), until every annual storage
+    change is below tol or `years` pass; summarize_spinups measures them against
+    one equilibrium."""
+    import time
+    m, clim = basin.m, basin.clim
+    pr = prior(m, clim)
+    make = {"cold": lambda: cold_state(m), "steady": lambda: prior_state(m, clim, pr, "steady"),
+            "prior": lambda: prior_state(m, clim, pr),
+            "warm": lambda: warm_state(m, sur.warm_start(m, clim, pr), np.zeros(m.n))}
+    t0 = time.time()
+    runs = [spinup_from_starts(m, clim, {nm: make[nm]() for nm in g}, years=years, tol=tol, log=log)
+            for g in groups]
+    res = summarize_spinups(runs, tol)
+    res["years_run"] = max(int(r["hist"].shape[0] - 1) for r in runs)
+    res["member_years"] = int(sum((r["hist"].shape[0] - 1) * len(r["names"]) for r in runs))
+    res["spinup_seconds"] = round(sum(r["seconds"] for r in runs), 1)
     res["seconds"] = round(time.time() - t0, 1)
     return res

@@ -105,15 +105,19 @@ def _haversine(lat1, lon1, lat2, lon2):
 
 def make_network(nx: int, ny: int, seed: int, land_frac: float = 0.2315,
                  relief: float = 3000.0, west=-180.0, east=180.0, north=90.0,
-                 south=-90.0, regional: bool = False) -> SynthNetwork:
+                 south=-90.0, regional: bool = False, max_lat: Optional[float] = None) -> SynthNetwork:
     """River network on an nx x ny grid (see module docstring). regional=True
     makes the outer ring of the grid ocean, so that no river crosses the edges
-    of a regional extent."""
+    of a regional extent. max_lat makes every cell poleward of +-max_lat ocean
+    (land_frac stays the fraction of the whole grid)."""
     rng = np.random.default_rng(seed)
     z = _fractal(nx, ny, 3.0, rng)
     if regional:
         z[0, :] = z[-1, :] = z.min()
         z[:, 0] = z[:, -1] = z.min()
+    if max_lat is not None:
+        lat_row = north - (np.arange(ny) + 0.5) * ((north - south) / ny)
+        z[np.abs(lat_row) > max_lat, :] = z.min()
     thr = np.quantile(z, 1.0 - land_frac)
     land = z > thr
     elev = np.where(land, (z - thr) / (z.max() - thr) * relief, 0.0).astype(np.float32)
@@ -209,10 +213,12 @@ def add_levees(net: SynthNetwork, frac: float, seed: int, hmin=2.0, hmax=8.0,
 
 
 def add_dams(net: SynthNetwork, ndam: int, seed: int, mu=math.log(1e8), sigma=1.5,
-             mean_runoff_mm_day=1.5) -> None:
+             mean_runoff_mm_day=1.5, years=None) -> None:
     """Dams on river cells with large upstream area; capacity lognormal(mu, sigma) m3.
     Flood-control volume 30%, conservation volume 50% of capacity; normal discharge
-    from mean runoff over the upstream area, flood discharge 3x normal."""
+    from mean runoff over the upstream area, flood discharge 3x normal. years (a
+    sequence, cycled over the dams) gives every dam a construction year, the last
+    column of the reference's dam table with LDAMYBY (damout:173-175)."""
     rng = np.random.default_rng(seed + 303)
     river = net.nextx > 0
     idx = np.flatnonzero(river.ravel())
@@ -231,6 +237,8 @@ def add_dams(net: SynthNetwork, ndam: int, seed: int, mu=math.log(1e8), sigma=1.
                          upreal=a * 1e-6, ix=ix + 1, iy=iy + 1,
                          fldvol=0.3 * cap * 1e-6, convol=0.5 * cap * 1e-6, totvol=cap * 1e-6,
                          qn=qn, qf=3.0 * qn))
+        if years is not None:
+            rows[-1]["year"] = int(years[d % len(years)])
     net.dams = rows
 
 
@@ -254,6 +262,33 @@ def make_runoff(net: SynthNetwork, nrec: int, seed: int, mean_mm_day=1.5, shape=
         q[land] = mm * 1e-3 / 86400.0 * area[land]
         out[r] = q
     return out
+
+
+# ---------------------------------------------------------------------------
+# Regional domains of the batched runoff ensemble
+
+def ensemble_domain_sizes(ndom: int = 16, smin: float = 2.0e4, smax: float = 4.0e5,
+                          power: float = 10.0) -> np.ndarray:
+    """Target river-cell counts of ndom regional domains of uneven size, from smin
+    to smax with most domains near smax: smin + (smax - smin) (1 - (1 - x)^power),
+    x = 0 .. 1 (16 domains: 2e4 .. 4e5 cells, about 5.7e6 in total)."""
+    x = np.arange(ndom) / max(ndom - 1, 1)
+    return smin + (smax - smin) * (1.0 - (1.0 - x) ** power)
+
+
+def regional_networks(sizes, seed: int, land_frac: float = 0.6, res: float = 0.1) -> list:
+    """One regional network of about sizes[d] river cells per domain, on a res-degree
+    grid of aspect 3:2 (outer ring ocean, so no river leaves the domain), placed
+    side by side in longitude; seed + 101 d seeds domain d."""
+    nets = []
+    for d, target in enumerate(sizes):
+        side = int(np.sqrt(target / land_frac / 1.5))
+        nx, ny = int(side * 1.5), side
+        north = 70.0 - 8.0 * (d % 8)
+        west = -180.0 + 20.0 * (d % 18)
+        nets.append(make_network(nx, ny, seed=seed + 101 * d, land_frac=land_frac, west=west,
+                                 east=west + res * nx, north=north, south=north - res * ny, regional=True))
+    return nets
 
 
 # ---------------------------------------------------------------------------
@@ -328,11 +363,15 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
                            llevee=False, ldamout=False, ladpstp=True, lfldout=True,
                            pmanriv=0.03, pmanfld=0.10, pcadp=0.7, pdstmth=10000.0,
                            restart: bool = False, outputs: Optional[str] = None,
-                           lkine: bool = False) -> None:
+                           lkine: bool = False, restart_in: Optional[str] = None,
+                           lstoonly: bool = False, ldamh22: bool = False) -> None:
     """Write the map files, tables, runoff records and input_cmf.nam that the
     reference needs (formats of src/cmf_ctrl_maps_mod.F90 READ_MAP_BIN,
     READ_TOPO_BIN, READ_BIFPARAM; src/cmf_ctrl_damout_mod.F90:141-179;
-    src/cmf_ctrl_levee_mod.F90:121-133)."""
+    src/cmf_ctrl_levee_mod.F90:121-133). restart_in names an FP64 binary restart
+    file in d to start from (LRESTART, CRESTSTO; LSTOONLY with lstoonly). Dams with a
+    "year" write the year column and switch LDAMYBY on; ldamh22 selects the
+    Hanazaki-2022 dam rule (LDAMH22)."""
     os.makedirs(d, exist_ok=True)
     nx, ny = net.nx, net.ny
 
@@ -361,14 +400,16 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
     if llevee:
         wf("levhgt.bin", net.levhgt)
         wf("levfrc.bin", net.levfrc)
+    ldamyby = bool(ldamout and net.dams and "year" in net.dams[0])
     if ldamout:
         with open(os.path.join(d, "dam_params.csv"), "w") as f:
             f.write(f"{len(net.dams)}\n")
-            f.write("ID,Name,Lat,Lon,Area,IX,IY,FldVol_mcm,ConVol_mcm,TotVol_mcm,Qn,Qf\n")
+            f.write("ID,Name,Lat,Lon,Area,IX,IY,FldVol_mcm,ConVol_mcm,TotVol_mcm,Qn,Qf" +
+                    (",year\n" if ldamyby else "\n"))
             for r in net.dams:
                 f.write(f"{r['id']},{r['name']},{r['lat']!r},{r['lon']!r},{r['upreal']!r},"
                         f"{r['ix']},{r['iy']},{r['fldvol']!r},{r['convol']!r},{r['totvol']!r},"
-                        f"{r['qn']!r},{r['qf']!r}\n")
+                        f"{r['qn']!r},{r['qf']!r}" + (f",{r['year']}\n" if ldamyby else "\n"))
     np.ascontiguousarray(runoff, dtype=np.float64).tofile(os.path.join(d, "runoff.bin"))
 
     import datetime as _dt
@@ -378,6 +419,8 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
     # restart at the end of the run (IFRQ_RST = 0), binary FP64, restart<YYYYMMDDHH>.bin
     restart_nml = ('IFRQ_RST = 0\nCRESTDIR = "./"\nCVNREST = "restart"\nLRESTCDF = .FALSE.\n'
                    'LRESTDBL = .TRUE.') if restart else "IFRQ_RST = -1"
+    if restart_in is not None:
+        restart_nml += f'\nCRESTSTO = "{restart_in}"' + ('' if restart else '\nLRESTCDF = .FALSE.\nLRESTDBL = .TRUE.')
     nml = f"""&NRUNVER
 LADPSTP  = {L(ladpstp)}
 LFLDOUT  = {L(lfldout)}
@@ -385,7 +428,8 @@ LPTHOUT  = {L(lpthout)}
 LDAMOUT  = {L(ldamout)}
 LLEVEE   = {L(llevee)}
 LKINE    = {L(lkine)}
-LRESTART = .FALSE.
+LRESTART = {L(restart_in is not None)}
+LSTOONLY = {L(lstoonly)}
 LOUTPUT  = {L(outputs is not None)}
 LSPAMAT  = .TRUE.
 /
@@ -441,6 +485,8 @@ IFRQ_OUT = 24
 &NDAMOUT
 CDAMFILE = "dam_params.csv"
 LDAMTXT  = .FALSE.
+LDAMH22  = {L(ldamh22)}
+LDAMYBY  = {L(ldamyby)}
 /
 &NLEVEE
 CLEVHGT = "levhgt.bin"

@@ -10,7 +10,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -45,6 +47,7 @@ struct StateArr {
 struct cmf_hip_ctx {
     cmf_dims d{};
     cmf_options o{};
+    bool hf32 = false;                // JPRB host arrays are float (jprb = 4)
     int device = 0;
     hipStream_t stream = nullptr;
     bool map_set = false;
@@ -54,14 +57,22 @@ struct cmf_hip_ctx {
     DevState<float> S32{};
     void* vel_buf = nullptr;          // D2RIVOUT when it differs from D2RIVOUT_PRE
     bool vel_pending = false;
+    void* fld_buf = nullptr;          // D2FLDOUT when it differs from D2FLDOUT_PRE (allocated when needed)
+    bool fld_pending = false;
+    void* pth_buf = nullptr;          // D1PTHFLW when it differs from D1PTHFLW_PRE (allocated when needed)
+    bool pth_pending = false;
     double* h_ctl = nullptr;          // pinned scratch for small device->host reads
     // ensemble: ndom domains concatenated in NSEQ, nmem = 2^lgm members each
     int ndom = 1, nmem = 1, lgm = 0, ninst = 1;
     std::vector<int32_t> dom_off;     // NSEQ offset of each domain (+ total)
+    int32_t* d_dom_off = nullptr;     // first cell of each domain in device order (+ total)
+    double *d_ist_part = nullptr, *d_ist_out = nullptr;   // cmf_hip_instance_storage scratch
+    int ist_nch = 0;
     InstStep* h_inst = nullptr;       // pinned [ninst]
     unsigned long long* h_cfl = nullptr;  // pinned [ninst]
     std::vector<int32_t> last_nt;     // NT of each instance in the last advance
     std::vector<double> last_dt;
+    std::vector<double> last_dtmin;   // DT_MIN of CALC_ADPSTP of each instance (0 without LADPSTP)
     InstStep* d_inst = nullptr;
     double *d_rbase = nullptr, *d_rfac = nullptr;   // staging of cmf_hip_set_runoff_scaled
     RoffMap roff{};                                  // input matrix of cmf_hip_set_inpmat
@@ -79,6 +90,9 @@ struct cmf_hip_ctx {
     // CMF_HIP_ORDER=nseq); the C-ABI always speaks NSEQ order
     std::vector<int32_t> perm, iperm;
     bool permuted = false;
+    int32_t* d_perm = nullptr;        // perm on the device (NULL when not permuted)
+    void* stage = nullptr;            // staging buffer of host <-> device cell-array transfers
+    size_t stage_cap = 0;             // bytes
     // host index of every device slot: cells (ns), path levels (npthlev*nsp), paths (nsp)
     std::vector<int64_t> hx_cell, hx_pthl, hx_pth;
     // launch configuration
@@ -163,6 +177,7 @@ int alloc_state(cmf_hip_ctx* h, DevState<R>& S) {
     S.partials = nullptr;
     TRY(dalloc(h, &S.glb_hist, (size_t)h->hist_cap * NGLB));
     TRY(dalloc(h, &S.cfl, (size_t)h->ninst));
+    TRY(dalloc(h, &S.cfl_part, (ns + 255) / 256));                // one value per stage block
     TRY(dalloc(h, &h->d_inst, (size_t)h->ninst));
     S.inst = h->d_inst;
     TRY(dalloc(h, &S.bar, 2));
@@ -186,40 +201,159 @@ const std::vector<int64_t>& hx_of(const cmf_hip_ctx* h, Kind k) {
 // Host <-> device copies of per-slot arrays with element-type conversion. Host
 // arrays hold nmem member planes, each in NSEQ (or path) order; the device holds
 // the member-minor slots in device cell order.
-template <class R>
-int put(cmf_hip_ctx* h, R* dst, const double* src, Kind kind) {
+// size of one member plane of a host array of this kind
+size_t plane_of(const cmf_hip_ctx* h, Kind k) {
+    const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
+    return k == K_CELL ? (size_t)h->d.nseqall : (k == K_PTHL ? np * (size_t)h->d.npthlev : np);
+}
+
+int stage_bytes(cmf_hip_ctx* h, size_t bytes) {
+    if (bytes <= h->stage_cap) return 0;
+    char* p = nullptr;
+    TRY(dalloc(h, &p, bytes));
+    if (h->stage) {
+        (void)hipFree(h->stage);
+        for (auto& a : h->allocs) if (a == h->stage) a = nullptr;
+    }
+    h->stage = p;
+    h->stage_cap = bytes;
+    return 0;
+}
+
+// Host -> device copy of a per-slot array of element type H into R. Cell arrays go
+// to the device as the host holds them and are permuted and converted there;
+// path arrays are reordered on the host. shared: the host array holds one plane
+// that every member receives.
+template <class R, class H>
+int put_h(cmf_hip_ctx* h, R* dst, const H* src, Kind kind, bool shared = false) {
     const std::vector<int64_t>& hx = hx_of(h, kind);
     const size_t n = hx.size();
     if (!src || n == 0) return 0;
+    if (shared && h->nmem == 1) shared = false;
     const bool ident = !h->permuted && h->nmem == 1;
-    if (sizeof(R) == sizeof(double) && ident) {
-        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (std::is_same<R, H>::value && ident) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(R), hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;
     }
+    if (kind == K_CELL) {
+        const size_t hn = shared ? (size_t)h->d.nseqall : n;
+        TRY(stage_bytes(h, hn * sizeof(H)));
+        HIPCHK(hipMemcpyAsync(h->stage, src, hn * sizeof(H), hipMemcpyHostToDevice, h->stream));
+        launch_cells_from_host<R, H>(dst, static_cast<const H*>(h->stage), h->d_perm, h->d.nseqall, h->lgm, (int)n,
+                                     shared ? 1 : 0, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    const size_t plane = plane_of(h, kind);
     std::vector<R> tmp(n);
-    for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[hx[k]];
+    for (size_t k = 0; k < n; ++k) tmp[k] = (R)src[shared ? (size_t)hx[k] % plane : (size_t)hx[k]];
     HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
 template <class R>
-int get(cmf_hip_ctx* h, double* dst, const R* src, Kind kind) {
+int put(cmf_hip_ctx* h, R* dst, const double* src, Kind kind, bool shared = false) {
+    return put_h<R, double>(h, dst, src, kind, shared);
+}
+
+// Device -> host copy of a per-slot array into host element type H (cell arrays
+// permuted on the device, path arrays on the host)
+template <class R, class H>
+int get_h(cmf_hip_ctx* h, H* dst, const R* src, Kind kind) {
     const std::vector<int64_t>& hx = hx_of(h, kind);
     const size_t n = hx.size();
     if (!dst || n == 0) return 0;
     const bool ident = !h->permuted && h->nmem == 1;
-    if (sizeof(R) == sizeof(double) && ident) {
-        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (std::is_same<R, H>::value && ident) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (kind == K_CELL) {
+        TRY(stage_bytes(h, n * sizeof(H)));
+        launch_cells_to_host<R, H>(static_cast<H*>(h->stage), src, h->d_perm, h->d.nseqall, h->lgm, (int)n, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dst, h->stage, n * sizeof(H), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;
     }
     std::vector<R> tmp(n);
     HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (size_t k = 0; k < n; ++k) dst[hx[k]] = (double)tmp[k];
+    for (size_t k = 0; k < n; ++k) dst[hx[k]] = (H)tmp[k];
     return 0;
+}
+
+template <class R>
+int get(cmf_hip_ctx* h, double* dst, const R* src, Kind kind) {
+    return get_h<R, double>(h, dst, src, kind);
+}
+
+// JPRB host arrays are double, or float for a handle created with jprb = 4 (the
+// reference's -DSinglePrec_CMF build: D2* maps and diagnostics, D1PTHFLW, dam
+// and levee tables, D2RUNOFF are JPRB; the P2* storages are JPRD = double in
+// both builds). Widen gives double copies of float arrays for the duration of a
+// call; Narrow gives double buffers whose values finish() rounds into the
+// caller's float arrays.
+struct Widen {
+    const cmf_hip_ctx* h;
+    std::vector<std::vector<double>> keep;
+    explicit Widen(const cmf_hip_ctx* hh) : h(hh) {}
+    const double* operator()(const double* p, size_t n) {
+        if (!p || !h->hf32) return p;
+        const float* f = reinterpret_cast<const float*>(p);
+        keep.emplace_back(f, f + n);
+        return keep.back().data();
+    }
+};
+struct Narrow {
+    const cmf_hip_ctx* h;
+    std::vector<std::pair<float*, std::vector<double>>> keep;
+    explicit Narrow(const cmf_hip_ctx* hh) : h(hh) {}
+    double* operator()(double* p, size_t n) {
+        if (!p || !h->hf32) return p;
+        keep.emplace_back(reinterpret_cast<float*>(p), std::vector<double>(n));
+        return keep.back().second.data();
+    }
+    void finish() {
+        for (auto& k : keep)
+            for (size_t i = 0; i < k.second.size(); ++i) k.first[i] = (float)k.second[i];
+    }
+};
+
+// the JPRB members of cmf_state and the kind of array each is
+struct StateField { double* cmf_state::*f; int kind; };   // kind: 0 cells, 1 path levels, 2 paths
+const StateField JPRB_STATE[] = {
+    {&cmf_state::rivout, 0}, {&cmf_state::fldout, 0}, {&cmf_state::rivout_pre, 0}, {&cmf_state::fldout_pre, 0},
+    {&cmf_state::rivdph_pre, 0}, {&cmf_state::fldsto_pre, 0}, {&cmf_state::pthflw, 1}, {&cmf_state::pthflw_pre, 1},
+    {&cmf_state::rivdph, 0}, {&cmf_state::flddph, 0}, {&cmf_state::fldfrc, 0}, {&cmf_state::fldare, 0},
+    {&cmf_state::sfcelv, 0}, {&cmf_state::storge, 0}, {&cmf_state::outflw, 0}, {&cmf_state::rivvel, 0},
+    {&cmf_state::rivinf, 0}, {&cmf_state::fldinf, 0}, {&cmf_state::pthout, 0}, {&cmf_state::levdph, 0},
+    {&cmf_state::pthflwsum, 2}, {&cmf_state::rivout_avg, 0}, {&cmf_state::fldout_avg, 0},
+    {&cmf_state::outflw_avg, 0}, {&cmf_state::rivvel_avg, 0}, {&cmf_state::pthout_avg, 0},
+    {&cmf_state::runoff_avg, 0}, {&cmf_state::outflw_max, 0}, {&cmf_state::rivdph_max, 0},
+    {&cmf_state::storge_max, 0}, {&cmf_state::daminf_avg, 0}, {&cmf_state::pthflw_avg, 1},
+    {&cmf_state::pthflwsum_avg, 2}};
+
+size_t field_size(const cmf_hip_ctx* h, int kind, size_t planes) {
+    const size_t np = h->o.lpthout ? (size_t)h->d.npthout : 0;
+    const size_t one = kind == 0 ? (size_t)h->d.nseqall : (kind == 1 ? np * (size_t)h->d.npthlev : np);
+    return one * planes;
+}
+
+cmf_state widen_state(const cmf_hip_ctx* h, const cmf_state* st, size_t planes, Widen& w) {
+    cmf_state c = *st;
+    for (const StateField& f : JPRB_STATE) c.*(f.f) = const_cast<double*>(w(st->*(f.f), field_size(h, f.kind, planes)));
+    return c;
+}
+
+cmf_state narrow_state(const cmf_hip_ctx* h, const cmf_state* st, Narrow& nw) {
+    cmf_state c = *st;
+    for (const StateField& f : JPRB_STATE) c.*(f.f) = nw(st->*(f.f), field_size(h, f.kind, (size_t)h->nmem));
+    return c;
 }
 
 // per-cell host array in device order (planes of n cells)
@@ -288,14 +422,25 @@ bool dfs_order(int n, const std::vector<int32_t>& nxt, const std::vector<int32_t
 
 // Recompute RIVSTOMAX, RIVELV (maps:716-722) and SET_FLDSTG (maps:969-999) and check
 // the host tables bit for bit: the kernels derive them from the float32 sources.
+// A single-precision host (jprb = 4) computed its tables in float: they are
+// checked to 1e-5 of their magnitude instead (float rounding of the same
+// recurrence; a wrong table differs by far more).
 int validate_tables(const cmf_hip_ctx* h, const cmf_map* mp) {
     const int n = h->d.nseqall, nl = h->d.nlfp;
     const double dfr = 1.0 / (double)nl;
-    if (mp->dfrcinc != dfr) return fail(CMF_EMAP, "DFRCINC=%.17g differs from 1/NLFP", mp->dfrcinc);
+    const bool f32 = h->hf32;
+    auto differ = [f32](double want, double got) {
+        if (!f32) return want != got;
+        return std::fabs(want - got) > 1e-5 * std::fmax(std::fabs(want), std::fabs(got));
+    };
+    if (f32 ? mp->dfrcinc != (double)(float)dfr : mp->dfrcinc != dfr)
+        return fail(CMF_EMAP, "DFRCINC=%.17g differs from 1/NLFP", mp->dfrcinc);
     for (int i = 0; i < n; ++i) {
         const double rsm = h->o.lfplain ? (mp->rivlen[i] * mp->rivwth[i]) * mp->rivhgt[i] : (double)1.e18f;
-        if (rsm != mp->rivstomax[i]) return fail(CMF_EMAP, "RIVSTOMAX(%d) differs from RIVLEN*RIVWTH*RIVHGT", i + 1);
-        if (mp->elevtn[i] - mp->rivhgt[i] != mp->rivelv[i]) return fail(CMF_EMAP, "RIVELV(%d) differs from ELEVTN-RIVHGT", i + 1);
+        if (differ(rsm, mp->rivstomax[i])) return fail(CMF_EMAP, "RIVSTOMAX(%d) differs from RIVLEN*RIVWTH*RIVHGT", i + 1);
+        if (f32 ? std::fabs(mp->elevtn[i] - mp->rivhgt[i] - mp->rivelv[i]) > 1e-5 * (std::fabs(mp->elevtn[i]) + 1.0)
+                : mp->elevtn[i] - mp->rivhgt[i] != mp->rivelv[i])
+            return fail(CMF_EMAP, "RIVELV(%d) differs from ELEVTN-RIVHGT", i + 1);
         double stopre = rsm, hpre = 0.0;
         const double dwthinc = (mp->grarea[i] * (1.0 / mp->rivlen[i])) * dfr;
         for (int l = 0; l < nl; ++l) {
@@ -303,7 +448,7 @@ int validate_tables(const cmf_hip_ctx* h, const cmf_map* mp) {
             const double dstonow = (mp->rivlen[i] * (mp->rivwth[i] + dwthinc * ((double)(l + 1) - 0.5))) * (hl - hpre);
             const double smax = stopre + dstonow;
             const double grd = (hl - hpre) * (1.0 / dwthinc);
-            if (smax != mp->fldstomax[(size_t)l * n + i] || grd != mp->fldgrd[(size_t)l * n + i])
+            if (differ(smax, mp->fldstomax[(size_t)l * n + i]) || differ(grd, mp->fldgrd[(size_t)l * n + i]))
                 return fail(CMF_EMAP, "FLDSTOMAX/FLDGRD(%d,%d) differ from the SET_FLDSTG recurrence", i + 1, l + 1);
             stopre = smax; hpre = hl;
         }
@@ -329,9 +474,10 @@ template <>
 DevState<float>& state_of<float>(cmf_hip_ctx* h) { return h->S32; }
 
 template <class R>
-int set_state_t(cmf_hip_ctx* h, const cmf_state* st) {
+int set_state_t(cmf_hip_ctx* h, const cmf_state* st, bool shared = false) {
     DevState<R>& S = state_of<R>(h);
-    auto putd = [&](double* dst, const double* src) -> int { return put<double>(h, dst, src, K_CELL); };
+    auto putd = [&](double* dst, const double* src) -> int { return put<double>(h, dst, src, K_CELL, shared); };
+    auto putr = [&](R* dst, const double* src, Kind k) -> int { return put<R>(h, dst, src, k, shared); };
     TRY(putd(S.rivsto, st->rivsto));
     TRY(putd(S.fldsto, st->fldsto));
     if (h->o.llevee) TRY(putd(S.levsto, st->levsto));
@@ -339,29 +485,54 @@ int set_state_t(cmf_hip_ctx* h, const cmf_state* st) {
     // D2RIVOUT_PRE lives in the RIVOUT array; a different D2RIVOUT is kept for the
     // RIVVEL of the next substep only (outflw:75 reads the pre-update RIVOUT).
     const double* rpre = st->rivout_pre ? st->rivout_pre : st->rivout;
-    TRY(put<R>(h, S.rivout, rpre, K_CELL));
+    TRY(putr(S.rivout, rpre, K_CELL));
     if (st->rivout) {
-        TRY(put<R>(h, (R*)h->vel_buf, st->rivout, K_CELL));
+        TRY(putr((R*)h->vel_buf, st->rivout, K_CELL));
         h->vel_pending = true;
     }
     const double* fpre = st->fldout_pre ? st->fldout_pre : st->fldout;
-    TRY(put<R>(h, S.fldout, fpre, K_CELL));
-    TRY(put<R>(h, S.rivdph_pre, st->rivdph_pre, K_CELL));
-    TRY(put<R>(h, S.fldsto_pre, st->fldsto_pre, K_CELL));
+    TRY(putr(S.fldout, fpre, K_CELL));
+    // a D2FLDOUT given beside D2FLDOUT_PRE (a restart zeroes it, physics:254-276
+    // makes the two equal again in the next substep) is kept for get_state only
+    if (st->fldout && st->fldout_pre) {
+        if (!h->fld_buf) {
+            R* fb = nullptr;
+            TRY(dalloc(h, &fb, (size_t)h->M.ns));
+            h->fld_buf = fb;
+        }
+        TRY(putr((R*)h->fld_buf, st->fldout, K_CELL));
+        h->fld_pending = true;
+    } else if (fpre) {
+        h->fld_pending = false;
+    }
+    TRY(putr(S.rivdph_pre, st->rivdph_pre, K_CELL));
+    TRY(putr(S.fldsto_pre, st->fldsto_pre, K_CELL));
     if (h->o.lpthout) {
         const double* ppre = st->pthflw_pre ? st->pthflw_pre : st->pthflw;
-        TRY(put<R>(h, S.pthflw, ppre, K_PTHL));
-        TRY(put<R>(h, S.pthflwsum, st->pthflwsum, K_PTH));
+        TRY(putr(S.pthflw, ppre, K_PTHL));
+        // a D1PTHFLW given beside D1PTHFLW_PRE is kept for get_state only, as D2FLDOUT
+        if (st->pthflw && st->pthflw_pre) {
+            if (!h->pth_buf) {
+                R* pb = nullptr;
+                TRY(dalloc(h, &pb, (size_t)h->M.nsp * (size_t)h->d.npthlev));
+                h->pth_buf = pb;
+            }
+            TRY(putr((R*)h->pth_buf, st->pthflw, K_PTHL));
+            h->pth_pending = true;
+        } else if (ppre) {
+            h->pth_pending = false;
+        }
+        TRY(putr(S.pthflwsum, st->pthflwsum, K_PTH));
     }
-    TRY(put<R>(h, S.rivdph, st->rivdph, K_CELL));
-    TRY(put<R>(h, S.flddph, st->flddph, K_CELL));
-    TRY(put<R>(h, S.fldfrc, st->fldfrc, K_CELL));
-    TRY(put<R>(h, S.fldare, st->fldare, K_CELL));
-    TRY(put<R>(h, S.sfcelv, st->sfcelv, K_CELL));
-    TRY(put<R>(h, S.storge, st->storge, K_CELL));
-    TRY(put<R>(h, S.outflw, st->outflw, K_CELL));
-    TRY(put<R>(h, S.rivvel, st->rivvel, K_CELL));
-    if (h->o.llevee) TRY(put<R>(h, S.levdph, st->levdph, K_CELL));
+    TRY(putr(S.rivdph, st->rivdph, K_CELL));
+    TRY(putr(S.flddph, st->flddph, K_CELL));
+    TRY(putr(S.fldfrc, st->fldfrc, K_CELL));
+    TRY(putr(S.fldare, st->fldare, K_CELL));
+    TRY(putr(S.sfcelv, st->sfcelv, K_CELL));
+    TRY(putr(S.storge, st->storge, K_CELL));
+    TRY(putr(S.outflw, st->outflw, K_CELL));
+    TRY(putr(S.rivvel, st->rivvel, K_CELL));
+    if (h->o.llevee) TRY(putr(S.levdph, st->levdph, K_CELL));
     TRY(putd(S.rivinf, st->rivinf));
     TRY(putd(S.fldinf, st->fldinf));
     TRY(putd(S.pthout, st->pthout));
@@ -379,12 +550,12 @@ int get_state_t(cmf_hip_ctx* h, cmf_state* st) {
     if (h->o.ldamout) { TRY(getd(st->damsto, S.damsto)); TRY(getd(st->daminf, S.daminf)); }
     TRY(get<R>(h, st->rivout, h->vel_pending ? (const R*)h->vel_buf : S.rivout, K_CELL));
     TRY(get<R>(h, st->rivout_pre, S.rivout, K_CELL));
-    TRY(get<R>(h, st->fldout, S.fldout, K_CELL));
+    TRY(get<R>(h, st->fldout, h->fld_pending ? (const R*)h->fld_buf : S.fldout, K_CELL));
     TRY(get<R>(h, st->fldout_pre, S.fldout, K_CELL));
     TRY(get<R>(h, st->rivdph_pre, S.rivdph_pre, K_CELL));
     TRY(get<R>(h, st->fldsto_pre, S.fldsto_pre, K_CELL));
     if (h->o.lpthout) {
-        TRY(get<R>(h, st->pthflw, S.pthflw, K_PTHL));
+        TRY(get<R>(h, st->pthflw, h->pth_pending ? (const R*)h->pth_buf : S.pthflw, K_PTHL));
         TRY(get<R>(h, st->pthflw_pre, S.pthflw, K_PTHL));
         TRY(get<R>(h, st->pthflwsum, S.pthflwsum, K_PTH));
         TRY(get<R>(h, st->pthflw_avg, S.pthflw_avg, K_PTHL));
@@ -451,10 +622,19 @@ int coop_capacity(cmf_hip_ctx* h, int bs, int* out) {
 // block size of the update phase (its tiles carry the budget partial sums)
 int upd_bs(const cmf_hip_ctx* h) { return h->tune.fused ? h->tune.bs_fused : h->tune.bs_update; }
 
+// captured substep graphs hold the device pointers of the state and scratch
+// arrays they were recorded with: drop them whenever one of those changes
+void drop_graphs(cmf_hip_ctx* h) {
+    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
 template <class R>
 int grow_partials(cmf_hip_ctx* h, DevState<R>& S, int nrun) {
     const size_t need = (size_t)nrun * tiles_cells(h->M, upd_bs(h)) * NGLB;
     if (need > h->partial_cap) {
+        // k_update of a cached graph writes into S.partials: the old buffer is freed here
+        drop_graphs(h);
         const size_t cap = need + need / 4;
         double* q = nullptr;
         TRY(dalloc(h, &q, cap));
@@ -466,6 +646,7 @@ int grow_partials(cmf_hip_ctx* h, DevState<R>& S, int nrun) {
         h->partial_cap = cap;
     }
     if (nrun > h->hist_cap) {
+        drop_graphs(h);
         const int cap = nrun + nrun / 4;
         double* p = nullptr;
         TRY(dalloc(h, &p, (size_t)cap * NGLB));
@@ -500,6 +681,7 @@ int enqueue_substeps(cmf_hip_ctx* h, DevState<R>& S, int32_t ntmax, bool vel_fir
     StepArgs A = base_args(h);
     for (int32_t it = 0; it < ntmax; ++it) {
         A.it = it;
+        A.prow = it;
         DevState<R> Sk = S;
         Sk.rivdph_next = S.rivdph_pre;
         Sk.rivout_vel = (vel_first && it == 0) ? (const R*)h->vel_buf : S.rivout;
@@ -538,6 +720,41 @@ int run_substeps_graph(cmf_hip_ctx* h, DevState<R>& S, int32_t ntmax) {
     return 0;
 }
 
+// NT and DT of every instance from the CFL bounds in h->h_cfl (CALC_ADPSTP,
+// physics:205-206), written to the host step table h->h_inst; returns the
+// largest substep count and whether an instance needs its RIVDPH pair swapped back
+int plan_step(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* ntmax_out, bool* anyfix_out) {
+    const int ni = h->ninst;
+    int32_t ntmax = 0;
+    for (int k = 0; k < ni; ++k) {
+        int32_t nt = 1;
+        double dts = dt_def;
+        h->last_dtmin[(size_t)k] = 0.0;
+        if (h->o.ladpstp) {
+            double dt_min;
+            std::memcpy(&dt_min, &h->h_cfl[k], sizeof dt_min);
+            h->last_dtmin[(size_t)k] = dt_min;
+            nt = (int32_t)(dt_def * (1.0 / dt_min) - K_0P01F) + 1;       // physics:205
+            dts = dt_def * (double)(1.0f / (float)nt);                   // physics:206
+        }
+        h->last_nt[(size_t)k] = nt;
+        h->last_dt[(size_t)k] = dts;
+        const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
+        double nadd = 0.0;                                               // diag:108
+        for (int32_t it = 0; it < nrun; ++it) nadd = nadd + dts;
+        h->h_inst[k] = InstStep{dts, h->o.pgrv * dts, nadd, nrun, 0};
+        if (nrun > ntmax) ntmax = nrun;
+    }
+    bool anyfix = false;
+    for (int k = 0; k < ni; ++k) {
+        h->h_inst[k].swapfix = (ntmax - h->h_inst[k].nt) & 1;
+        anyfix = anyfix || h->h_inst[k].swapfix;
+    }
+    *ntmax_out = ntmax;
+    *anyfix_out = anyfix;
+    return 0;
+}
+
 // One outer step of every instance. nt_cap > 0 caps the substeps (timing).
 template <class R>
 int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, double* dt_out, double* glb,
@@ -557,28 +774,8 @@ int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, do
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     int32_t ntmax = 0;
-    for (int k = 0; k < ni; ++k) {
-        int32_t nt = 1;
-        double dts = dt_def;
-        if (h->o.ladpstp) {
-            double dt_min;
-            std::memcpy(&dt_min, &h->h_cfl[k], sizeof dt_min);
-            nt = (int32_t)(dt_def * (1.0 / dt_min) - K_0P01F) + 1;       // physics:205
-            dts = dt_def * (double)(1.0f / (float)nt);                   // physics:206
-        }
-        h->last_nt[(size_t)k] = nt;
-        h->last_dt[(size_t)k] = dts;
-        const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
-        double nadd = 0.0;                                               // diag:108
-        for (int32_t it = 0; it < nrun; ++it) nadd = nadd + dts;
-        h->h_inst[k] = InstStep{dts, h->o.pgrv * dts, nadd, nrun, 0};
-        if (nrun > ntmax) ntmax = nrun;
-    }
     bool anyfix = false;
-    for (int k = 0; k < ni; ++k) {
-        h->h_inst[k].swapfix = (ntmax - h->h_inst[k].nt) & 1;
-        anyfix = anyfix || h->h_inst[k].swapfix;
-    }
+    TRY(plan_step(h, dt_def, nt_cap, &ntmax, &anyfix));
     HIPCHK(hipMemcpyAsync(h->d_inst, h->h_inst, (size_t)ni * sizeof(InstStep), hipMemcpyHostToDevice, h->stream));
     if (A.budget) TRY(grow_partials<R>(h, S, ntmax));
     const bool vel_first = h->vel_pending;
@@ -592,6 +789,8 @@ int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, do
         launch_budget(S.partials, tiles_cells(h->M, upd_bs(h)), ntmax, S.glb_hist, h->stream);
     HIPCHK(hipGetLastError());
     h->vel_pending = false;
+    h->fld_pending = false;
+    h->pth_pending = false;
     if (A.budget && glb && glb_cap > 0) {
         const int32_t m = ntmax < glb_cap ? ntmax : glb_cap;
         HIPCHK(hipMemcpyAsync(glb, S.glb_hist, (size_t)m * NGLB * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -599,8 +798,14 @@ int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, do
     if (h->tune.fused)
         HIPCHK(hipMemcpyAsync(h->h_ctl, S.err, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (h->tune.fused && *reinterpret_cast<int32_t*>(h->h_ctl) != 0)
+    if (h->tune.fused && *reinterpret_cast<int32_t*>(h->h_ctl) != 0) {
+        // clear the flag and the barrier counters (arrival count, generation) so
+        // that the next fused launch starts from a consistent barrier
+        HIPCHK(hipMemsetAsync(S.err, 0, sizeof(int32_t), h->stream));
+        HIPCHK(hipMemsetAsync(S.bar, 0, 2 * sizeof(unsigned), h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
         return fail(CMF_EHIP, "grid barrier of the fused substep kernel timed out");
+    }
     if (h->timing) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
@@ -652,7 +857,7 @@ int autotune_t(cmf_hip_ctx* h, double dt_def, int32_t nsub, cmf_tune* best, doub
     DevState<R>& S = state_of<R>(h);
     const cmf_tune keep = h->tune;
     const bool keep_timing = h->timing;
-    const bool keep_vel = h->vel_pending;
+    const bool keep_vel = h->vel_pending, keep_fld = h->fld_pending, keep_pth = h->pth_pending;
     // an advance moves only the RIVDPH buffer pair among the state pointers;
     // scratch pointers (budget partials, history) may be reallocated by a trial
     // and must keep their new values
@@ -670,12 +875,12 @@ int autotune_t(cmf_hip_ctx* h, double dt_def, int32_t nsub, cmf_tune* best, doub
         int32_t nt;
         double dto;
         // one untimed outer step warms the code, then the timed one
-        restore_ptrs(); h->vel_pending = keep_vel;
+        restore_ptrs(); h->vel_pending = keep_vel; h->fld_pending = keep_fld; h->pth_pending = keep_pth;
         TRY(snapshot(h, true));
         TRY(advance_t<R>(h, dt_def, nsub, nullptr, nullptr, nullptr, 0));
         (void)nt; (void)dto;
         for (int k = 0; k < 5; ++k) h->t_ms[k] = 0.0;
-        restore_ptrs(); h->vel_pending = keep_vel;
+        restore_ptrs(); h->vel_pending = keep_vel; h->fld_pending = keep_fld; h->pth_pending = keep_pth;
         TRY(snapshot(h, true));
         TRY(advance_t<R>(h, dt_def, nsub, nullptr, nullptr, nullptr, 0));
         for (int k = 0; k < 3; ++k) per_phase[k] = h->t_ms[k];
@@ -733,10 +938,9 @@ int autotune_t(cmf_hip_ctx* h, double dt_def, int32_t nsub, cmf_tune* best, doub
             t.graph = g;
             h->tune = t;
             h->timing = false;
-            for (auto& e : h->graphs) (void)hipGraphExecDestroy(e.exec);
-            h->graphs.clear();
+            drop_graphs(h);
             for (int rep = 0; rep < 2; ++rep) {
-                restore_ptrs(); h->vel_pending = keep_vel;
+                restore_ptrs(); h->vel_pending = keep_vel; h->fld_pending = keep_fld; h->pth_pending = keep_pth;
                 TRY(snapshot(h, true));
                 const auto c0 = std::chrono::steady_clock::now();
                 TRY(advance_t<R>(h, dt_def, nsub, nullptr, nullptr, nullptr, 0));
@@ -751,13 +955,14 @@ int autotune_t(cmf_hip_ctx* h, double dt_def, int32_t nsub, cmf_tune* best, doub
             ++nrep;
         }
         bt.graph = wall[1] < wall[0] ? 1 : 0;
-        for (auto& e : h->graphs) (void)hipGraphExecDestroy(e.exec);
-        h->graphs.clear();
+        drop_graphs(h);
     }
     // restore the state and the caller's settings; keep the best configuration
     restore_ptrs();
     TRY(snapshot(h, true));
     h->vel_pending = keep_vel;
+    h->fld_pending = keep_fld;
+    h->pth_pending = keep_pth;
     h->timing = keep_timing;
     std::memcpy(h->t_ms, keep_t, sizeof keep_t);
     h->tune = bt;
@@ -786,6 +991,8 @@ int cmf_hip_create_ensemble(const cmf_dims* dims, const cmf_options* opts, const
     if (opts->ldamout && dims->ndam <= 0) return fail(CMF_EINVAL, "LDAMOUT needs NDAM>0");
     if (opts->precision != CMF_PREC_FP64 && opts->precision != CMF_PREC_MIXED)
         return fail(CMF_EINVAL, "unknown precision %d", opts->precision);
+    if (opts->jprb != 0 && opts->jprb != 4 && opts->jprb != 8)
+        return fail(CMF_EINVAL, "jprb=%d: host real kind must be 4 or 8 bytes (0 = 8)", opts->jprb);
     int ndom = 1, nmem = 1;
     std::vector<int32_t> off{0, dims->nseqall};
     if (ens) {
@@ -807,11 +1014,13 @@ int cmf_hip_create_ensemble(const cmf_dims* dims, const cmf_options* opts, const
     cmf_hip_ctx* h = new cmf_hip_ctx();
     h->d = *dims;
     h->o = *opts;
+    h->hf32 = opts->jprb == 4;
     h->ndom = ndom; h->nmem = nmem; h->ninst = ndom * nmem;
     while ((1 << h->lgm) < nmem) ++h->lgm;
     h->dom_off = off;
     h->last_nt.assign((size_t)h->ninst, 0);
     h->last_dt.assign((size_t)h->ninst, 0.0);
+    h->last_dtmin.assign((size_t)h->ninst, 0.0);
     h->tune = cmf_tune{256, 256, 256, 0, 256, 0, 0, 0};
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
@@ -853,17 +1062,38 @@ void cmf_hip_destroy(cmf_handle h) {
     delete h;
 }
 
-int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
-    if (!h || !mp) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_set_map(cmf_handle h, const cmf_map* mp_in) {
+    if (!h || !mp_in) return fail(CMF_EINVAL, "NULL argument");
     if (h->map_set) return fail(CMF_ESTATE, "map already set for this handle");
     const int n = h->d.nseqall, nl = h->d.nlfp;
+    // JPRB tables of a single-precision host, widened for this call
+    Widen w(h);
+    cmf_map wm = *mp_in;
+    {
+        const size_t cn = (size_t)n, np = h->o.lpthout ? (size_t)h->d.npthout : 0;
+        const size_t nlv = h->o.lpthout ? (size_t)h->d.npthlev : 0, nd = h->o.ldamout ? (size_t)h->d.ndam : 0;
+        for (const double** f : {&wm.grarea, &wm.elevtn, &wm.nxtdst, &wm.rivlen, &wm.rivwth, &wm.rivhgt, &wm.rivman,
+                                 &wm.rivstomax, &wm.rivelv, &wm.dwnelv, &wm.levhgt, &wm.levfrc, &wm.bashgt, &wm.levdst,
+                                 &wm.levbassto, &wm.levtopsto, &wm.levfilsto})
+            *f = w(*f, cn);
+        for (const double** f : {&wm.fldhgt, &wm.fldstomax, &wm.fldgrd}) *f = w(*f, cn * (size_t)nl);
+        wm.pth_dst = w(wm.pth_dst, np);
+        wm.pth_elv = w(wm.pth_elv, np * nlv);
+        wm.pth_wth = w(wm.pth_wth, np * nlv);
+        wm.pth_man = w(wm.pth_man, nlv);
+        for (const double** f : {&wm.dam_convol, &wm.dam_emevol, &wm.dam_adjvol, &wm.dam_qa, &wm.dam_qn, &wm.dam_qf})
+            *f = w(*f, nd);
+    }
+    const cmf_map* mp = &wm;
     if (!mp->i1next || !mp->rivstomax || !mp->rivelv || !mp->dwnelv || !mp->fldhgt || !mp->fldstomax ||
         !mp->fldgrd || !mp->i2mask)
         return fail(CMF_EINVAL, "required map array missing");
     DevMap& M = h->M;
     M.n = n; M.nseqriv = h->d.nseqriv; M.nlfp = nl;
     M.lfplain = h->o.lfplain;
-    M.dfrcinc = mp->dfrcinc;
+    // a single-precision host holds DFRCINC = 1/NLFP rounded to float; the device
+    // tables use the FP64 value of both builds' recurrence (maps:984-995)
+    M.dfrcinc = h->hf32 ? 1.0 / (double)nl : mp->dfrcinc;
     M.lgm = h->lgm; M.nmem = h->nmem;
     M.ns = n * h->nmem;
     // domain of every NSEQ cell
@@ -900,6 +1130,7 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
     if (!h->permuted) { h->perm.resize((size_t)n); for (int i = 0; i < n; ++i) h->perm[i] = i; }
     h->iperm.assign((size_t)n, 0);
     for (int p = 0; p < n; ++p) h->iperm[(size_t)h->perm[p]] = p;
+    if (h->permuted) TRY(put_raw(h, &h->d_perm, h->perm.data(), (size_t)n));
     const std::vector<int32_t>& P = h->perm;
     const std::vector<int32_t>& IP = h->iperm;
     // host index of every device slot
@@ -1054,33 +1285,96 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp) {
         TRY(put_raw(h, &pd, mp->dam_qn, (size_t)nd)); M.dam_qn = pd;
         TRY(put_raw(h, &pd, mp->dam_qf, (size_t)nd)); M.dam_qf = pd;
     }
+    {
+        // first cell of every domain in device order (domains stay contiguous)
+        std::vector<int32_t> doff((size_t)h->ndom + 1, 0);
+        for (int p = 0; p < n; ++p) doff[(size_t)ddom[(size_t)p] + 1] += 1;
+        for (int d = 0; d < h->ndom; ++d) doff[(size_t)d + 1] += doff[(size_t)d];
+        for (int p = 1; p < n; ++p)
+            if (ddom[(size_t)p] < ddom[(size_t)p - 1]) return fail(CMF_EMAP, "domains are not contiguous in device order");
+        TRY(put_raw(h, &h->d_dom_off, doff.data(), doff.size()));
+    }
     h->map_set = true;
     return 0;
 }
 
+int cmf_hip_instance_storage(cmf_handle h, double* out) {
+    if (!h || !out) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (!h->d_ist_out) {
+        // chunks per instance: about 2048 blocks in all, each over at least 1024 cells
+        int nch = 2048 / h->ninst;
+        const int maxch = (h->d.nseqall / h->ndom) / 1024;
+        if (nch > maxch) nch = maxch;
+        if (nch < 1) nch = 1;
+        h->ist_nch = nch;
+        TRY(dalloc(h, &h->d_ist_part, (size_t)h->ninst * (size_t)nch));
+        TRY(dalloc(h, &h->d_ist_out, (size_t)h->ninst));
+    }
+    const double *rs, *fs, *ls;
+    if (h->o.precision == CMF_PREC_FP64) { rs = h->S64.rivsto; fs = h->S64.fldsto; ls = h->o.llevee ? h->S64.levsto : nullptr; }
+    else { rs = h->S32.rivsto; fs = h->S32.fldsto; ls = h->o.llevee ? h->S32.levsto : nullptr; }
+    launch_inst_storage(rs, fs, ls, h->d_dom_off, h->lgm, h->ninst, h->ist_nch, h->d_ist_part, h->d_ist_out, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, h->d_ist_out, (size_t)h->ninst * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+namespace {
+// state copies with double host arrays (the library's own callers)
+int set_state_impl(cmf_hip_ctx* h, const cmf_state* st, bool shared) {
+    return h->o.precision == CMF_PREC_FP64 ? set_state_t<double>(h, st, shared) : set_state_t<float>(h, st, shared);
+}
+int get_state_impl(cmf_hip_ctx* h, cmf_state* st) {
+    return h->o.precision == CMF_PREC_FP64 ? get_state_t<double>(h, st) : get_state_t<float>(h, st);
+}
+}  // namespace
+
 int cmf_hip_set_state(cmf_handle h, const cmf_state* st) {
     if (!h || !st) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
-    return h->o.precision == CMF_PREC_FP64 ? set_state_t<double>(h, st) : set_state_t<float>(h, st);
+    Widen w(h);
+    const cmf_state c = widen_state(h, st, (size_t)h->nmem, w);
+    return set_state_impl(h, &c, false);
+}
+
+int cmf_hip_set_state_shared(cmf_handle h, const cmf_state* st) {
+    if (!h || !st) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    Widen w(h);
+    const cmf_state c = widen_state(h, st, 1, w);
+    return set_state_impl(h, &c, true);
 }
 
 int cmf_hip_get_state(cmf_handle h, cmf_state* st) {
     if (!h || !st) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
-    return h->o.precision == CMF_PREC_FP64 ? get_state_t<double>(h, st) : get_state_t<float>(h, st);
+    Narrow nw(h);
+    cmf_state c = narrow_state(h, st, nw);
+    TRY(get_state_impl(h, &c));
+    nw.finish();
+    return 0;
 }
 
-int cmf_hip_set_runoff(cmf_handle h, const double* runoff) {
-    if (!h || !runoff) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_set_runoff(cmf_handle h, const double* runoff_in) {
+    if (!h || !runoff_in) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
-    return h->o.precision == CMF_PREC_FP64 ? put<double>(h, h->S64.runoff, runoff, K_CELL)
-                                           : put<float>(h, h->S32.runoff, runoff, K_CELL);
+    if (h->hf32) {
+        const float* r = reinterpret_cast<const float*>(runoff_in);
+        return h->o.precision == CMF_PREC_FP64 ? put_h<double, float>(h, h->S64.runoff, r, K_CELL)
+                                               : put_h<float, float>(h, h->S32.runoff, r, K_CELL);
+    }
+    return h->o.precision == CMF_PREC_FP64 ? put<double>(h, h->S64.runoff, runoff_in, K_CELL)
+                                           : put<float>(h, h->S32.runoff, runoff_in, K_CELL);
 }
 
-int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* factor) {
-    if (!h || !base || !factor) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base_in, const double* factor) {
+    if (!h || !base_in || !factor) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     const size_t n = (size_t)h->d.nseqall;
+    Widen w(h);
+    const double* base = w(base_in, n);
     if (!h->d_rbase) {
         TRY(dalloc(h, &h->d_rbase, n));
         TRY(dalloc(h, &h->d_rfac, (size_t)h->ninst));
@@ -1096,8 +1390,10 @@ int cmf_hip_set_runoff_scaled(cmf_handle h, const double* base, const double* fa
 }
 
 int cmf_hip_set_inpmat(cmf_handle h, int32_t inpn, int32_t nxin, int32_t nyin, const int32_t* inpx,
-                       const int32_t* inpy, const double* inpa, double drofunit, double rmis) {
-    if (!h || !inpx || !inpy || !inpa) return fail(CMF_EINVAL, "NULL argument");
+                       const int32_t* inpy, const double* inpa_in, double drofunit, double rmis) {
+    if (!h || !inpx || !inpy || !inpa_in) return fail(CMF_EINVAL, "NULL argument");
+    Widen w(h);
+    const double* inpa = inpn > 0 ? w(inpa_in, (size_t)h->d.nseqall * (size_t)inpn) : inpa_in;
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     if (h->roff.inpx) return fail(CMF_ESTATE, "the input matrix is already set");
     if (inpn <= 0 || nxin <= 0 || nyin <= 0) return fail(CMF_EINVAL, "bad INPN/NXIN/NYIN (%d, %d, %d)", inpn, nxin, nyin);
@@ -1128,10 +1424,12 @@ int cmf_hip_interp_runoff(cmf_handle h, const double* d_grid) {
     return 0;
 }
 
-int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid) {
-    if (!h || !grid) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid_in) {
+    if (!h || !grid_in) return fail(CMF_EINVAL, "NULL argument");
     if (!h->roff.inpx) return fail(CMF_ESTATE, "cmf_hip_set_inpmat must come first");
     const size_t ng = (size_t)h->roff.nxin * (size_t)h->roff.nyin;
+    Widen w(h);
+    const double* grid = w(grid_in, ng);
     if (h->grid_cap < ng) {
         TRY(dalloc(h, &h->d_grid, ng));
         h->grid_cap = ng;
@@ -1191,9 +1489,8 @@ int cmf_hip_diag_getave_output(cmf_handle h, double* nadd_out) {
     return 0;
 }
 
-int cmf_hip_get_output(cmf_handle h, const cmf_output* o) {
-    if (!h || !o) return fail(CMF_EINVAL, "NULL argument");
-    if (!h->out.rivout) return fail(CMF_ESTATE, "no output period: call cmf_hip_diag_avemax_output first");
+namespace {
+int get_output_impl(cmf_hip_ctx* h, const cmf_output* o) {
     const std::pair<double*, const double*> cell[] = {
         {o->rivout, h->out.rivout}, {o->fldout, h->out.fldout}, {o->outflw, h->out.outflw},
         {o->rivvel, h->out.rivvel}, {o->pthout, h->out.pthout}, {o->runoff, h->out.runoff},
@@ -1203,21 +1500,41 @@ int cmf_hip_get_output(cmf_handle h, const cmf_output* o) {
     if (o->pthflw && npth_slots(h) > 0) TRY(get<double>(h, o->pthflw, h->out.pthflw, K_PTHL));
     return 0;
 }
+}  // namespace
 
-int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv) {
-    if (!h || !dwnelv) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_get_output(cmf_handle h, const cmf_output* o) {
+    if (!h || !o) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->out.rivout) return fail(CMF_ESTATE, "no output period: call cmf_hip_diag_avemax_output first");
+    Narrow nw(h);
+    cmf_output c = *o;
+    const size_t ns = (size_t)h->M.ns;
+    for (double** f : {&c.rivout, &c.fldout, &c.outflw, &c.rivvel, &c.pthout, &c.runoff, &c.daminf, &c.outflw_max,
+                       &c.rivdph_max, &c.storge_max})
+        *f = nw(*f, ns);
+    c.pthflw = nw(c.pthflw, field_size(h, 1, (size_t)h->nmem));
+    TRY(get_output_impl(h, &c));
+    nw.finish();
+    return 0;
+}
+
+int cmf_hip_set_dwnelv(cmf_handle h, const double* dwnelv_in) {
+    if (!h || !dwnelv_in) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     const size_t n = (size_t)h->d.nseqall;
+    Widen w(h);
+    const double* dwnelv = w(dwnelv_in, n);
     std::vector<double> v = to_dev(h, dwnelv, n);
     HIPCHK(hipMemcpyAsync(h->dwnelv_dev, v.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
-int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf) {
-    if (!h || !upsinf) return fail(CMF_EINVAL, "NULL argument");
+int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf_in) {
+    if (!h || !upsinf_in) return fail(CMF_EINVAL, "NULL argument");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     const size_t ns = (size_t)h->M.ns;
+    Widen w(h);
+    const double* upsinf = w(upsinf_in, ns);
     if (h->o.precision == CMF_PREC_FP64) {
         if (!h->S64.upsinf) TRY(dalloc(h, &h->S64.upsinf, ns));
         TRY(put<double>(h, h->S64.upsinf, upsinf, K_CELL));
@@ -1227,9 +1544,7 @@ int cmf_hip_set_upsinf(cmf_handle h, const double* upsinf) {
     }
     if (!h->upsinf_on) {
         h->upsinf_on = 1;
-        // captured substep graphs hold the old arguments
-        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
-        h->graphs.clear();
+        drop_graphs(h);                                  // captured graphs hold the old arguments
     }
     return 0;
 }
@@ -1239,17 +1554,18 @@ int cmf_hip_set_kinematic(cmf_handle h, int32_t lkine) {
     if (lkine != 0 && lkine != 1) return fail(CMF_EINVAL, "lkine=%d must be 0 or 1", (int)lkine);
     if (h->lkine != lkine) {
         h->lkine = lkine;
-        // captured substep graphs hold the old arguments
-        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
-        h->graphs.clear();
+        drop_graphs(h);                                  // captured graphs hold the old arguments
     }
     return 0;
 }
 
-int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* sealev) {
+int cmf_hip_update_boundary(cmf_handle h, const double* meansl_in, const double* sealev_in) {
     if (!h) return fail(CMF_EINVAL, "NULL handle");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     const size_t n = (size_t)h->d.nseqall;
+    Widen w(h);
+    const double* meansl = w(meansl_in, n);
+    const double* sealev = w(sealev_in, n);
     const double *dm = nullptr, *dsl = nullptr;
     if (meansl) {
         if (!h->d_meansl) TRY(dalloc(h, &h->d_meansl, n));
@@ -1314,7 +1630,7 @@ int cmf_hip_restart_write(cmf_handle h, const char* file, int32_t nx, int32_t ny
     for (size_t k = 0; k < fields.size(); ++k) *state_slot(&st, fields[k]) = vec[k].data();
     std::vector<double> pth;
     if (h->o.lpthout) { pth.resize((size_t)h->d.npthout * h->d.npthlev); st.pthflw_pre = pth.data(); }
-    TRY(cmf_hip_get_state(h, &st));
+    TRY(get_state_impl(h, &st));
     FILE* f = std::fopen(file, "wb");
     if (!f) return fail(CMF_EIO, "cannot open %s for writing", file);
     bool ok = true;
@@ -1380,7 +1696,7 @@ int cmf_hip_output_write(cmf_handle h, const char* var, const char* file, int32_
         else if (v == "maxdph") o.rivdph_max = p;
         else if (v == "maxsto") o.storge_max = p;
         else o.pthflw = p;
-        TRY(cmf_hip_get_output(h, &o));
+        TRY(get_output_impl(h, &o));
     } else {
         cmf_state st{};
         vec.resize(n);
@@ -1397,7 +1713,7 @@ int cmf_hip_output_write(cmf_handle h, const char* var, const char* file, int32_
         else if (v == "levsto") st.levsto = p;
         else if (v == "levdph") st.levdph = p;
         else return fail(CMF_EINVAL, "unknown output variable %s", var);
-        TRY(cmf_hip_get_state(h, &st));
+        TRY(get_state_impl(h, &st));
     }
     std::vector<float> rec;
     if (v == "pthflw" || loutvec) {                                   // WRTE_OUTPTH / WRTE_OUTVEC
@@ -1438,10 +1754,31 @@ int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny,
     }
     std::fclose(f);
     if (!ok) return fail(CMF_EIO, "%s is shorter than %zu records of %d x %d", file, fields.size(), nx, ny);
+    // CMF_RESTART_INIT (restart:100-132): every array the file does not carry is
+    // zero (D2RIVOUT and D2FLDOUT always, so the first substep's RIVVEL uses 0;
+    // the four t-1 records and D1PTHFLW_PRE with LSTOONLY), then with LSTOONLY
+    // D2FLDSTO_PRE = P2FLDSTO
     cmf_state st{};
     for (size_t k = 0; k < fields.size(); ++k) *state_slot(&st, fields[k]) = vec[k].data();
-    std::vector<double> pth;
-    if (h->o.lpthout) {
+    std::vector<double> zero(n, 0.0), fpre;
+    st.rivout = zero.data();
+    st.fldout = zero.data();
+    if (lstoonly) {
+        fpre = vec[1];                                                // P2FLDSTO
+        st.rivout_pre = zero.data();
+        st.fldout_pre = zero.data();
+        st.rivdph_pre = zero.data();
+        st.fldsto_pre = fpre.data();
+    }
+    std::vector<double> pth, pth0;
+    if (h->o.lpthout) {                                               // D1PTHFLW = 0 (restart:111-114)
+        pth0.assign((size_t)h->d.npthout * h->d.npthlev, 0.0);
+        st.pthflw = pth0.data();
+    }
+    if (h->o.lpthout && lstoonly) {                                   // restart:212-214
+        pth.assign((size_t)h->d.npthout * h->d.npthlev, 0.0);
+        st.pthflw_pre = pth.data();
+    } else if (h->o.lpthout) {
         pth.resize((size_t)h->d.npthout * h->d.npthlev);
         const std::string pf = std::string(file) + ".pth";
         FILE* g = std::fopen(pf.c_str(), "rb");
@@ -1456,7 +1793,46 @@ int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny,
         if (!ok) return fail(CMF_EIO, "%s is too short", pf.c_str());
         st.pthflw_pre = pth.data();
     }
-    return cmf_hip_set_state(h, &st);
+    return set_state_impl(h, &st, false);
+}
+
+int cmf_hip_calc_outpre(cmf_handle h) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    const size_t ns = (size_t)h->M.ns;
+    auto run = [&](auto& S) -> int {
+        using R = typename std::remove_reference<decltype(*S.rivout)>::type;
+        // OUTPRE writes D2RIVOUT_PRE only: keep D2RIVOUT (the RIVVEL input of the
+        // next substep) in the separate buffer before the shared array changes
+        if (!h->vel_pending) {
+            HIPCHK(hipMemcpyAsync(h->vel_buf, S.rivout, ns * sizeof(R), hipMemcpyDeviceToDevice, h->stream));
+            h->vel_pending = true;
+        }
+        if (!h->fld_pending) {                          // and D2FLDOUT, for get_state
+            if (!h->fld_buf) {
+                R* fb = nullptr;
+                TRY(dalloc(h, &fb, ns));
+                h->fld_buf = fb;
+            }
+            HIPCHK(hipMemcpyAsync(h->fld_buf, S.fldout, ns * sizeof(R), hipMemcpyDeviceToDevice, h->stream));
+            h->fld_pending = true;
+        }
+        const size_t npl = (size_t)h->M.nsp * (size_t)h->d.npthlev;
+        if (h->o.lpthout && npl > 0 && !h->pth_pending) {        // and D1PTHFLW
+            if (!h->pth_buf) {
+                R* pb = nullptr;
+                TRY(dalloc(h, &pb, npl));
+                h->pth_buf = pb;
+            }
+            HIPCHK(hipMemcpyAsync(h->pth_buf, S.pthflw, npl * sizeof(R), hipMemcpyDeviceToDevice, h->stream));
+            h->pth_pending = true;
+        }
+        launch_outpre<R>(h->M, S, h->o.lfldout, h->o.lpthout, h->o.pmanfld, h->o.pdstmth, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    };
+    return h->o.precision == CMF_PREC_FP64 ? run(h->S64) : run(h->S32);
 }
 
 int cmf_hip_physics_fldstg(cmf_handle h) {
@@ -1495,6 +1871,12 @@ int cmf_hip_ensemble_advance(cmf_handle h, double dt, int32_t nt_cap, int32_t* n
                                            : advance_t<float>(h, dt, nt_cap, nt_out, dt_out, nullptr, 0);
 }
 
+int cmf_hip_get_dt_min(cmf_handle h, double* dt_min) {
+    if (!h || !dt_min) return fail(CMF_EINVAL, "NULL argument");
+    for (int k = 0; k < h->ninst; ++k) dt_min[k] = h->last_dtmin[(size_t)k];
+    return 0;
+}
+
 int cmf_hip_instance_times(cmf_handle h, double* ms) {
     if (!h || !ms) return fail(CMF_EINVAL, "NULL argument");
     if (h->inst_ms.size() != (size_t)h->ninst) return fail(CMF_ESTATE, "no timed advance yet");
@@ -1519,8 +1901,7 @@ int cmf_hip_set_tune(cmf_handle h, const cmf_tune* t) {
     if (v.graph != h->tune.graph || v.fused != h->tune.fused || v.bs_outflow != h->tune.bs_outflow ||
         v.bs_rate != h->tune.bs_rate || v.bs_update != h->tune.bs_update || v.bs_fused != h->tune.bs_fused ||
         v.grid_fused != h->tune.grid_fused || v.xcd != h->tune.xcd) {
-        for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
-        h->graphs.clear();
+        drop_graphs(h);
     }
     h->tune = v;
     return 0;
@@ -1536,8 +1917,7 @@ int cmf_hip_autotune(cmf_handle h, double dt, int32_t nsub, cmf_tune* best, doub
     if (!h) return fail(CMF_EINVAL, "NULL handle");
     if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
     if (!(dt > 0.0) || nsub <= 0) return fail(CMF_EINVAL, "autotune needs DT>0 and nsub>0");
-    for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.exec);
-    h->graphs.clear();
+    drop_graphs(h);
     return h->o.precision == CMF_PREC_FP64 ? autotune_t<double>(h, dt, nsub, best, report, report_cap)
                                            : autotune_t<float>(h, dt, nsub, best, report, report_cap);
 }

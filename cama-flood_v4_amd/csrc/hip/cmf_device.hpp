@@ -133,6 +133,7 @@ struct DevState {
     double* partials;                          // [nt][ntiles][NGLB] (single instance only)
     double* glb_hist;                          // [nt_cap][NGLB]
     unsigned long long* cfl;                   // [ninst]: bits of the CFL bound (positive doubles)
+    double* cfl_part;                          // single instance: CFL bound of each stage block
     const InstStep* inst;                      // [ninst]
     unsigned* bar;                             // [2]: grid barrier of the fused substep kernel
     int32_t* err;                              // [1]: device error flag (barrier time-out)
@@ -142,6 +143,7 @@ struct StepArgs {
     double pgrv, pmanfld, pdstmth, pcadp, pminslp;
     int32_t lfldout, lpthout, ldamout, llevee;
     int32_t it;                                // substep index within the outer step
+    int32_t prow;                              // row of S.partials this substep's budget sums go to
     int32_t budget;                            // write the per-tile water-budget sums
     int32_t xcd;                               // XCD-contiguous tile order
     int32_t lupsinf;                           // add S.upsinf to the river in STONXT

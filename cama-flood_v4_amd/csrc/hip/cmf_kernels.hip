@@ -50,6 +50,50 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* [NV][B
     __syncthreads();
 }
 
+// Compensated block sum (TwoSum, Knuth): every partial sum is carried as hi + lo
+// with lo the exact rounding error of each addition, so the result is the
+// correctly rounded sum of the NV inputs up to O(2^-106) relative terms, whatever
+// the number of cells. Needs IEEE additions without contraction (the files are
+// compiled with -ffp-contract=off). Result in lds[k * NW] as hi + lo, rounded.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+template <int BS, int NV>
+__device__ __forceinline__ void block_sum_comp(double (&v)[NV], double* lds /* [2][NV][BS/64] */) {
+    constexpr int NW = BS / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double lo[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lo[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double oh = __shfl_down(v[k], off, 64), ol = __shfl_down(lo[k], off, 64);
+            double sh, e;
+            two_sum(v[k], oh, sh, e);
+            v[k] = sh;
+            lo[k] = (lo[k] + ol) + e;
+        }
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) { lds[k * NW + wave] = v[k]; lds[(NV + k) * NW + wave] = lo[k]; }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double hi = 0.0, l = 0.0;
+        for (int w = 0; w < NW; ++w) {
+            double sh, e;
+            two_sum(hi, lds[threadIdx.x * NW + w], sh, e);
+            hi = sh;
+            l = (l + lds[(NV + threadIdx.x) * NW + w]) + e;
+        }
+        lds[threadIdx.x * NW] = hi + l;
+    }
+    __syncthreads();
+}
+
 // Tile t of a launch of G blocks: with xcd set and G a multiple of 8, the blocks
 // that the dispatcher places on one XCD (b, b+8, b+16, ...) take one contiguous
 // eighth of the tiles, so neighbour gathers across tile edges mostly hit the
@@ -730,10 +774,10 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
         // budget sums are kept for single-instance handles only (MB = false), so
         // the ensemble kernels carry no budget registers
         if (!MB && A.budget) {
-            block_sum<BS, NGLB>(g, lds);
+            block_sum_comp<BS, NGLB>(g, lds);
             // per-substep slot: reduced once for all substeps at the end of the outer step
             if (threadIdx.x < NGLB)
-                S.partials[((size_t)A.it * ntc + t) * NGLB + threadIdx.x] = lds[threadIdx.x * (BS / 64)];
+                S.partials[((size_t)A.prow * ntc + t) * NGLB + threadIdx.x] = lds[threadIdx.x * (BS / 64)];
             __syncthreads();
         }
     } else {
@@ -763,7 +807,7 @@ __global__ __launch_bounds__(BS) void k_rate(DevMap M, DevState<R> S, StepArgs A
 // the FP64 ensemble update would spill at 6.
 template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS, (MB && sizeof(R) == 4) ? 6 : 5) void k_update(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
-    __shared__ double lds[NGLB * (BS / 64)];
+    __shared__ double lds[2 * NGLB * (BS / 64)];
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
     if (t < nt_all) update_tile<R, BS, MB>(M, S, A, t, ntc, lds);
 }
@@ -813,7 +857,7 @@ __device__ __forceinline__ void tile_range(int b, int G, int ntiles, int xcd, in
 template <class R, int BS, bool MB>
 __global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArgs A, int ntc_o, int nt_o,
                                                 int ntc_r, int ntc_u, int nt_u) {
-    __shared__ double lds[NGLB * (BS / 64)];
+    __shared__ double lds[2 * NGLB * (BS / 64)];
     int t0, dt, te;
     tile_range(blockIdx.x, gridDim.x, nt_o, A.xcd, t0, dt, te);
     for (int t = t0; t < te; t += dt) {
@@ -828,19 +872,26 @@ __global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArg
     for (int t = t0; t < te; t += dt) update_tile<R, BS, MB>(M, S, A, t, ntc_u, lds);
 }
 
-// deterministic second-level reduction of the per-tile budget sums: block b
-// reduces substep b of the outer step
+// deterministic second-level reduction of the per-tile budget sums, compensated:
+// block b reduces substep b of the outer step
 constexpr int RBS = 256;
 __global__ __launch_bounds__(RBS) void k_reduce_budget(const double* partials, int nb, double* out) {
-    __shared__ double lds[NGLB * (RBS / 64)];
+    __shared__ double lds[2 * NGLB * (RBS / 64)];
     const double* src = partials + (size_t)blockIdx.x * nb * NGLB;
-    double v[NGLB];
+    double v[NGLB], lo[NGLB];
 #pragma unroll
-    for (int k = 0; k < NGLB; ++k) v[k] = 0.0;
+    for (int k = 0; k < NGLB; ++k) { v[k] = 0.0; lo[k] = 0.0; }
     for (int b = threadIdx.x; b < nb; b += RBS)
 #pragma unroll
-        for (int k = 0; k < NGLB; ++k) v[k] += src[(size_t)b * NGLB + k];
-    block_sum<RBS, NGLB>(v, lds);
+        for (int k = 0; k < NGLB; ++k) {
+            double sh, e;
+            two_sum(v[k], src[(size_t)b * NGLB + k], sh, e);
+            v[k] = sh;
+            lo[k] = lo[k] + e;
+        }
+#pragma unroll
+    for (int k = 0; k < NGLB; ++k) v[k] = v[k] + lo[k];
+    block_sum_comp<RBS, NGLB>(v, lds);
     if (threadIdx.x < NGLB) out[(size_t)blockIdx.x * NGLB + threadIdx.x] = lds[threadIdx.x * (RBS / 64)];
 }
 
@@ -851,58 +902,184 @@ __global__ void k_cfl_init(unsigned long long* cfl, int ninst, double dt_def) {
     if (k < ninst) cfl[k] = (unsigned long long)__double_as_longlong(dt_def);
 }
 
-template <class R, int BS, bool MB>
+constexpr double CFL_NONE = 1.0e300;            // no bound (masked cell, slot outside the map)
+
+__device__ __forceinline__ unsigned long long cfl_bits(double c) {
+    return (unsigned long long)__double_as_longlong(c);
+}
+
+// CMF_PHYSICS_FLDSTG of one slot; returns the cell's CFL bound of CALC_ADPSTP
+// (physics:178-197) when with_cfl, else CFL_NONE; inst receives the slot's instance
+template <class R, bool MB>
+__device__ __forceinline__ double stage_slot(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s,
+                                             int with_cfl, int& inst) {
+    const Slot x = cell_slot<MB>(M, s);
+    const int i = x.c;
+    inst = x.inst;
+    Stage o;
+    const double levsto = A.llevee ? S.levsto[s] : 0.0;
+    const Geom geo = load_geom(M, i);
+    if (A.llevee) stage_levee(M, i, geo, load_lev(M, i), S.rivsto[s], S.fldsto[s], levsto, o);
+    else stage_def(M, i, geo, S.rivsto[s], S.fldsto[s], o);
+    S.rivsto[s] = o.rivsto;
+    S.fldsto[s] = o.fldsto;
+    const R rivdph = R(o.rivdph);
+    S.rivdph[s] = rivdph;
+    S.flddph[s] = R(o.flddph);
+    S.fldfrc[s] = R(o.fldfrc);
+    S.fldare[s] = R(o.fldare);
+    S.sfcelv[s] = R(rivelv_of(M, i)) + rivdph;
+    double stg = o.rivsto + o.fldsto;
+    if (A.llevee) {
+        S.levsto[s] = o.levsto;
+        S.levdph[s] = R(o.levdph);
+        stg = stg + o.levsto;
+    }
+    S.storge[s] = R(stg);
+    if (with_cfl && M.i2mask[i] == 0) {
+        const double ddph = vmax((double)rivdph, 0.01);
+        const double ddst = (M.next[i] >= 0) ? (double)M.nxtdst[i] : A.pdstmth;
+        return (A.pcadp * ddst) * (1.0 / sqrt(A.pgrv * ddph));
+    }
+    return CFL_NONE;
+}
+
+// Single instance: one block per tile of BS slots; the block minimum of the CFL
+// bound goes to cfl_part[block] (no atomics) and k_cfl_reduce takes the minimum
+// of the blocks.
+template <class R, int BS>
 __global__ __launch_bounds__(BS) void k_stage(DevMap M, DevState<R> S, StepArgs A, int with_cfl) {
+    __shared__ double lds[BS / 64];
     const int s = blockIdx.x * BS + threadIdx.x;
-    double c = 1.0e300;
-    int inst = -1;
-    if (s < M.ns) {
-        const Slot x = cell_slot<MB>(M, s);
-        const int i = x.c;
-        inst = x.inst;
-        Stage o;
-        const double levsto = A.llevee ? S.levsto[s] : 0.0;
-        const Geom geo = load_geom(M, i);
-        if (A.llevee) stage_levee(M, i, geo, load_lev(M, i), S.rivsto[s], S.fldsto[s], levsto, o);
-        else stage_def(M, i, geo, S.rivsto[s], S.fldsto[s], o);
-        S.rivsto[s] = o.rivsto;
-        S.fldsto[s] = o.fldsto;
-        const R rivdph = R(o.rivdph);
-        S.rivdph[s] = rivdph;
-        S.flddph[s] = R(o.flddph);
-        S.fldfrc[s] = R(o.fldfrc);
-        S.fldare[s] = R(o.fldare);
-        S.sfcelv[s] = R(rivelv_of(M, i)) + rivdph;
-        double stg = o.rivsto + o.fldsto;
-        if (A.llevee) {
-            S.levsto[s] = o.levsto;
-            S.levdph[s] = R(o.levdph);
-            stg = stg + o.levsto;
-        }
-        S.storge[s] = R(stg);
-        if (with_cfl && M.i2mask[i] == 0) {                          // physics:178-197
-            const double ddph = vmax((double)rivdph, 0.01);
-            const double ddst = (M.next[i] >= 0) ? (double)M.nxtdst[i] : A.pdstmth;
-            c = (A.pcadp * ddst) * (1.0 / sqrt(A.pgrv * ddph));
+    double c = CFL_NONE;
+    int inst;
+    if (s < M.ns) c = stage_slot<R, false>(M, S, A, s, with_cfl, inst);
+    if (!with_cfl) return;
+    for (int off = 32; off > 0; off >>= 1) c = vmin(c, __shfl_xor(c, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = lds[0];
+        for (int w = 1; w < BS / 64; ++w) b = vmin(b, lds[w]);
+        S.cfl_part[blockIdx.x] = b;
+    }
+}
+
+// DT_MIN = min(DT_DEF, bounds of all blocks) (physics:178, the minimum is exact
+// in any order), written as the bits of the positive double
+__global__ __launch_bounds__(1024) void k_cfl_reduce(const double* part, int nb, double dt_def,
+                                                     unsigned long long* cfl) {
+    __shared__ double lds[16];
+    double c = dt_def;
+    for (int b = threadIdx.x; b < nb; b += 1024) c = vmin(c, part[b]);
+    for (int off = 32; off > 0; off >>= 1) c = vmin(c, __shfl_xor(c, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; ++w) c = vmin(c, lds[w]);
+        cfl[0] = cfl_bits(vmin(c, lds[0]));
+    }
+}
+
+// Several instances: block b runs the ch tiles b*ch .. b*ch+ch-1. A lane always
+// holds the same member (BS is a multiple of NMEM), so it keeps the running
+// minimum of its member over the cells of one domain and flushes it with an
+// atomic minimum when the domain changes. At the end the lanes of one member
+// are combined across the wavefront and across the block's wavefronts, so a
+// block issues about NMEM atomics for ch*BS slots. Positive doubles order like
+// their bit patterns, so the atomic minimum over the bits is the numeric one.
+template <class R, int BS>
+__global__ __launch_bounds__(BS) void k_stage_mb(DevMap M, DevState<R> S, StepArgs A, int with_cfl, int ch) {
+    constexpr int NW = BS / 64;
+    __shared__ double lds_c[NW * 64];
+    __shared__ int lds_d[NW];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = threadIdx.x & (M.nmem - 1);
+    double c = CFL_NONE;
+    int dcur = -1;
+    const long long t0 = (long long)blockIdx.x * ch;
+    for (int k = 0; k < ch; ++k) {
+        const long long s64 = (t0 + k) * BS + threadIdx.x;
+        if (s64 >= M.ns) break;
+        int inst;
+        const double cc = stage_slot<R, true>(M, S, A, (int)s64, with_cfl, inst);
+        if (cc < CFL_NONE) {
+            const int d = inst >> M.lgm;                              // inst = d * NMEM + m
+            if (d != dcur) {
+                if (dcur >= 0 && c < CFL_NONE) atomicMin(&S.cfl[dcur * M.nmem + m], cfl_bits(c));
+                c = CFL_NONE;
+                dcur = d;
+            }
+            c = vmin(c, cc);
         }
     }
     if (!with_cfl) return;
-    // per-instance minimum: the lanes of one member (stride nmem) reduce inside
-    // the wavefront; positive doubles order like their bit patterns, so the
-    // atomic minimum over the bits is the numeric minimum (order-independent).
-    // A wavefront starts at a multiple of 64 slots and nmem divides 64, so lane
-    // k < nmem holds member k.
-    const int lane = threadIdx.x & 63;
-    const int d = (s < M.ns) ? (inst - (inst & (M.nmem - 1))) / M.nmem : 0;
-    const int d0 = __shfl(d, 0, 64);
-    const bool one_dom = __all(d == d0 || s >= M.ns);
-    if (one_dom) {
-        for (int off = M.nmem; off < 64; off <<= 1) c = vmin(c, __shfl_xor(c, off, 64));
-        if (lane < M.nmem && c < 1.0e300)
-            atomicMin(&S.cfl[d0 * M.nmem + lane], (unsigned long long)__double_as_longlong(c));
-    } else if (inst >= 0 && c < 1.0e300) {
-        atomicMin(&S.cfl[inst], (unsigned long long)__double_as_longlong(c));
+    const bool has = c < CFL_NONE;
+    const unsigned long long bal = __ballot(has);
+    if (bal == 0ull) {
+        if (lane == 0) lds_d[wave] = -1;
+    } else {
+        const int d0 = __shfl(dcur, __ffsll((long long)bal) - 1, 64);
+        if (__all(!has || dcur == d0)) {
+            for (int off = M.nmem; off < 64; off <<= 1) c = vmin(c, __shfl_xor(c, off, 64));
+            if (lane < M.nmem) lds_c[wave * 64 + lane] = c;
+            if (lane == 0) lds_d[wave] = d0;
+        } else {
+            if (has) atomicMin(&S.cfl[dcur * M.nmem + m], cfl_bits(c));
+            if (lane == 0) lds_d[wave] = -1;
+        }
     }
+    __syncthreads();
+    if (wave == 0 && lane < M.nmem) {
+        int dacc = -1;
+        double cacc = CFL_NONE;
+        for (int w = 0; w < NW; ++w) {
+            const int d = lds_d[w];
+            if (d < 0) continue;
+            const double v = lds_c[w * 64 + lane];
+            if (d != dacc) {
+                if (dacc >= 0 && cacc < CFL_NONE) atomicMin(&S.cfl[dacc * M.nmem + lane], cfl_bits(cacc));
+                dacc = d;
+                cacc = v;
+            } else {
+                cacc = vmin(cacc, v);
+            }
+        }
+        if (dacc >= 0 && cacc < CFL_NONE) atomicMin(&S.cfl[dacc * M.nmem + lane], cfl_bits(cacc));
+    }
+}
+
+// Total storage of every instance (P2RIVSTO + P2FLDSTO (+ P2LEVSTO)), in two
+// deterministic passes: block (j, k) sums chunk j of the cells of instance k's
+// domain (each thread its strided subset in order, then the fixed block tree)
+// into part[k * nch + j]; k_inst_storage_final adds an instance's nch chunks in
+// order.
+constexpr int ISBS = 256;
+__global__ __launch_bounds__(ISBS) void k_inst_storage_part(const double* rivsto, const double* fldsto,
+                                                            const double* levsto, const int32_t* dom_off, int lgm,
+                                                            int nch, double* part) {
+    __shared__ double lds[ISBS / 64];
+    const int j = blockIdx.x, k = blockIdx.y;
+    const int d = k >> lgm, m = k & ((1 << lgm) - 1);
+    const long long c0 = dom_off[d], nc = (long long)dom_off[d + 1] - c0;
+    const long long lo = c0 + nc * j / nch, hi = c0 + nc * (j + 1) / nch;
+    double v[1] = {0.0};
+    for (long long c = lo + threadIdx.x; c < hi; c += ISBS) {
+        const size_t sl = ((size_t)c << lgm) | (size_t)m;
+        double x = rivsto[sl] + fldsto[sl];
+        if (levsto) x = x + levsto[sl];
+        v[0] += x;
+    }
+    block_sum<ISBS, 1>(v, lds);
+    if (threadIdx.x == 0) part[(size_t)k * nch + j] = lds[0];
+}
+
+__global__ void k_inst_storage_final(const double* part, int nch, int ninst, double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ninst) return;
+    double s = 0.0;
+    for (int j = 0; j < nch; ++j) s += part[(size_t)k * nch + j];
+    out[k] = s;
 }
 
 // instances that idled an odd number of substeps hold RIVDPH in the buffer the
@@ -916,6 +1093,107 @@ __global__ void k_swapfix(DevMap M, DevState<R> S) {
     const R a = S.rivdph[s];
     S.rivdph[s] = S.rivdph_pre[s];
     S.rivdph_pre[s] = a;
+}
+
+// CMF_CALC_OUTPRE (src/cmf_opt_outflw_mod.F90:308-421): previous-step flows of a
+// storage-only restart from the current stage. Per cell slot: SFCELV = RIVELV +
+// RIVDPH, RIVDPH_PRE = RIVDPH, and the diffusive-wave (Manning) discharge
+// AREA * ((1/n) * DFLW^(2/3) * |S|^(1/2)) signed by the slope S, river channel
+// (slope to the downstream surface over NXTDST; mouths: to DWNELV over PDSTMTH)
+// and floodplain (slope clamped to +-0.005), written as RIVOUT_PRE / FLDOUT_PRE.
+// Evaluated in FP64 in the reference's statement order and rounded once to R.
+__device__ __forceinline__ double outpre_q(double area, double rman_inv, double dflw, double dslope) {
+    const double q = area * ((rman_inv * pow(dflw, K_2O3F)) * sqrt(fabs(dslope)));
+    return dslope < 0.0 ? -q : q;
+}
+
+template <class R, bool MB>
+__global__ void k_outpre(DevMap M, DevState<R> S, int lfldout, double pmanfld, double pdstmth) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M.ns) return;
+    const Slot x = cell_slot<MB>(M, s);
+    const int i = x.c;
+    const int nxt = M.next[i];
+    const double rivelv = rivelv_of(M, i);
+    const double rd = (double)S.rivdph[s];
+    const double sfc = rivelv + rd;                                      // :324-327
+    double dslope, dflw, dflw_f;
+    if (nxt >= 0) {                                                      // :331-363
+        const double sfc_j = rivelv_of(M, nxt) + (double)S.rivdph[sidx<MB>(M, nxt, x.m)];
+        const double dsfcmax = vmax(sfc, sfc_j);
+        dslope = (sfc - sfc_j) * (1.0 / (double)M.nxtdst[i]);
+        dflw = dsfcmax - rivelv;
+        dflw_f = vmax(dsfcmax - (double)M.elevtn[i], 0.0);
+    } else {                                                             // :368-393
+        dslope = (sfc - M.dwnelv[i]) * (1.0 / pdstmth);
+        dflw = rd;
+        dflw_f = vmax(sfc - (double)M.elevtn[i], 0.0);
+    }
+    const double dslope_f = vmax(-0.005, vmin(0.005, dslope));
+    const double rivwth = (double)M.rivwth[i];
+    const double darea = rivwth * dflw;
+    const double qr = darea > K_1EM5F ? outpre_q(darea, 1.0 / (double)M.rivman[i], dflw, dslope) : 0.0;
+    double dare_f = S.fldsto[s] * (1.0 / (double)M.rivlen[i]);
+    dare_f = vmax(dare_f - (double)S.flddph[s] * rivwth, 0.0);
+    double qf = dare_f > K_1EM5F ? outpre_q(dare_f, 1.0 / pmanfld, dflw_f, dslope_f) : 0.0;
+    if (!lfldout) qf = 0.0;                                              // :416-418
+    S.sfcelv[s] = R(sfc);
+    S.rivdph_pre[s] = S.rivdph[s];
+    S.rivout[s] = R(qr);                                                 // D2RIVOUT_PRE
+    S.fldout[s] = R(qf);                                                 // D2FLDOUT_PRE
+}
+
+// the bifurcation levels of CMF_CALC_OUTPRE (:396-413), per path slot; a path
+// with an endpoint outside the domain gets 0
+template <class R, bool MB>
+__global__ void k_outpre_path(DevMap M, DevState<R> S) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= M.nsp) return;
+    const Slot x = path_slot<MB>(M, q);
+    const int p = x.c;
+    const int ip = M.pth_upst[p], jp = M.pth_down[p];
+    const bool ok = ip >= 0 && jp >= 0;
+    double si = 0.0, sj = 0.0, dslope = 0.0;
+    if (ok) {
+        si = rivelv_of(M, ip) + (double)S.rivdph[sidx<MB>(M, ip, x.m)];
+        sj = rivelv_of(M, jp) + (double)S.rivdph[sidx<MB>(M, jp, x.m)];
+        dslope = (si - sj) * (1.0 / M.pth_dst[p]);
+    }
+    const int np = M.npth;
+    for (int l = 0; l < M.npthlev; ++l) {
+        double qq = 0.0;
+        if (ok) {
+            double dflw = vmax(si, sj) - M.pth_elv[(size_t)l * np + p];
+            dflw = vmax(dflw, 0.0);
+            if (dflw > K_1EM5F) {
+                qq = (M.pth_wth[(size_t)l * np + p] * dflw) * (((1.0 / M.pth_man[l]) * pow(dflw, K_2O3F)) * sqrt(fabs(dslope)));
+                if (dslope < 0.0) qq = -qq;
+            }
+        }
+        S.pthflw[(size_t)l * M.nsp + q] = R(qq);                         // D1PTHFLW_PRE
+    }
+}
+
+// Host <-> device transfer of per-slot cell arrays: the host array (member-major
+// planes of NSEQALL values in NSEQ order, element type H) is copied to the device
+// as it is and permuted into the member-minor slots in device cell order here
+// (perm[p] = NSEQ index of device cell p; NULL = identity), converted to R with
+// the same rounding as a host-side cast. shared: one plane for every member.
+template <class R, class H>
+__global__ void k_cells_from_host(R* dst, const H* src, const int32_t* perm, int n, int lgm, int ns, int shared) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int p = s >> lgm, m = s & ((1 << lgm) - 1);
+    const size_t hi = (shared ? (size_t)0 : (size_t)m * (size_t)n) + (size_t)(perm ? perm[p] : p);
+    dst[s] = (R)src[hi];
+}
+
+template <class R, class H>
+__global__ void k_cells_to_host(H* dst, const R* src, const int32_t* perm, int n, int lgm, int ns) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int p = s >> lgm, m = s & ((1 << lgm) - 1);
+    dst[(size_t)m * (size_t)n + (size_t)(perm ? perm[p] : p)] = (H)src[s];
 }
 
 // runoff[s] = base[c] * factor[inst] (multiplicative member perturbation)
@@ -1014,13 +1292,21 @@ __global__ void k_boundary(int n, const float* elevtn, const double* meansl, con
 static inline int cdiv(long long a, int b) { return (int)((a + b - 1) / b); }
 static inline int pad8(int g, int xcd) { return xcd ? (g + 7) & ~7 : g; }
 static inline bool multi(const DevMap& M) { return M.nmem > 1 || M.cell_dom != nullptr; }
+constexpr int STAGE_MB_BLOCKS = 4096;
 
 template <class R>
 void launch_stage(const DevMap& M, const DevState<R>& S, const StepArgs& A, int with_cfl, int ninst, double dt_def,
                   hipStream_t st) {
-    if (with_cfl) hipLaunchKernelGGL(k_cfl_init, dim3(cdiv(ninst, 64)), dim3(64), 0, st, S.cfl, ninst, dt_def);
-    if (multi(M)) hipLaunchKernelGGL((k_stage<R, 256, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, A, with_cfl);
-    else hipLaunchKernelGGL((k_stage<R, 256, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, A, with_cfl);
+    const int ntiles = cdiv(M.ns, 256);
+    if (multi(M)) {
+        if (with_cfl) hipLaunchKernelGGL(k_cfl_init, dim3(cdiv(ninst, 64)), dim3(64), 0, st, S.cfl, ninst, dt_def);
+        // about 4096 blocks, each over ch consecutive tiles
+        const int ch = ntiles > STAGE_MB_BLOCKS ? cdiv(ntiles, STAGE_MB_BLOCKS) : 1;
+        hipLaunchKernelGGL((k_stage_mb<R, 256>), dim3(cdiv(ntiles, ch)), dim3(256), 0, st, M, S, A, with_cfl, ch);
+    } else {
+        hipLaunchKernelGGL((k_stage<R, 256>), dim3(ntiles), dim3(256), 0, st, M, S, A, with_cfl);
+        if (with_cfl) hipLaunchKernelGGL(k_cfl_reduce, dim3(1), dim3(1024), 0, st, S.cfl_part, ntiles, dt_def, S.cfl);
+    }
 }
 
 int tiles_cells(const DevMap& M, int bs) { return cdiv(M.ns, bs); }
@@ -1095,6 +1381,36 @@ void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st) {
     hipLaunchKernelGGL((k_swapfix<R, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S);
 }
 
+template <class R, class H>
+void launch_cells_from_host(R* dst, const H* src, const int32_t* perm, int n, int lgm, int ns, int shared,
+                            hipStream_t st) {
+    hipLaunchKernelGGL((k_cells_from_host<R, H>), dim3(cdiv(ns, 256)), dim3(256), 0, st, dst, src, perm, n, lgm, ns,
+                       shared);
+}
+template <class R, class H>
+void launch_cells_to_host(H* dst, const R* src, const int32_t* perm, int n, int lgm, int ns, hipStream_t st) {
+    hipLaunchKernelGGL((k_cells_to_host<R, H>), dim3(cdiv(ns, 256)), dim3(256), 0, st, dst, src, perm, n, lgm, ns);
+}
+#define CMF_INSTANTIATE_XFER(R, H)                                                                        \
+    template void launch_cells_from_host<R, H>(R*, const H*, const int32_t*, int, int, int, int, hipStream_t); \
+    template void launch_cells_to_host<R, H>(H*, const R*, const int32_t*, int, int, int, hipStream_t);
+CMF_INSTANTIATE_XFER(double, double)
+CMF_INSTANTIATE_XFER(double, float)
+CMF_INSTANTIATE_XFER(float, double)
+CMF_INSTANTIATE_XFER(float, float)
+
+template <class R>
+void launch_outpre(const DevMap& M, const DevState<R>& S, int lfldout, int lpthout, double pmanfld, double pdstmth,
+                   hipStream_t st) {
+    if (multi(M)) {
+        hipLaunchKernelGGL((k_outpre<R, true>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, lfldout, pmanfld, pdstmth);
+        if (lpthout && M.nsp > 0) hipLaunchKernelGGL((k_outpre_path<R, true>), dim3(cdiv(M.nsp, 256)), dim3(256), 0, st, M, S);
+    } else {
+        hipLaunchKernelGGL((k_outpre<R, false>), dim3(cdiv(M.ns, 256)), dim3(256), 0, st, M, S, lfldout, pmanfld, pdstmth);
+        if (lpthout && M.nsp > 0) hipLaunchKernelGGL((k_outpre_path<R, false>), dim3(cdiv(M.nsp, 256)), dim3(256), 0, st, M, S);
+    }
+}
+
 template <class R>
 void launch_scale_runoff(const DevMap& M, const DevState<R>& S, const double* base, const double* factor,
                          hipStream_t st) {
@@ -1131,6 +1447,13 @@ void launch_boundary(int n, const float* elevtn, const double* meansl, const dou
     hipLaunchKernelGGL(k_boundary, dim3(cdiv(n, 256)), dim3(256), 0, st, n, elevtn, meansl, sealev, dwnelv);
 }
 
+void launch_inst_storage(const double* rivsto, const double* fldsto, const double* levsto, const int32_t* dom_off,
+                         int lgm, int ninst, int nch, double* part, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_inst_storage_part, dim3(nch, ninst), dim3(ISBS), 0, st, rivsto, fldsto, levsto, dom_off, lgm,
+                       nch, part);
+    hipLaunchKernelGGL(k_inst_storage_final, dim3(cdiv(ninst, 256)), dim3(256), 0, st, part, nch, ninst, out);
+}
+
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_budget, dim3(nrun), dim3(RBS), 0, st, partials, ntiles, glb_hist);
 }
@@ -1143,6 +1466,7 @@ void launch_budget(const double* partials, int ntiles, int nrun, double* glb_his
     template hipError_t launch_substep<R>(const DevMap&, const DevState<R>&, const StepArgs&, int, int,        \
                                           hipStream_t);                                                        \
     template void launch_swapfix<R>(const DevMap&, const DevState<R>&, hipStream_t);                           \
+    template void launch_outpre<R>(const DevMap&, const DevState<R>&, int, int, double, double, hipStream_t);  \
     template void launch_scale_runoff<R>(const DevMap&, const DevState<R>&, const double*, const double*,     \
                                          hipStream_t);                                                          \
     template void launch_roff_interp<R>(const DevMap&, const DevState<R>&, const RoffMap&, const double*,      \

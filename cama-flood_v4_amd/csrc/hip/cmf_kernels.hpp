@@ -24,6 +24,17 @@ hipError_t launch_substep(const DevMap& M, const DevState<R>& S, const StepArgs&
                           hipStream_t st);
 template <class R>
 const void* substep_kernel(const DevMap& M, int bs);
+// per-slot cell arrays between the host layout (member planes, NSEQ order, type H)
+// in a device staging buffer and the device slots (type R); perm NULL = identity
+template <class R, class H>
+void launch_cells_from_host(R* dst, const H* src, const int32_t* perm, int n, int lgm, int ns, int shared,
+                            hipStream_t st);
+template <class R, class H>
+void launch_cells_to_host(H* dst, const R* src, const int32_t* perm, int n, int lgm, int ns, hipStream_t st);
+// CMF_CALC_OUTPRE: RIVOUT_PRE, FLDOUT_PRE, RIVDPH_PRE, SFCELV (and D1PTHFLW_PRE with paths)
+template <class R>
+void launch_outpre(const DevMap& M, const DevState<R>& S, int lfldout, int lpthout, double pmanfld, double pdstmth,
+                   hipStream_t st);
 template <class R>
 void launch_swapfix(const DevMap& M, const DevState<R>& S, hipStream_t st);
 template <class R>
@@ -39,6 +50,10 @@ void launch_out_getave(const OutState& O, int ns, int npth_slots, double nadd, i
 void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st);
 void launch_boundary(int n, const float* elevtn, const double* meansl, const double* sealev, double* dwnelv,
                      hipStream_t st);
+// total storage of every instance into out[ninst] (part: ninst * nch doubles;
+// dom_off: first cell of each domain in device order, and the end)
+void launch_inst_storage(const double* rivsto, const double* fldsto, const double* levsto, const int32_t* dom_off,
+                         int lgm, int ninst, int nch, double* part, double* out, hipStream_t st);
 // reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]
 void launch_budget(const double* partials, int ntiles, int nrun, double* glb_hist, hipStream_t st);
 

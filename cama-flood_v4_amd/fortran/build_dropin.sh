@@ -8,6 +8,8 @@
 #                       tests compare with the reference's own)
 #   cmf_dropin_devdiag  this package's device-side cmf_calc_diag_mod.F90; main
 #                       program: this package's cmf_dropin_main.F90
+#   cmf_dropin32        as cmf_dropin, with the reference in its default
+#                       single-precision mode (-DSinglePrec_CMF, JPRB = 4)
 set -euo pipefail
 REF=${CMF_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -25,11 +27,12 @@ cmf_ctrl_boundary_mod cmf_ctrl_upsinf_mod cmf_ctrl_output_mod cmf_ctrl_restart_m
 cmf_ctrl_time_mod cmf_ctrl_maps_mod cmf_ctrl_vars_mod cmf_ctrl_nmlist_mod
 cmf_drv_control_mod cmf_drv_advance_mod"
 
-# build_variant NAME DEVICE_DIAG MAIN: DEVICE_DIAG=1 links this package's
-# device-side CMF_CALC_DIAG_MOD (compiled after the physics module it uses) in
-# place of the reference's; MAIN is the main program's source
+# build_variant NAME DEVICE_DIAG MAIN [EXTRA_FFLAGS]: DEVICE_DIAG=1 links this
+# package's device-side CMF_CALC_DIAG_MOD (compiled after the physics module it
+# uses) in place of the reference's; MAIN is the main program's source
 build_variant() {
-  local name=$1 devdiag=$2 main=$3 obj=$OUT/obj_$1 objs="" f o
+  local name=$1 devdiag=$2 main=$3 extra=${4:-} obj=$OUT/obj_$1 objs="" f o
+  local FFLAGS="$FFLAGS $extra"
   mkdir -p "$obj"
   for s in $SRCS; do
     case $s in
@@ -49,3 +52,4 @@ build_variant() {
 mkdir -p "$OUT"
 build_variant cmf_dropin 0 "$ROOT/oracle/ref_driver.F90"
 build_variant cmf_dropin_devdiag 1 "$HERE/cmf_dropin_main.F90"
+build_variant cmf_dropin32 0 "$ROOT/oracle/ref_driver.F90" -DSinglePrec_CMF

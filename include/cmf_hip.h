@@ -51,12 +51,23 @@ typedef struct cmf_dims {
 
 /* Namelist switches and parameters read by the physics
  * (src/cmf_ctrl_nmlist_mod.F90:77-115, :227-232). LSPAMAT is implied (gather
- * inflow); LKINE, LSLPMIX, LSTG_ES, LGDWDLY, LROSPLIT, LWEVAP, LUPSINF, LOUTINS
- * must be false. */
+ * inflow). LKINE and LUPSINF are switched on by their own entry points
+ * (cmf_hip_set_kinematic, cmf_hip_set_upsinf); LSTG_ES, LGDWDLY, LROSPLIT,
+ * LWEVAP and LOUTINS must be false. */
 typedef struct cmf_options {
     int32_t ladpstp, lfplain, lfldout, lpthout, ldamout, llevee;
     int32_t precision;                 /* CMF_PREC_* */
     double pgrv, pmanfld, pdstmth, pcadp, pminslp;
+    /* bytes of the host's JPRB (src/parkind1.F90:34-38): 8 (or 0) for the FP64
+     * build, 4 for the reference's default -DSinglePrec_CMF build. With 4 every
+     * JPRB array passed to or filled by the handle is float (c_float): the map
+     * tables of cmf_map, the D2 and D1 arrays of cmf_state, cmf_output, runoff,
+     * D2DWNELV, D2UPSINF, D2MEANSL/D2SEALEV, INPA and the runoff grid, although
+     * the pointer members and parameters are declared double* (cast float* to
+     * them); the P2* storages, the budget sums, DT, DT_MIN and the ensemble
+     * factors stay double. The device arithmetic is chosen by `precision`; the
+     * float map tables are checked against the FP64 recurrence to 1e-5. */
+    int32_t jprb;
 } cmf_options;
 
 /* Static map (YOS_CMF_MAP, levee and dam module tables). Required: i1next,
@@ -136,6 +147,10 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* map);
 /* Host -> device copy of the non-NULL arrays of st (after CMF_PROG_INIT,
  * CMF_RESTART_INIT, CMF_DAMOUT_INIT, or any host-side change of state). */
 int cmf_hip_set_state(cmf_handle h, const cmf_state* st);
+/* Ensembles: as cmf_hip_set_state, but each array holds one plane (NSEQALL cell
+ * values, NPTHLEV*NPTHOUT or NPTHOUT path values) that every member of every
+ * domain receives (a common initial state). No reference counterpart. */
+int cmf_hip_set_state_shared(cmf_handle h, const cmf_state* st);
 /* Device -> host copy into the non-NULL arrays of st. */
 int cmf_hip_get_state(cmf_handle h, cmf_state* st);
 
@@ -225,13 +240,24 @@ int cmf_hip_output_write(cmf_handle h, const char* var, const char* file, int32_
  * unless lstoonly, D2RIVOUT_PRE, D2FLDOUT_PRE, D2RIVDPH_PRE, D2FLDSTO_PRE, then
  * P2DAMSTO (LDAMOUT) and P2LEVSTO (LLEVEE); with LPTHOUT, D1PTHFLW_PRE
  * (NPTHOUT x NPTHLEV) in the file `file`.pth. The file name is the caller's
- * (the reference builds CRESTDIR//CVNREST//YYYYMMDDHH//CSUFBIN). The reader
- * loads the same records into the device state; the caller then runs
- * cmf_hip_physics_fldstg as the reference's restart initialisation does. */
+ * (the reference builds CRESTDIR//CVNREST//YYYYMMDDHH//CSUFBIN). The reader is
+ * CMF_RESTART_INIT (restart:95-133): the state arrays the file does not carry are
+ * zero (D2RIVOUT, D2FLDOUT; with lstoonly also the four t-1 records and
+ * D1PTHFLW_PRE, the .pth file not read), the records are loaded into the device
+ * state, and with lstoonly D2FLDSTO_PRE = P2FLDSTO. For a storage-only restart
+ * the caller then runs cmf_hip_physics_fldstg and cmf_hip_calc_outpre, as
+ * CMF_DRV_INIT does (src/cmf_drv_control_mod.F90:273-278). */
 int cmf_hip_restart_write(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
                           const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly, double dmis, double rmis);
 int cmf_hip_restart_read(cmf_handle h, const char* file, int32_t nx, int32_t ny, const int32_t* seqx,
                          const int32_t* seqy, int32_t lrestdbl, int32_t lstoonly);
+
+/* CMF_CALC_OUTPRE (src/cmf_opt_outflw_mod.F90:308-421): the previous-step flows
+ * D2RIVOUT_PRE, D2FLDOUT_PRE (0 without LFLDOUT) and D1PTHFLW_PRE from the
+ * current stage (diffusive-wave discharge of the water surface), with
+ * D2SFCELV and D2RIVDPH_PRE = D2RIVDPH; D2RIVOUT is left as it is. Call after
+ * cmf_hip_physics_fldstg. */
+int cmf_hip_calc_outpre(cmf_handle h);
 
 /* CMF_PHYSICS_FLDSTG: storage -> stage on the device state. */
 int cmf_hip_physics_fldstg(cmf_handle h);
@@ -252,6 +278,18 @@ int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap,
  * receive NT and DT of instance k; nt_cap > 0 caps the substeps (timing). No
  * budget sums are kept for more than one instance. No reference counterpart. */
 int cmf_hip_ensemble_advance(cmf_handle h, double dt, int32_t nt_cap, int32_t* nt_out, double* dt_out);
+
+/* DT_MIN of CALC_ADPSTP (src/cmf_ctrl_physics_mod.F90:178-201) of the last
+ * advance, one value per instance (dt_min[ndom*members]; 0 without LADPSTP): the
+ * third value of the reference's "ADPSTP: NT=" log line (physics:208). */
+int cmf_hip_get_dt_min(cmf_handle h, double* dt_min);
+
+/* Total storage of every instance: out[ndom*members] = the sum over the
+ * instance's cells of P2RIVSTO + P2FLDSTO (+ P2LEVSTO with levees), the
+ * P0GLBRIVSTO + P0GLBFLDSTO (+ P0GLBLEVSTO) of the reference's budget
+ * (src/cmf_calc_fldstg_mod.F90:99-108) per instance. Deterministic (fixed
+ * summation order for a given handle). No reference counterpart. */
+int cmf_hip_instance_storage(cmf_handle h, double* out);
 
 /* With kernel timing enabled: milliseconds from the start of the last advance to
  * the end of the last substep of each instance (ms[ndom*members]). */

@@ -11,6 +11,8 @@
 #   omp64    : FP64, -O3 -fopenmp -DNoAtom_CMF. Timing build (OpenMP CPU baseline).
 #   omp32    : reference default single-precision mode (-DSinglePrec_CMF),
 #              -O3 -fopenmp -DNoAtom_CMF. CPU counterpart of the mixed path.
+#   serial32 : single-precision mode, -O2 -ffp-contract=off, no OpenMP: the
+#              fixtures of the single-precision drop-in (JPRB = 4 host).
 #
 # OpenMP builds need a build copy of two files: flang rejects the loop index INUM
 # listed in THREADPRIVATE (cmf_calc_outflw_mod.F90:394, cmf_ctrl_tracer_mod.F90:542).
@@ -41,6 +43,7 @@ for V in $VARIANTS; do
     serial64) FFLAGS="-O2 -cpp -ffp-contract=off"; OMP="" ;;
     omp64)    FFLAGS="-O3 -cpp -fopenmp -DNoAtom_CMF"; OMP=1 ;;
     omp32)    FFLAGS="-O3 -cpp -fopenmp -DNoAtom_CMF -DSinglePrec_CMF"; OMP=1 ;;
+    serial32) FFLAGS="-O2 -cpp -ffp-contract=off -DSinglePrec_CMF"; OMP="" ;;
     *) echo "unknown variant $V" >&2; exit 2 ;;
   esac
   D=$OUT/$V

@@ -506,6 +506,36 @@ void cmfo_levee_opt_pthout(cmfo_model* m, double dt)
 /* ------------------------------------------------------------------------
  * CMF_DAMOUT_CALC  src/cmf_ctrl_damout_mod.F90:316-498 (Yamazaki & Funato rule;
  * the NoAtom_CMF serial order of the dam-inflow scatter at :456-464) */
+/* CMF_DAMOUT_CALC, Hanazaki-2022 branch (LDAMH22)  src/cmf_ctrl_damout_mod.F90:354-378.
+ * The ratio r_volupa is the dam's own FldVol*1.E-6/upreal (damout:204 writes every
+ * dam's ratio to R_VolUpa(NDAM); here each dam keeps its own). */
+double cmfo_dam_rule_h22(double vol, double inflow, double norvol, double convol, double emevol, double qn,
+                         double qf, double r_volupa)
+{
+    const double F0P2 = (double)0.2f;                                      /* default-kind 0.2 */
+    double out;
+    if (vol <= norvol) {                                                    /* :356-357 */
+        out = qn * (vol / convol);
+    } else if (norvol < vol && vol <= convol) {                             /* :359-365 */
+        if (qf <= inflow) {
+            out = qn * 0.5 + ((vol - norvol) / (convol - norvol)) * (qf - qn);
+        } else {
+            const double x = (vol - norvol) / (emevol - norvol);
+            out = qn * 0.5 + (x * x) * (qf - qn);
+        }
+    } else if (convol < vol && vol < emevol) {                              /* :367-373 */
+        if (qf <= inflow) {
+            out = qf + ((DMAX(1.0 - r_volupa / F0P2, 0.0) * (vol - convol)) / (emevol - convol)) * (inflow - qf);
+        } else {
+            const double x = (vol - norvol) / (emevol - norvol);
+            out = qn * 0.5 + (x * x) * (qf - qn);
+        }
+    } else {                                                                /* :375-376 */
+        out = DMAX(inflow, qf);
+    }
+    return out;
+}
+
 void cmfo_damout_calc(cmfo_model* m, double dt)
 {
     const int n = m->n;
@@ -548,7 +578,9 @@ void cmfo_damout_calc(cmfo_model* m, double dt)
         const double cv = m->dam_convol[d], av = m->dam_adjvol[d], ev = m->dam_emevol[d];
         const double qn = m->dam_qn[d], qf = m->dam_qf[d], qa = m->dam_qa[d];
         double out = 0.0;
-        if (vol <= cv) {
+        if (m->ldamh22) {
+            out = cmfo_dam_rule_h22(vol, inf, m->dam_norvol[d], cv, ev, qn, qf, m->dam_r_volupa[d]);
+        } else if (vol <= cv) {
             out = qn * sqrt(vol / cv);
         } else if (vol > cv && vol <= av) {
             const double x = (vol - cv) / (av - cv);
@@ -788,6 +820,72 @@ void cmfo_calc_outflw_kine(cmfo_model* m, double dt)
 }
 
 /* ------------------------------------------------------------------------
+ * CMF_CALC_OUTPRE  src/cmf_opt_outflw_mod.F90:308-421: previous-step flows for a
+ * storage-only restart (called after CMF_PHYSICS_FLDSTG, drv_control:273-278):
+ * the diffusive-wave (Manning) discharge of the current water surface, signed by
+ * the surface slope, for rivers, mouths (slope to DWNELV over PDSTMTH) and
+ * bifurcation levels. Writes D2SFCELV, D2RIVDPH_PRE, D2RIVOUT_PRE, D2FLDOUT_PRE
+ * and D1PTHFLW_PRE. A path with an endpoint outside the domain (PTH_UPST or
+ * PTH_DOWN = 0) gets 0: the reference indexes D2SFCELV(0) there. */
+static double outpre_q(double area, double rman_inv, double dflw, double dslope)
+{
+    /* AREA * ( MAN**(-1.) * DFLW**(2./3.) * abs(DSLOPE)**(0.5) ), signed by DSLOPE */
+    const double q = area * ((rman_inv * pow(dflw, F2O3)) * sqrt(fabs(dslope)));
+    return dslope < 0.0 ? -q : q;
+}
+
+void cmfo_calc_outpre(cmfo_model* m)
+{
+    const int n = m->n, nr = m->nseqriv;
+    for (int i = 0; i < n; ++i) {                                           /* :324-327 */
+        m->sfcelv[i] = m->rivelv[i] + m->rivdph[i];
+        m->rivdph_pre[i] = m->rivdph[i];
+    }
+    for (int i = 0; i < n; ++i) {
+        double dslope, dflw, dflw_f;
+        if (i < nr) {                                                       /* :331-363 */
+            const int j = m->next[i];
+            const double dsfcmax = DMAX(m->sfcelv[i], m->sfcelv[j]);
+            dslope = (m->sfcelv[i] - m->sfcelv[j]) * (1.0 / m->nxtdst[i]);
+            dflw = dsfcmax - m->rivelv[i];
+            dflw_f = DMAX(dsfcmax - m->elevtn[i], 0.0);
+        } else {                                                            /* :368-393 */
+            dslope = (m->sfcelv[i] - m->dwnelv[i]) * (1.0 / m->pdstmth);
+            dflw = m->rivdph[i];
+            dflw_f = DMAX(m->sfcelv[i] - m->elevtn[i], 0.0);
+        }
+        const double dslope_f = DMAX(-0.005, DMIN(0.005, dslope));
+        const double darea = m->rivwth[i] * dflw;
+        m->rivout_pre[i] = darea > F1EM5 ? outpre_q(darea, 1.0 / m->rivman[i], dflw, dslope) : 0.0;
+        double dare_f = m->fldsto[i] * (1.0 / m->rivlen[i]);
+        dare_f = DMAX(dare_f - m->flddph[i] * m->rivwth[i], 0.0);
+        m->fldout_pre[i] = dare_f > F1EM5 ? outpre_q(dare_f, 1.0 / m->pmanfld, dflw_f, dslope_f) : 0.0;
+        if (!m->lfldout) m->fldout_pre[i] = 0.0;                            /* :416-418 */
+    }
+    if (m->lpthout) {                                                       /* :396-413 */
+        const int np = m->npth;
+        for (int p = 0; p < np; ++p) {
+            const int ip = m->pth_upst[p], jp = m->pth_down[p];
+            const int ok = ip >= 0 && jp >= 0;
+            const double dslope = ok ? (m->sfcelv[ip] - m->sfcelv[jp]) * (1.0 / m->pth_dst[p]) : 0.0;
+            for (int l = 0; l < m->npthlev; ++l) {
+                double q = 0.0;
+                if (ok) {
+                    double dflw = DMAX(m->sfcelv[ip], m->sfcelv[jp]) - m->pth_elv[(size_t)l * np + p];
+                    dflw = DMAX(dflw, 0.0);
+                    if (dflw > F1EM5) {
+                        q = (m->pth_wth[(size_t)l * np + p] * dflw) *
+                            (((1.0 / m->pth_man[l]) * pow(dflw, F2O3)) * sqrt(fabs(dslope)));
+                        if (dslope < 0.0) q = -q;
+                    }
+                }
+                m->pthflw_pre[(size_t)l * np + p] = q;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
  * CMF_PHYSICS_ADVANCE  src/cmf_ctrl_physics_mod.F90:21-155 */
 int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
 {
@@ -813,6 +911,8 @@ int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
         cmfo_calc_stonxt(m, *dt);
         cmfo_physics_fldstg(m);
         cmfo_diag_avemax_adpstp(m, *dt);
+        if (m->glb_hist && it < m->glb_cap)                                 /* :216-248 */
+            for (int k = 0; k < 11; ++k) m->glb_hist[(size_t)it * 11 + k] = m->glb[k];
     }
     *dt = dt_def;                                                           /* :145 */
     cmfo_diag_getave_adpstp(m);                                             /* :147 */

@@ -84,6 +84,15 @@ typedef struct cmfo_model {
     /* LKINE: kinematic-wave outflow (CMF_CALC_OUTFLW_KINE) in place of the local
      * inertial one; 0 = off */
     int32_t lkine;
+    /* optional record of the budget sums of every substep (the values
+     * CALC_WATBAL logs per substep, physics:216-248): glb of substep it goes to
+     * glb_hist[it*11 .. it*11+10] for it < glb_cap; NULL = off */
+    double* glb_hist;
+    int32_t glb_cap;
+    /* LDAMH22: the Hanazaki-2022 dam rule with NorVol and the flood-control volume
+     * over the upstream area per dam (damout:202-205, 354-378); 0 = Yamazaki-Funato */
+    int32_t ldamh22;
+    const double *dam_norvol, *dam_r_volupa;
 } cmfo_model;
 
 /* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */
@@ -99,9 +108,15 @@ void cmfo_levee_fldstg(cmfo_model* m);
 int32_t cmfo_calc_adpstp(cmfo_model* m, double dt_def, double* dt);
 void cmfo_calc_outflw(cmfo_model* m, double dt);
 void cmfo_calc_outflw_kine(cmfo_model* m, double dt);
+/* CMF_CALC_OUTPRE (src/cmf_opt_outflw_mod.F90:308-421): t-1 flows of a
+ * storage-only restart from the current stage */
+void cmfo_calc_outpre(cmfo_model* m);
 void cmfo_calc_pthout(cmfo_model* m, double dt);
 void cmfo_levee_opt_pthout(cmfo_model* m, double dt);
 void cmfo_damout_calc(cmfo_model* m, double dt);
+/* the Hanazaki-2022 operation rule of one dam before the limiter (damout:354-378) */
+double cmfo_dam_rule_h22(double vol, double inflow, double norvol, double convol, double emevol, double qn,
+                         double qf, double r_volupa);
 void cmfo_calc_inflow_lspamat(cmfo_model* m, double dt);
 void cmfo_damout_watbal(cmfo_model* m, double dt);
 void cmfo_calc_vars_pre(cmfo_model* m);

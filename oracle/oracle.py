@@ -35,7 +35,9 @@ _WORK = "w_sfcelv_pre w_dwnelv_pre w_flddph_pre w_stoout w_rate w_rivinf w_fldin
 
 
 class _Model(C.Structure):
-    _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK] + [("upsinf", _P), ("lkine", C.c_int32)]
+    _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK] + [
+        ("upsinf", _P), ("lkine", C.c_int32), ("glb_hist", _P), ("glb_cap", C.c_int32),
+        ("ldamh22", C.c_int32), ("dam_norvol", _P), ("dam_r_volupa", _P)]
 
 
 MAP_KEYS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax "
@@ -56,9 +58,11 @@ def _lib():
     lib = C.CDLL(LIB)
     lib.cmfo_physics_advance.restype = C.c_int32
     lib.cmfo_physics_advance.argtypes = [C.POINTER(_Model), C.POINTER(C.c_double), C.c_int32]
+    lib.cmfo_dam_rule_h22.restype = C.c_double
+    lib.cmfo_dam_rule_h22.argtypes = [C.c_double] * 8
     lib.cmfo_calc_adpstp.restype = C.c_int32
     lib.cmfo_calc_adpstp.argtypes = [C.POINTER(_Model), C.c_double, C.POINTER(C.c_double)]
-    for f in ("cmfo_physics_fldstg", "cmfo_calc_fldstg_def", "cmfo_levee_fldstg",
+    for f in ("cmfo_physics_fldstg", "cmfo_calc_fldstg_def", "cmfo_levee_fldstg", "cmfo_calc_outpre",
               "cmfo_calc_vars_pre", "cmfo_diag_reset_adpstp", "cmfo_diag_getave_adpstp"):
         getattr(lib, f).restype = None
         getattr(lib, f).argtypes = [C.POINTER(_Model)]
@@ -109,6 +113,11 @@ class OracleOutput:
         return self.s.nadd
 
 
+def dam_rule_h22(vol, inflow, norvol, convol, emevol, qn, qf, r_volupa):
+    """The oracle's Hanazaki-2022 dam rule (src/cmf_ctrl_damout_mod.F90:354-378)."""
+    return float(_lib().cmfo_dam_rule_h22(vol, inflow, norvol, convol, emevol, qn, qf, r_volupa))
+
+
 def roff_interp(inpx, inpy, inpa, nxin, nyin, grid, drofunit, rmis):
     """ROFF_INTERP of the C oracle (src/cmf_ctrl_forcing_mod.F90:757-790):
     inpx/inpy/inpa [inpn, n], grid [nyin, nxin] -> runoff [n]."""
@@ -152,6 +161,10 @@ class OracleModel:
         if o.ldamout:
             for k in DAM_KEYS:
                 self._bind(k, getattr(m, k))
+            if getattr(o, "ldamh22", False):
+                s.ldamh22 = 1
+                self._bind("dam_norvol", m.dam_norvol)
+                self._bind("dam_r_volupa", m.dam_r_volupa)
         self.st = {}
         for k in CELL_STATE:
             self.st[k] = np.ascontiguousarray(state.get(k, np.zeros(n)), dtype=np.float64).copy()
@@ -186,9 +199,25 @@ class OracleModel:
     def physics_fldstg(self):
         self.lib.cmfo_physics_fldstg(C.byref(self.s))
 
-    def physics_advance(self, dt: float, nt_cap: int = 0) -> int:
+    def calc_outpre(self):
+        """CMF_CALC_OUTPRE (src/cmf_opt_outflw_mod.F90:308-421)."""
+        self.lib.cmfo_calc_outpre(C.byref(self.s))
+
+    def physics_advance(self, dt: float, nt_cap: int = 0, budget: bool = False):
+        """One outer step; returns NT, or (NT, budget sums [NT, 11] of every
+        substep) when budget=True."""
         d = C.c_double(dt)
-        return int(self.lib.cmfo_physics_advance(C.byref(self.s), C.byref(d), nt_cap))
+        if not budget:
+            return int(self.lib.cmfo_physics_advance(C.byref(self.s), C.byref(d), nt_cap))
+        cap = 8192
+        hist = np.zeros((cap, 11))
+        self.s.glb_hist, self.s.glb_cap = hist.ctypes.data, cap
+        try:
+            nt = int(self.lib.cmfo_physics_advance(C.byref(self.s), C.byref(d), nt_cap))
+        finally:
+            self.s.glb_hist, self.s.glb_cap = None, 0
+        nrun = min(nt if nt_cap <= 0 else min(nt, nt_cap), cap)
+        return nt, hist[:nrun].copy()
 
     @property
     def glb(self) -> dict:

@@ -43,7 +43,12 @@ GOLDEN = {
     "g_hourly": (90, 45, 16, 1, 3600.0, True, True, True, 20.0, 0.3),
     "c1_1deg":  (360, 180, 2026, 30, 86400.0, False, False, False, 1.5, 0.0),
     "g_kine":   (90, 45, 17, 4, 86400.0, False, False, True, 20.0, 0.0),
+    "g_dam_yby": (90, 45, 18, 4, 86400.0, False, False, True, 20.0, 0.0),
 }
+# cases whose dams carry construction years (LDAMYBY, damout:216-225): cycled over
+# the dams, against the start year 2001 of the reference inputs: 0 (no year, active),
+# 1990 (active), 2001 (new this year), 2005 (not yet built)
+YBY_YEARS = {"g_dam_yby": (0, 1990, 2001, 2005)}
 # cases run with LKINE = .TRUE. (kinematic-wave outflow, src/cmf_opt_outflw_mod.F90:27-127)
 KINE_CASES = ("g_kine",)
 FINAL_KEYS = ["rivsto", "fldsto", "rivout", "fldout", "outflw", "rivvel"]
@@ -66,7 +71,7 @@ def network_for(name):
     if llev:
         synth.add_levees(net, levfrac, seed, hmin=0.5, hmax=3.0, fmin=0.05, fmax=0.5)
     if ldam:
-        synth.add_dams(net, max(3, net.ncells // 100), seed)
+        synth.add_dams(net, max(3, net.ncells // 100), seed, years=YBY_YEARS.get(name))
     nrec = min(days, 5) if dt >= 86400 else 4   # records are cycled by the driver
     ro = synth.make_runoff(net, nrec, seed, mean_mm_day=mean)
     return net, ro
@@ -162,6 +167,97 @@ def make_restart(name):
     print(f"{name}: restart {files[0]} ({rec['bin'].size} bytes)")
 
 
+# runs that start from the end-of-run restart file of a golden case: storage only
+# (LSTOONLY: CMF_RESTART_INIT reads P2RIVSTO/P2FLDSTO, then CMF_PHYSICS_FLDSTG and
+# CMF_CALC_OUTPRE rebuild the previous-step flows, drv_control:273-278) and the
+# full restart (the t-1 records read from the file). The dump after the
+# initialisation holds OUTPRE's result.
+RESTART_IN = {"g_allwet_sto": ("g_allwet", True, 2), "g_allwet_rst": ("g_allwet", False, 2)}
+
+
+def restart_input(base, lstoonly):
+    """(file bytes, .pth bytes or None) of the restart file a RESTART_IN run
+    reads: the base case's end-of-run restart; for LSTOONLY its records in the
+    storage-only layout of WRTE_REST_BIN (restart:418-447): P2RIVSTO, P2FLDSTO,
+    then P2DAMSTO (LDAMOUT) and P2LEVSTO (LLEVEE), the four t-1 records left out."""
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = GOLDEN[base]
+    z = np.load(os.path.join(HERE, f"{base}_restart.npz"), allow_pickle=False)
+    raw = z["bin"].tobytes()
+    rec = 8 * nx * ny
+    nfull = 6 + int(ldam) + int(llev)
+    if len(raw) != nfull * rec:
+        raise RuntimeError(f"{base}: restart file has {len(raw)} bytes, expected {nfull} records of {rec}")
+    recs = [raw[k * rec:(k + 1) * rec] for k in range(nfull)]
+    if lstoonly:
+        return b"".join(recs[:2] + recs[6:]), None
+    return raw, (z["pth"].tobytes() if lpth else None)
+
+
+def make_restart_in(name):
+    base, lstoonly, days = RESTART_IN[name]
+    nx, ny, seed, _days, dt, lpth, llev, ldam, mean, levfrac = GOLDEN[base]
+    net, ro = load_network(os.path.join(HERE, f"{base}_inputs.npz"))
+    binb, pthb = restart_input(base, lstoonly)
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, restart_in="restart_in.bin",
+                                     lstoonly=lstoonly)
+        with open(os.path.join(d, "restart_in.bin"), "wb") as f:
+            f.write(binb)
+        if pthb is not None:
+            with open(os.path.join(d, "restart_in.bin.pth"), "wb") as f:
+                f.write(pthb)
+        r = subprocess.run([REF, "runoff.bin", "dump.bin", "1"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr)
+        mp, states = refdump.read_dump(os.path.join(d, "dump.bin"), lpth, llev, ldam)
+        nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+    spec = [nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac]
+    out = dict(case=name, base=base, lstoonly=lstoonly, spec=spec, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
+               steps=[])
+    skeys = refdump.STATE_P + refdump.STATE_D + (["levsto", "levdph"] if llev else []) + \
+        (["damsto", "daminf"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])
+    for s_ in states:
+        out["steps"].append(dict(step=s_["step"], sha={k: sha(s_[k]) for k in skeys},
+                                 glb={k: float(v) for k, v in s_["glb"].items()}))
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    s0 = states[0]
+    k0 = ["rivsto", "fldsto", "rivout", "fldout", "rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre", "sfcelv",
+          "rivdph"] + (["pthflw", "pthflw_pre"] if lpth else []) + (["damsto"] if ldam else []) + \
+        (["levsto"] if llev else [])
+    np.savez_compressed(os.path.join(HERE, f"{name}_step0.npz"), **{k: s0[k] for k in k0})
+    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"), **{k: states[-1][k] for k in FINAL_KEYS})
+    print(f"{name}: n={mp['n']} steps={len(states) - 1} sum NT={sum(nts)} (from {base}, lstoonly={lstoonly})")
+
+
+# the reference in its default single-precision mode (-DSinglePrec_CMF, serial,
+# no FMA contraction) on the inputs of golden cases: the counterpart of the
+# single-precision drop-in (a JPRB = 4 host over the mixed device path)
+REF32 = os.path.join(ROOT, "oracle", "_ref", "serial32", "cmf_ref")
+SP_CASES = {"g_allwet_sp": "g_allwet", "g_hourly_sp": "g_hourly"}
+
+
+def make_sp(name):
+    base = SP_CASES[name]
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = GOLDEN[base]
+    net, ro = load_network(os.path.join(HERE, f"{base}_inputs.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     lpthout=lpth, llevee=llev, ldamout=ldam)
+        r = subprocess.run([REF32, "runoff.bin", "dump.bin", "1"], cwd=d, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr)
+        mp, states = refdump.read_dump(os.path.join(d, "dump.bin"), lpth, llev, ldam)
+        nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+    out = dict(case=name, base=base, spec=GOLDEN[base], n=mp["n"], nt=nts,
+               glb=[{k: float(v) for k, v in s_["glb"].items()} for s_ in states])
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"), **{k: states[-1][k] for k in FINAL_KEYS})
+    print(f"{name}: n={mp['n']} steps={len(states) - 1} sum NT={sum(nts)} (single precision, from {base})")
+
+
 OUTPUT_CASES = ("g_pth",)
 OUTPUT_VARS = ("rivout,fldout,outflw,rivvel,pthout,runoff,maxflw,maxdph,maxsto,rivsto,fldsto,rivdph,"
                "flddph,fldfrc,fldare,sfcelv,storge,pthflw")
@@ -191,7 +287,11 @@ def make_output(name):
 if __name__ == "__main__":
     names = sys.argv[1:] or list(GOLDEN)
     for name in names:
-        if name.endswith("_restart"):
+        if name in RESTART_IN:
+            make_restart_in(name)
+        elif name in SP_CASES:
+            make_sp(name)
+        elif name.endswith("_restart"):
             make_restart(name[:-len("_restart")])
         elif name.endswith("_output"):
             make_output(name[:-len("_output")])
@@ -202,3 +302,7 @@ if __name__ == "__main__":
             make_restart(name)
         for name in OUTPUT_CASES:
             make_output(name)
+        for name in RESTART_IN:
+            make_restart_in(name)
+        for name in SP_CASES:
+            make_sp(name)

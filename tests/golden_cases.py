@@ -13,7 +13,7 @@ from cmf import maps
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GDIR = os.path.join(HERE, "golden")
-NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg", "g_kine"]
+NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg", "g_kine", "g_dam_yby"]
 
 
 def sha(a):
@@ -61,3 +61,68 @@ def runoff_for_step(rvec, step, dt):
     ifrq = int(dt // 3600)
     spr = int(round(ifrq * 3600.0 / dt))
     return rvec[((step - 1) // spr) % len(rvec)]
+
+
+RESTART_IN_NAMES = ["g_allwet_sto", "g_allwet_rst"]
+
+
+def restart_init(m, st, binb, pthb, lstoonly):
+    """CMF_RESTART_INIT (src/cmf_ctrl_restart_mod.F90:95-133) on host arrays: zero
+    the storages, flows and t-1 values, read the FP64 records of READ_REST_BIN
+    (:140-213) placed by I1SEQX/I1SEQY, and with LSTOONLY set D2FLDSTO_PRE =
+    P2FLDSTO and D1PTHFLW_PRE = 0. Updates st in place."""
+    n = m.n
+    nxy = m.nx * m.ny
+    g = (m.seqy - 1) * m.nx + (m.seqx - 1)
+    recs = np.frombuffer(binb, np.float64)
+    fields = ["rivsto", "fldsto"]
+    if not lstoonly:
+        fields += ["rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre"]
+    if m.opts.ldamout:
+        fields.append("damsto")
+    if m.opts.llevee:
+        fields.append("levsto")
+    for k in ("rivsto", "fldsto", "rivout", "fldout", "rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre",
+              "damsto", "levsto"):
+        if k in st:
+            st[k] = np.zeros(n)
+    for r, k in enumerate(fields):
+        st[k] = recs[r * nxy:(r + 1) * nxy][g].copy()
+    if m.opts.lpthout:
+        st["pthflw"] = np.zeros((m.npthlev, m.npth))
+        if lstoonly:
+            st["pthflw_pre"] = np.zeros((m.npthlev, m.npth))
+        else:
+            st["pthflw_pre"] = np.frombuffer(pthb, np.float64).reshape(m.npthlev, m.npth).copy()
+    if lstoonly:
+        st["fldsto_pre"] = st["fldsto"].copy()
+    return st
+
+
+def restart_setup(name):
+    """Inputs of a run that starts from a restart file (tests/golden make_golden
+    RESTART_IN): map with dam tables, the state before CMF_RESTART_INIT (cold
+    start and its stage), the restart file bytes, runoff vectors and the
+    reference's outputs."""
+    import sys
+    from oracle.oracle import OracleModel
+    sys.path.insert(0, GDIR)
+    from make_golden import load_network, restart_input
+    with open(os.path.join(GDIR, f"{name}.json")) as f:
+        ref = json.load(f)
+    base, lstoonly = ref["base"], ref["lstoonly"]
+    net, ro = load_network(os.path.join(GDIR, f"{base}_inputs.npz"))
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
+    m = maps.build_map(net, maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam))
+    om = OracleModel(m, maps.cold_start(m))
+    om.physics_fldstg()                               # drv_control:240 (before the restart)
+    st = {k: v.copy() for k, v in om.st.items()}
+    if ldam:
+        maps.init_dams(m, net, {"rivsto": st["rivsto"].copy(), "fldsto": st["fldsto"].copy()}, restart=True)
+        st["daminf"] = np.zeros(m.n)
+    binb, pthb = restart_input(base, lstoonly)
+    step0 = dict(np.load(os.path.join(GDIR, f"{name}_step0.npz")))
+    fin = dict(np.load(os.path.join(GDIR, f"{name}_final.npz")))
+    rvec = [m.vec(ro[r]) for r in range(ro.shape[0])]
+    nsteps = int(round(days * 86400.0 / dt))
+    return m, net, st, binb, pthb, lstoonly, rvec, ref, step0, fin, dt, nsteps

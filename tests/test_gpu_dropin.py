@@ -152,3 +152,54 @@ def test_dropin_restart_matches_reference(name, exe, tmp_path):
         wp = z["pth"].view(np.float64)
         assert gp.size == wp.size
         assert np.abs(gp - wp).max() <= 1e-10 * max(np.abs(wp).max(), 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["g_allwet_sp", "g_hourly_sp"])
+def test_single_precision_dropin_matches_reference(name, tmp_path):
+    """The reference built in its default single precision (-DSinglePrec_CMF,
+    JPRB = 4) linked with this package's module: the JPRB arrays cross the C-ABI
+    as float, the device runs its mixed path (FP32 flux and stage arithmetic,
+    FP64 storage). Against the reference's own single-precision run of the same
+    inputs (tests/golden/*_sp: paths, levees, dams; daily and hourly DT):
+    identical NT at every step, final storage within 2e-5 of the largest cell
+    storage and 1e-6 in total, outflow within 1e-3 of the largest. The two single-
+    precision paths round differently (the reference's FP32 flux arithmetic
+    against the device's difference-form FP32 arithmetic), so they agree to the
+    size of either's distance from the FP64 reference (measured on these cases:
+    1e-6 cell, 2e-8 total, 3e-5 outflow), not bit for bit."""
+    import json
+    path = os.path.join(BUILD, "cmf_dropin32")
+    if not os.path.exists(path):
+        pytest.skip("cmf_dropin32 not built")
+    from cmf import synth
+    from oracle import refdump
+    gdir = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(gdir, f"{name}.json")) as f:
+        sp = json.load(f)
+    fin = dict(np.load(os.path.join(gdir, f"{name}_final.npz")))
+    net, ro, ref, _ = load(sp["base"])
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
+    d = str(tmp_path)
+    synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                 lpthout=lpth, llevee=llev, ldamout=ldam)
+    nsteps = len(sp["nt"])
+    env = dict(os.environ, CMF_HIP_DOWNLOAD="every")
+    r = subprocess.run([path, "runoff.bin", "dump.bin", str(nsteps)], cwd=d, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    nts = refdump.count_substeps(os.path.join(d, "log_CaMa.txt"))
+    log = open(os.path.join(d, "log_CaMa.txt")).read()
+    assert "ADPSTP: NT=" in log
+    mp, states = refdump.read_dump(os.path.join(d, "dump.bin"), lpth, llev, ldam)
+    last = states[-1]
+    assert last["step"] == nsteps
+    print(f"  NT device {nts[:6]}... reference {sp['nt'][:6]}...", flush=True)
+    assert nts == sp["nt"]
+    tot_d = last["rivsto"] + last["fldsto"]
+    tot_r = fin["rivsto"] + fin["fldsto"]
+    e_cell = float(np.abs(tot_d - tot_r).max() / np.abs(tot_r).max())
+    e_tot = abs(tot_d.sum() - tot_r.sum()) / tot_r.sum()
+    e_out = rel_err(last["outflw"], fin["outflw"])
+    print(f"  storage cell {e_cell:.3e} total {e_tot:.3e} outflow {e_out:.3e}", flush=True)
+    assert e_cell <= 2e-5 and e_tot <= 1e-6 and e_out <= 1e-3

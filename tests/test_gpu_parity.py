@@ -55,3 +55,35 @@ def test_fp64_matches_oracle(name):
         assert nt_o == nt_g
         for k, v in e.items():
             assert v <= TOL_FP64, f"{name} day {day + 1}: {k} rel err {v:.3e}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["plain", "allwet"])
+def test_dt_min_closed_form(name):
+    """DT_MIN of CALC_ADPSTP (physics:178-197) returned by cmf_hip_get_dt_min
+    equals the closed form min(DT_DEF, min over I2MASK == 0 of
+    PCADP * dx / sqrt(PGRV * max(RIVDPH, 0.01))), dx = NXTDST for river cells
+    and PDSTMTH for mouths, on the stage at the start of the outer step; and NT
+    follows from it as physics:205 writes it."""
+    from cmf.model import CMFModel
+    from oracle.oracle import OracleModel
+    m, net, st, rvec = prepare(name, 2)
+    om = OracleModel(m, st)
+    gm = CMFModel(m, "fp64")
+    gm.set_state(upload_keys(st))
+    o = m.opts
+    for d in range(2):
+        om.physics_fldstg()                       # the stage CALC_ADPSTP sees (physics:54)
+        ddph = np.maximum(om.st["rivdph"], 0.01)
+        dx = np.where(m.next >= 0, m.nxtdst, o.pdstmth)
+        c = (o.pcadp * dx) * (1.0 / np.sqrt(o.pgrv * ddph))
+        want = min(86400.0, float(c[m.i2mask == 0].min()))
+        om.st["runoff"][:] = rvec[d]
+        gm.set_runoff(rvec[d])
+        nt_o = om.physics_advance(86400.0)
+        nt_g = gm.physics_advance(86400.0)
+        got = float(gm.dt_min()[0])
+        print(f"  day {d + 1}: DT_MIN device {got!r} closed form {want!r}, NT {nt_g}", flush=True)
+        assert got == want
+        assert nt_g == nt_o == int(86400.0 * (1.0 / got) - float(np.float32(0.01))) + 1
+    gm.close()

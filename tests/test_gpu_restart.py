@@ -92,3 +92,50 @@ def test_restart_matches_reference(name):
             assert np.array_equal(back4[k], got[r, cell].astype(np.float32).astype(np.float64)), k
         g2.close()
     gm.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["g_allwet_sto", "g_allwet_rst"])
+def test_run_from_restart_matches_reference(name):
+    """A run started from the reference's restart file, storage only (LSTOONLY:
+    cmf_hip_restart_read, cmf_hip_physics_fldstg, cmf_hip_calc_outpre, as
+    CMF_DRV_INIT does, drv_control:273-278) or full, against the reference's own
+    run from the same file (paths, levees, dams): the state after the
+    initialisation (OUTPRE's previous-step flows included) within 1e-12, then
+    identical NT at every step and the final fields within 1e-10. The first
+    substep's RIVVEL reads D2RIVOUT = 0 (CMF_RESTART_INIT zeroes it), which the
+    final RIVVEL average carries."""
+    from cases import rel_err
+    from cmf.model import CMFModel
+    from golden_cases import restart_setup
+    print(f"run from restart {name}", flush=True)
+    m, net, st, binb, pthb, lstoonly, rvec, ref, step0, fin, dt, nsteps = restart_setup(name)
+    gm = CMFModel(m, "fp64")
+    gm.set_state(upload_keys(st))
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "restart_in.bin")
+        with open(f, "wb") as fh:
+            fh.write(binb)
+        if pthb is not None:
+            with open(f + ".pth", "wb") as fh:
+                fh.write(pthb)
+        gm.restart_read(f, lstoonly=lstoonly)
+    if lstoonly:
+        gm.physics_fldstg()
+        gm.calc_outpre()
+    g = gm.get_state(list(step0))
+    for k, v in step0.items():
+        e = rel_err(g[k], v)
+        print(f"  after init {k}: rel err {e:.3e}", flush=True)
+        assert e <= 1e-12, f"{name} after init: {k} rel err {e:.3e}"
+    nts = []
+    for step in range(1, nsteps + 1):
+        gm.set_runoff(runoff_for_step(rvec, step, dt))
+        nts.append(gm.physics_advance(dt))
+    g = gm.get_state(list(fin) + ["rivvel_avg"])
+    gm.close()
+    assert nts == ref["nt"]
+    for k, v in fin.items():
+        e = rel_err(g[k], v)
+        print(f"  final {k}: rel err {e:.3e}", flush=True)
+        assert e <= TOL, f"{name}: {k} rel err {e:.3e}"

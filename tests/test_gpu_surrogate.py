@@ -11,7 +11,10 @@ of one handle.
 Checked: the prior and the warm start begin within 5% of the cold start's
 distance from the equilibrium, the warm start changes less than 5% as much in
 its first year, and neither needs more years than the cold start to reach an
-annual storage change below 0.1%."""
+annual storage change below 0.1%. What the trained network adds to the method
+without it (the physical prior it corrects): the warm start begins closer to the
+equilibrium than the prior, changes less in its first year, and needs no more
+years."""
 import numpy as np
 import pytest
 

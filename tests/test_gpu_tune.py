@@ -70,3 +70,32 @@ def test_autotune_restores_state():
         assert np.array_equal(sa[k], sb[k]), k
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_graph_replay_after_budget_buffer_growth():
+    """Graph replay stays correct when the per-substep budget buffers grow: the
+    substep count is capped 10, 10, 20, 10. The second call captures the
+    10-substep graph, the third needs more budget rows (the buffers are
+    reallocated and every cached graph is dropped), the fourth replays a
+    10-substep graph again. State and budget sums equal the run without
+    graphs."""
+    from cmf.model import CMFModel
+    m, net, st, rvec = prepare("all", 1)
+    runs = []
+    for graph in (0, 1):
+        gm = CMFModel(m, "fp64")
+        gm.set_tune(graph=graph)
+        gm.set_state(upload_keys(st))
+        gm.set_runoff(rvec[0])
+        out = []
+        for cap in (10, 10, 20, 10):
+            nt, glb = gm.physics_advance(86400.0, nt_cap=cap, budget=True)
+            out.append((glb, gm.get_state()))
+        gm.close()
+        runs.append(out)
+    for k, ((g0, s0), (g1, s1)) in enumerate(zip(*runs)):
+        assert g0.shape == g1.shape
+        for key in s0:
+            assert np.array_equal(s0[key], s1[key]), f"call {k + 1}: {key} differs"
+        assert np.allclose(g0, g1, rtol=1e-12, atol=0.0), f"call {k + 1}: budget sums differ"
