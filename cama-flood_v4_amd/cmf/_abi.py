@@ -39,7 +39,7 @@ class cmf_options(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("ladpstp", "lfplain", "lfldout", "lpthout", "ldamout",
                                            "llevee", "precision")] + \
                [(k, C.c_double) for k in ("pgrv", "pmanfld", "pdstmth", "pcadp", "pminslp")] + \
-               [("jprb", C.c_int32)]
+               [("jprb", C.c_int32), ("ldamh22", C.c_int32)]
 
 
 MAP_FIELDS = [("i1next", P)] + [(k, P) for k in (
@@ -47,7 +47,7 @@ MAP_FIELDS = [("i1next", P)] + [(k, P) for k in (
     "fldhgt fldstomax fldgrd").split()] + [("dfrcinc", C.c_double)] + [(k, P) for k in (
     "i2mask pth_upst pth_down pth_dst pth_elv pth_wth pth_man levhgt levfrc bashgt levdst "
     "levbassto levtopsto levfilsto i1dam damseq damstat dam_convol dam_emevol dam_adjvol "
-    "dam_qa dam_qn dam_qf").split()]
+    "dam_qa dam_qn dam_qf dam_norvol dam_r_volupa").split()]
 
 
 class cmf_map(C.Structure):
@@ -96,6 +96,7 @@ HIP_FUNCS = {
     "cmf_hip_set_dwnelv": (I, [H, P]),
     "cmf_hip_set_upsinf": (I, [H, P]),
     "cmf_hip_set_kinematic": (I, [H, C.c_int32]),
+    "cmf_hip_set_slopemix": (I, [H, C.c_int32]),
     "cmf_hip_restart_write": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32, C.c_double,
                                   C.c_double]),
     "cmf_hip_restart_read": (I, [H, C.c_char_p, C.c_int32, C.c_int32, P, P, C.c_int32, C.c_int32]),

@@ -31,6 +31,7 @@ class Options:
     llevee: bool = False
     lkine: bool = False           # kinematic-wave outflow (CMF_CALC_OUTFLW_KINE)
     ldamh22: bool = False         # Hanazaki-2022 dam rule (NDAMOUT LDAMH22)
+    lslpmix: bool = False         # slope-mixed outflow (CMF_CALC_OUTFLW_KINEMIX; I2MASK = 1 cells kinematic)
     pmanriv: float = 0.03
     pmanfld: float = 0.10
     pgrv: float = 9.8
@@ -238,6 +239,15 @@ def cold_start(m: CMFMap) -> dict:
     return st
 
 
+def apply_slope_mask(m: CMFMap, mask: np.ndarray) -> None:
+    """LSLPMIX: mark the cells of the slope mask ([ny, nx], 1 = kinematic wave;
+    SET_SLOPEMIX, src/cmf_ctrl_maps_mod.F90:1005-1040) in I2MASK. Cells already
+    coded by the dam set-up (2 at dams, 1 upstream) keep their codes, as when the
+    mask is read first and CMF_DAMOUT_INIT writes over it."""
+    v = np.asarray(mask)[m.seqy - 1, m.seqx - 1]
+    m.i2mask[(m.i2mask == 0) & (v == 1)] = 1
+
+
 def init_dams(m: CMFMap, net, st: dict, restart: bool = False, syear: int = 2001) -> None:
     """CMF_DAMOUT_INIT (src/cmf_ctrl_damout_mod.F90:123-308): the Yamazaki & Funato
     rule, or with m.opts.ldamh22 the Hanazaki-2022 rule (NorVol and the ratio of
@@ -246,7 +256,9 @@ def init_dams(m: CMFMap, net, st: dict, restart: bool = False, syear: int = 2001
     (LDAMYBY) when the dam rows carry a "year", against the start year syear
     (ISYYYY, damout:216-225). Must run after the initial flood-stage call, as in
     src/cmf_drv_control_mod.F90:243-252; updates st in place. With restart (LRESTART)
-    the dam storage read from the restart file is kept (damout:256-285)."""
+    the dam storage read from the restart file is kept, except that a dam new in the
+    start year (LDAMYBY) starts from its cell's river and floodplain storage
+    (damout:256-285, LiVnorm off)."""
     f32 = lambda x: float(np.float32(x))
     vecidx = np.zeros((m.ny, m.nx), np.int32)
     vecidx[m.seqy - 1, m.seqx - 1] = np.arange(1, m.n + 1)
@@ -305,6 +317,11 @@ def init_dams(m: CMFMap, net, st: dict, restart: bool = False, syear: int = 2001
                 m.i2mask[i] = 2
     if not restart:
         st["damsto"] = np.zeros(m.n)
+    if restart and yby:                                    # damout:274-285 (LiVnorm = .FALSE.)
+        for d in range(nd):
+            if damstat[d] == 1:
+                s = damseq[d]
+                st["damsto"][s] = st["rivsto"][s] + st["fldsto"][s]
     for d in range(nd):                                    # damout:257-272
         if restart or damstat[d] == -9999:
             continue

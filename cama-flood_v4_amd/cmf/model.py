@@ -63,7 +63,8 @@ class CMFModel:
                      m.npthlev if o.lpthout else 0, m.ndam if o.ldamout else 0)
         op = cmf_options(int(o.ladpstp), int(o.lfplain), int(o.lfldout), int(o.lpthout),
                          int(o.ldamout), int(o.llevee), self.prec,
-                         o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp, 8)
+                         o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp, 8,
+                         int(bool(getattr(o, "ldamh22", False))))
         h = C.c_void_p()
         if self.ensemble:
             dc = np.ascontiguousarray(self.dom_cells, np.int32)
@@ -76,6 +77,8 @@ class CMFModel:
         self._set_map(m)
         if getattr(m.opts, "lkine", False):
             self.set_kinematic(True)
+        if getattr(m.opts, "lslpmix", False):
+            self.set_slopemix(True)
 
     # ------------------------------------------------------------------
     def _check(self, rc):
@@ -111,6 +114,9 @@ class CMFModel:
             mp.damstat = self._p(_i32(m.damstat))
             for k in ("dam_convol", "dam_emevol", "dam_adjvol", "dam_qa", "dam_qn", "dam_qf"):
                 setattr(mp, k, self._p(_f64(getattr(m, k))))
+            if getattr(o, "ldamh22", False):
+                mp.dam_norvol = self._p(_f64(m.dam_norvol))
+                mp.dam_r_volupa = self._p(_f64(m.dam_r_volupa))
         self._check(self.lib.cmf_hip_set_map(self.h, C.byref(mp)))
         self._keep.clear()
 
@@ -253,6 +259,12 @@ class CMFModel:
         """LKINE: kinematic-wave outflow (CMF_CALC_OUTFLW_KINE) instead of the
         local inertial one; applied from maps.Options.lkine at construction."""
         self._check(self.lib.cmf_hip_set_kinematic(self.h, int(bool(on))))
+
+    def set_slopemix(self, on: bool) -> None:
+        """LSLPMIX: CMF_CALC_OUTFLW_KINEMIX, the kinematic wave on the cells with
+        I2MASK != 0 of the map (maps.apply_slope_mask), local inertial elsewhere;
+        applied from maps.Options.lslpmix at construction."""
+        self._check(self.lib.cmf_hip_set_slopemix(self.h, int(bool(on))))
 
     def update_boundary(self, meansl: Optional[np.ndarray] = None, sealev: Optional[np.ndarray] = None) -> None:
         """D2DWNELV = D2ELEVTN (+ meansl) (+ sealev) on the device."""

@@ -59,6 +59,7 @@ class SynthNetwork:
     levhgt: Optional[np.ndarray] = None
     levfrc: Optional[np.ndarray] = None
     dams: Optional[list] = None       # rows of the dam table
+    slpmix_mask: Optional[np.ndarray] = None   # int32 [ny,nx], 1 = kinematic wave (LSLPMIX)
 
     @property
     def land(self) -> np.ndarray:
@@ -242,6 +243,22 @@ def add_dams(net: SynthNetwork, ndam: int, seed: int, mu=math.log(1e8), sigma=1.
     net.dams = rows
 
 
+def add_slope_mask(net: SynthNetwork, frac: float) -> None:
+    """The LSLPMIX mask (mask_slope of the reference's netCDF river map): 1 on the
+    fraction `frac` of river cells with the steepest bed slope to their downstream
+    cell, where the kinematic wave replaces the local inertial equations."""
+    ny, nx = net.ny, net.nx
+    iy, ix = np.nonzero(net.nextx > 0)
+    jx, jy = net.nextx[iy, ix] - 1, net.nexty[iy, ix] - 1
+    slope = (net.elevtn[iy, ix].astype(np.float64) - net.elevtn[jy, jx]) / net.nxtdst[iy, ix]
+    mask = np.zeros((ny, nx), np.int32)
+    k = int(round(frac * slope.size))
+    if k > 0:
+        sel = np.argsort(-slope, kind="stable")[:k]
+        mask[iy[sel], ix[sel]] = 1
+    net.slpmix_mask = mask
+
+
 def make_runoff(net: SynthNetwork, nrec: int, seed: int, mean_mm_day=1.5, shape=0.5,
                 beta=3.0, scale_factors=None) -> np.ndarray:
     """Runoff records [nrec, ny, nx] float64 in m3/s per cell: a smooth Gaussian field
@@ -364,14 +381,18 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
                            pmanriv=0.03, pmanfld=0.10, pcadp=0.7, pdstmth=10000.0,
                            restart: bool = False, outputs: Optional[str] = None,
                            lkine: bool = False, restart_in: Optional[str] = None,
-                           lstoonly: bool = False, ldamh22: bool = False) -> None:
+                           lstoonly: bool = False, ldamh22: bool = False,
+                           slpmix_mask: Optional[np.ndarray] = None) -> None:
     """Write the map files, tables, runoff records and input_cmf.nam that the
     reference needs (formats of src/cmf_ctrl_maps_mod.F90 READ_MAP_BIN,
     READ_TOPO_BIN, READ_BIFPARAM; src/cmf_ctrl_damout_mod.F90:141-179;
     src/cmf_ctrl_levee_mod.F90:121-133). restart_in names an FP64 binary restart
     file in d to start from (LRESTART, CRESTSTO; LSTOONLY with lstoonly). Dams with a
     "year" write the year column and switch LDAMYBY on; ldamh22 selects the
-    Hanazaki-2022 dam rule (LDAMH22)."""
+    Hanazaki-2022 dam rule (LDAMH22). slpmix_mask ([ny, nx], 1 = kinematic wave)
+    switches LSLPMIX on and is written to slpmix_mask.bin, which the test driver
+    oracle/ref_driver.F90 applies to I2MASK (the reference reads it from its
+    netCDF river map only)."""
     os.makedirs(d, exist_ok=True)
     nx, ny = net.nx, net.ny
 
@@ -411,6 +432,8 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
                         f"{r['ix']},{r['iy']},{r['fldvol']!r},{r['convol']!r},{r['totvol']!r},"
                         f"{r['qn']!r},{r['qf']!r}" + (f",{r['year']}\n" if ldamyby else "\n"))
     np.ascontiguousarray(runoff, dtype=np.float64).tofile(os.path.join(d, "runoff.bin"))
+    if slpmix_mask is not None:
+        np.ascontiguousarray(slpmix_mask, dtype=np.int32).tofile(os.path.join(d, "slpmix_mask.bin"))
 
     import datetime as _dt
     t0 = _dt.datetime(2001, 1, 1)
@@ -428,6 +451,7 @@ LPTHOUT  = {L(lpthout)}
 LDAMOUT  = {L(ldamout)}
 LLEVEE   = {L(llevee)}
 LKINE    = {L(lkine)}
+LSLPMIX  = {L(slpmix_mask is not None)}
 LRESTART = {L(restart_in is not None)}
 LSTOONLY = {L(lstoonly)}
 LOUTPUT  = {L(outputs is not None)}

@@ -80,6 +80,7 @@ struct cmf_hip_ctx {
     double* dwnelv_dev = nullptr;                    // D2DWNELV in the device map (boundary updates)
     int upsinf_on = 0;                               // LUPSINF switched on by cmf_hip_set_upsinf
     int lkine = 0;                                   // LKINE set by cmf_hip_set_kinematic
+    int lslpmix = 0;                                 // LSLPMIX set by cmf_hip_set_slopemix
     double *d_meansl = nullptr, *d_sealev = nullptr; // staging of cmf_hip_update_boundary
     double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
@@ -595,6 +596,7 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.xcd = h->tune.xcd;
     A.lupsinf = h->upsinf_on;
     A.lkine = h->lkine;
+    A.lslpmix = h->lslpmix && !h->lkine;                 // physics:66-72: LKINE first
     return A;
 }
 
@@ -1081,7 +1083,8 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp_in) {
         wm.pth_elv = w(wm.pth_elv, np * nlv);
         wm.pth_wth = w(wm.pth_wth, np * nlv);
         wm.pth_man = w(wm.pth_man, nlv);
-        for (const double** f : {&wm.dam_convol, &wm.dam_emevol, &wm.dam_adjvol, &wm.dam_qa, &wm.dam_qn, &wm.dam_qf})
+        for (const double** f : {&wm.dam_convol, &wm.dam_emevol, &wm.dam_adjvol, &wm.dam_qa, &wm.dam_qn, &wm.dam_qf,
+                                 &wm.dam_norvol, &wm.dam_r_volupa})
             *f = w(*f, nd);
     }
     const cmf_map* mp = &wm;
@@ -1264,9 +1267,11 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp_in) {
     if (h->o.ldamout) {
         const int nd = h->d.ndam;
         M.ndam = nd;
-        if (!mp->i1dam || !mp->damseq || !mp->damstat || !mp->dam_convol || !mp->dam_emevol || !mp->dam_adjvol ||
-            !mp->dam_qa || !mp->dam_qn || !mp->dam_qf)
-            return fail(CMF_EINVAL, "LDAMOUT: dam tables missing");
+        const bool h22 = h->o.ldamh22 != 0;
+        if (!mp->i1dam || !mp->damseq || !mp->damstat || !mp->dam_convol || !mp->dam_emevol || !mp->dam_qn ||
+            !mp->dam_qf || (!h22 && (!mp->dam_adjvol || !mp->dam_qa)) ||
+            (h22 && (!mp->dam_norvol || !mp->dam_r_volupa)))
+            return fail(CMF_EINVAL, "LDAMOUT: dam tables missing (LDAMH22=%d)", (int)h22);
         std::vector<int32_t> doc(n, -1);
         for (int d = 0; d < nd; ++d) {
             if (mp->damstat[d] == -9999) continue;
@@ -1280,10 +1285,14 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp_in) {
         TRY(put_raw(h, &p32, mp->damstat, (size_t)nd)); M.damstat = p32;
         TRY(put_raw(h, &pd, mp->dam_convol, (size_t)nd)); M.dam_convol = pd;
         TRY(put_raw(h, &pd, mp->dam_emevol, (size_t)nd)); M.dam_emevol = pd;
-        TRY(put_raw(h, &pd, mp->dam_adjvol, (size_t)nd)); M.dam_adjvol = pd;
-        TRY(put_raw(h, &pd, mp->dam_qa, (size_t)nd)); M.dam_qa = pd;
+        const std::vector<double> zero((size_t)nd, 0.0);      // tables of the rule not in use
+        TRY(put_raw(h, &pd, h22 && !mp->dam_adjvol ? zero.data() : mp->dam_adjvol, (size_t)nd)); M.dam_adjvol = pd;
+        TRY(put_raw(h, &pd, h22 && !mp->dam_qa ? zero.data() : mp->dam_qa, (size_t)nd)); M.dam_qa = pd;
         TRY(put_raw(h, &pd, mp->dam_qn, (size_t)nd)); M.dam_qn = pd;
         TRY(put_raw(h, &pd, mp->dam_qf, (size_t)nd)); M.dam_qf = pd;
+        M.ldamh22 = h22 ? 1 : 0;
+        TRY(put_raw(h, &pd, h22 ? mp->dam_norvol : zero.data(), (size_t)nd)); M.dam_norvol = pd;
+        TRY(put_raw(h, &pd, h22 ? mp->dam_r_volupa : zero.data(), (size_t)nd)); M.dam_r_volupa = pd;
     }
     {
         // first cell of every domain in device order (domains stay contiguous)
@@ -1554,6 +1563,16 @@ int cmf_hip_set_kinematic(cmf_handle h, int32_t lkine) {
     if (lkine != 0 && lkine != 1) return fail(CMF_EINVAL, "lkine=%d must be 0 or 1", (int)lkine);
     if (h->lkine != lkine) {
         h->lkine = lkine;
+        drop_graphs(h);                                  // captured graphs hold the old arguments
+    }
+    return 0;
+}
+
+int cmf_hip_set_slopemix(cmf_handle h, int32_t lslpmix) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (lslpmix != 0 && lslpmix != 1) return fail(CMF_EINVAL, "lslpmix=%d must be 0 or 1", (int)lslpmix);
+    if (h->lslpmix != lslpmix) {
+        h->lslpmix = lslpmix;
         drop_graphs(h);                                  // captured graphs hold the old arguments
     }
     return 0;

@@ -34,9 +34,12 @@ constexpr double K_1EM10F = (double)1.e-10f;
 constexpr double K_0P01F  = (double)0.01f;
 constexpr double K_2O3F   = (double)(2.f / 3.f);
 constexpr double K_0P1F   = (double)0.1f;
+constexpr double K_0P2F   = (double)0.2f;
 constexpr double K_1E18F  = (double)1.e18f;
 constexpr double K_P7O3   = -7.0 / 3.0;     // -7._JPRB/3._JPRB
 constexpr double K_P4O3   = -4.0 / 3.0;     // -4._JPRB/3._JPRB
+constexpr double K_P7O3F  = (double)(-7.f / 3.f);   // default-real -7./3.
+constexpr double K_P4O3F  = (double)(-4.f / 3.f);   // default-real -4./3.
 constexpr double K_LEVSENT = 1.e18;         // 1.E18_JPRB sentinel of non-levee cells
 
 constexpr int NGLB = 11;
@@ -85,6 +88,9 @@ struct DevMap {
     const int32_t* dam_of_cell;                // per cell: dam row or -1
     const int32_t* damstat;
     const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
+    // Hanazaki-2022 rule (LDAMH22): NorVol and R_VolUpa of every dam
+    int32_t ldamh22;
+    const double *dam_norvol, *dam_r_volupa;
 };
 
 // Per-instance step of the current outer step (written by the host after the
@@ -148,6 +154,7 @@ struct StepArgs {
     int32_t xcd;                               // XCD-contiguous tile order
     int32_t lupsinf;                           // add S.upsinf to the river in STONXT
     int32_t lkine;                             // kinematic-wave outflow (LKINE)
+    int32_t lslpmix;                           // slope-mixed outflow (LSLPMIX, without LKINE)
 };
 
 // Cell / path slot -> static index, member and instance. MB = false is the

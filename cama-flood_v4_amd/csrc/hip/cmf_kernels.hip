@@ -152,12 +152,108 @@ __device__ __forceinline__ void kine_outflow(const DevMap& M, const DevState<R>&
     }
 }
 
+// Slope-mixed outflow, CMF_CALC_OUTFLW_KINEMIX (src/cmf_opt_outflw_mod.F90:118-300),
+// for a cell with I2MASK = 0 (the others take kine_outflow): the local inertial
+// equations in that routine's variant. Against CMF_CALC_OUTFLW: no floor on the
+// river flow area, RIVVEL from the new outflow, the default-real exponents -7./3.
+// and -4./3., the floodplain tested on its semi-implicit area against a
+// default-real 1.E-5, the floodplain flow dropped only when it opposes the river
+// flow (product < 0), and no reverse-flow limiter. Evaluated in FP64 for either
+// state type R, in the reference's statement order.
+template <class R, bool MB>
+__device__ __forceinline__ void kinemix_outflow(const DevMap& M, const DevState<R>& S, const StepArgs& A, int i,
+                                                int nxt, int s, int m, double dt, R& q_r, R& q_f, R& vel) {
+    const double gdt = A.pgrv * dt;
+    const double rivelv = rivelv_of(M, i), elevtn = (double)M.elevtn[i];
+    const double rivwth = (double)M.rivwth[i], rivman = (double)M.rivman[i];
+    const double rd = (double)S.rivdph[s], rdp = (double)S.rivdph_pre[s];
+    const double sfc = rivelv + rd, sfc_pre = rivelv + rdp;                  // :141-142
+    const double fdp = vmax(rdp - (double)M.rivhgt[i], 0.0);                 // :143
+    double dslope, dflw, dflw_pre, fa, fap;
+    if (nxt >= 0) {                                                          // :149-175
+        const int sj = sidx<MB>(M, nxt, m);
+        const double rivelv_j = rivelv_of(M, nxt);
+        const double sfc_j = rivelv_j + (double)S.rivdph[sj], sfc_j_pre = rivelv_j + (double)S.rivdph_pre[sj];
+        const double dsfcmax = vmax(sfc, sfc_j), dsfcmax_pre = vmax(sfc_pre, sfc_j_pre);
+        dslope = (sfc - sfc_j) * (1.0 / (double)M.nxtdst[i]);
+        dflw = dsfcmax - rivelv;
+        dflw_pre = dsfcmax_pre - rivelv;
+        fa = vmax(dsfcmax - elevtn, 0.0);                                   // :179
+        fap = dsfcmax_pre - elevtn;                                         // :184
+    } else {                                                                 // :232-253
+        dslope = (sfc - (double)M.dwnelv[i]) * (1.0 / A.pdstmth);
+        dflw = rd;
+        dflw_pre = rdp;
+        fa = sfc - elevtn;                                                  // :256
+        fap = sfc_pre - elevtn;                                             // :261
+    }
+    const double darea = rivwth * dflw;
+    double dflw_imp = sqrt(dflw * dflw_pre);
+    dflw_imp = vmax(dflw_imp, 1.e-6);
+    double qr = 0.0, v = 0.0;
+    if (dflw_imp > K_1EM5F && darea > K_1EM5F) {
+        const double dout_pre = (double)S.rivout[s] * (1.0 / rivwth);     // D2RIVOUT_PRE
+        qr = (rivwth * (dout_pre + (gdt * dflw_imp) * dslope)) *
+             (1.0 / (1.0 + ((gdt * (rivman * rivman)) * fabs(dout_pre)) * pow(dflw_imp, K_P7O3F)));
+        v = qr * (1.0 / darea);
+    }
+    double qf = 0.0;
+    if (A.lfldout) {                                                         // :178-200, :255-277
+        const double dslope_f = vmax(-0.005, vmin(0.005, dslope));
+        const double rrl = 1.0 / (double)M.rivlen[i];
+        const double fd = (double)S.flddph[s];
+        double dare_f = S.fldsto[s] * rrl;
+        dare_f = vmax(dare_f - fd * rivwth, 0.0);
+        const double t = fa * fap;
+        double dflw_imp_f = sqrt(vmax(t, 0.0));
+        dflw_imp_f = vmax(dflw_imp_f, 1.e-6);
+        double dare_pre_f = (double)S.fldsto_pre[s] * rrl;
+        dare_pre_f = vmax(dare_pre_f - fdp * rivwth, 1.e-6);
+        double dare_imp_f = sqrt(dare_f * dare_pre_f);
+        dare_imp_f = vmax(dare_imp_f, 1.e-6);
+        if (dflw_imp_f > K_1EM5F && dare_imp_f > K_1EM5F) {
+            const double dout_pre_f = (double)S.fldout[s];                  // D2FLDOUT_PRE
+            qf = (dout_pre_f + (gdt * dare_imp_f) * dslope_f) *
+                 (1.0 / (1.0 + (((gdt * (A.pmanfld * A.pmanfld)) * fabs(dout_pre_f)) * pow(dflw_imp_f, K_P4O3F)) *
+                                   (1.0 / dare_imp_f)));
+        }
+        if (qf * qr < 0.0) qf = 0.0;
+    }
+    q_r = R(qr);
+    q_f = R(qf);
+    vel = R(v);
+}
+
 // ---------------------------------------------------------------------------
 // phase 1a: river / mouth / floodplain outflow and dam operation, per cell slot
 // KINE selects the kinematic wave at compile time: k_outflow is instantiated per
 // scheme and the launch picks the one StepArgs::lkine asks for, so the default
 // (local inertial) kernel is compiled as if the option did not exist; the
 // persistent k_substep carries both and branches once per tile.
+// Hanazaki-2022 reservoir operation (LDAMH22), src/cmf_ctrl_damout_mod.F90:354-378,
+// before the storage limiter. Zones: normal (<= NorVol), water supply (<= ConVol),
+// flood control (< EmeVol), emergency. r_volupa is the dam's own flood-control
+// volume over its upstream area (damout:204 stores every dam's ratio at index NDAM;
+// here each dam reads its own). Default-real constants 1. and 0.2 widened.
+__device__ __forceinline__ double dam_rule_h22(double vol, double inflow, double norvol, double convol,
+                                               double emevol, double qn, double qf, double r_volupa) {
+    if (vol <= norvol) return qn * (vol / convol);
+    const double quad = (vol - norvol) / (emevol - norvol);        // ((V-NorVol)/(EmeVol-NorVol))**2
+    const double pre = qn * 0.5 + (quad * quad) * (qf - qn);
+    if (vol <= convol) {
+        if (qf <= inflow) return qn * 0.5 + ((vol - norvol) / (convol - norvol)) * (qf - qn);
+        return pre;
+    }
+    if (vol < emevol) {
+        if (qf <= inflow) {
+            const double cut = vmax(1.0 - r_volupa / K_0P2F, 0.0);
+            return qf + ((cut * (vol - convol)) / (emevol - convol)) * (inflow - qf);
+        }
+        return pre;
+    }
+    return vmax(inflow, qf);
+}
+
 template <class R, bool MB, bool KINE>
 __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>& S, const StepArgs& A, int s) {
     const Slot x = cell_slot<MB>(M, s);
@@ -193,8 +289,10 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
     R q_r, q_f = zero, vel;
     if constexpr (KINE) {
         // LKINE: CMF_CALC_OUTFLW_KINE (opt_outflw:27-127) in place of the local
-        // inertial equations and their reverse-flow limiter
-        kine_outflow<R>(M, S, A, i, nxt, s, I.dt, q_r, q_f, vel);
+        // inertial equations and their reverse-flow limiter; LSLPMIX (without
+        // LKINE): CMF_CALC_OUTFLW_KINEMIX, the same kinematic wave where I2MASK != 0
+        if (A.lslpmix && M.i2mask[i] == 0) kinemix_outflow<R, MB>(M, S, A, i, nxt, s, m, I.dt, q_r, q_f, vel);
+        else kine_outflow<R>(M, S, A, i, nxt, s, I.dt, q_r, q_f, vel);
     } else {
         // Water-surface terms of outflw:45-58. With R = double they are evaluated as
         // the reference writes them (SFCELV = RIVELV + RIVDPH, slopes and depths from
@@ -314,7 +412,7 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
             q_f = q_f * rate;
         }
     }
-    // ---- reservoir operation: damout:316-498 (Yamazaki & Funato rule)
+    // ---- reservoir operation: damout:316-498 (Yamazaki & Funato or Hanazaki-2022 rule)
     if (A.ldamout) {
         const int code = M.i1dam[i];
         if (code > 0) {
@@ -354,7 +452,9 @@ __device__ __forceinline__ void outflow_cell(const DevMap& M, const DevState<R>&
                 const double cv = M.dam_convol[d], av = M.dam_adjvol[d], ev = M.dam_emevol[d];
                 const double qn = M.dam_qn[d], qf = M.dam_qf[d], qa = M.dam_qa[d];
                 double out = 0.0;
-                if (vol <= cv) {
+                if (M.ldamh22) {
+                    out = dam_rule_h22(vol, inf, M.dam_norvol[d], cv, ev, qn, qf, M.dam_r_volupa[d]);
+                } else if (vol <= cv) {
                     out = qn * sqrt(vol / cv);
                 } else if (vol > cv && vol <= av) {
                     const double xx = (vol - cv) / (av - cv);
@@ -861,7 +961,7 @@ __global__ __launch_bounds__(BS) void k_substep(DevMap M, DevState<R> S, StepArg
     int t0, dt, te;
     tile_range(blockIdx.x, gridDim.x, nt_o, A.xcd, t0, dt, te);
     for (int t = t0; t < te; t += dt) {
-        if (A.lkine) outflow_tile<R, BS, MB, true>(M, S, A, t, ntc_o);
+        if (A.lkine || A.lslpmix) outflow_tile<R, BS, MB, true>(M, S, A, t, ntc_o);
         else outflow_tile<R, BS, MB, false>(M, S, A, t, ntc_o);
     }
     grid_barrier(S.bar, S.err);
@@ -1316,7 +1416,7 @@ template <class R, int BS, bool MB>
 static void launch_phases_bs(const DevMap& M, const DevState<R>& S, const StepArgs& A, int phase, hipStream_t st) {
     const int ntc = tiles_cells(M, BS), ntp = tiles_paths(M, A, BS);
     if (phase == 0)
-        if (A.lkine)
+        if (A.lkine || A.lslpmix)
             hipLaunchKernelGGL((k_outflow<R, BS, MB, true>), dim3(pad8(ntc + ntp, A.xcd)), dim3(BS), 0, st, M, S,
                                A, ntc, ntc + ntp);
         else

@@ -48,7 +48,7 @@ END TYPE
 TYPE, BIND(C) :: CMF_OPTIONS_T
   INTEGER(C_INT32_T) :: LADPSTP, LFPLAIN, LFLDOUT, LPTHOUT, LDAMOUT, LLEVEE, PRECISION
   REAL(C_DOUBLE)     :: PGRV, PMANFLD, PDSTMTH, PCADP, PMINSLP
-  INTEGER(C_INT32_T) :: JPRB
+  INTEGER(C_INT32_T) :: JPRB, LDAMH22
 END TYPE
 TYPE, BIND(C) :: CMF_MAP_T
   TYPE(C_PTR)    :: I1NEXT, GRAREA, ELEVTN, NXTDST, RIVLEN, RIVWTH, RIVHGT, RIVMAN
@@ -57,6 +57,7 @@ TYPE, BIND(C) :: CMF_MAP_T
   TYPE(C_PTR)    :: I2MASK, PTH_UPST, PTH_DOWN, PTH_DST, PTH_ELV, PTH_WTH, PTH_MAN
   TYPE(C_PTR)    :: LEVHGT, LEVFRC, BASHGT, LEVDST, LEVBASSTO, LEVTOPSTO, LEVFILSTO
   TYPE(C_PTR)    :: I1DAM, DAMSEQ, DAMSTAT, DAM_CONVOL, DAM_EMEVOL, DAM_ADJVOL, DAM_QA, DAM_QN, DAM_QF
+  TYPE(C_PTR)    :: DAM_NORVOL, DAM_R_VOLUPA
 END TYPE
 TYPE, BIND(C) :: CMF_STATE_T
   TYPE(C_PTR) :: RIVSTO, FLDSTO, LEVSTO, DAMSTO, DAMINF
@@ -115,6 +116,11 @@ INTERFACE
     TYPE(C_PTR), VALUE :: h
     INTEGER(C_INT32_T), VALUE :: lkine
   END FUNCTION
+  INTEGER(C_INT) FUNCTION cmf_hip_set_slopemix(h, lslpmix) BIND(C, NAME='cmf_hip_set_slopemix')
+    IMPORT :: C_INT, C_PTR, C_INT32_T
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT32_T), VALUE :: lslpmix
+  END FUNCTION
   INTEGER(C_INT) FUNCTION cmf_hip_physics_fldstg(h) BIND(C, NAME='cmf_hip_physics_fldstg')
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h
@@ -137,6 +143,9 @@ END INTERFACE
 TYPE(C_PTR), SAVE            :: HMAIN = C_NULL_PTR
 INTEGER(C_INT32_T), PARAMETER :: NGLB = 11, GLBCAP = 4096
 REAL(C_DOUBLE), SAVE, TARGET  :: GLB(NGLB, GLBCAP)
+!! LDAMH22: FldVol*1.E-6/upreal of every dam at its own index (the module's
+!! R_VolUpa holds only one value, at index NDAM: damout:204)
+REAL(KIND=JPRB), ALLOCATABLE, SAVE, TARGET :: RVOLUPA(:)
 
 CONTAINS
 !####################################################################
@@ -217,7 +226,8 @@ USE YOS_CMF_MAP,    ONLY: NSEQALL, NSEQRIV, NPTHOUT, NPTHLEV, I1NEXT, I2MASK, DF
                         & D2RIVSTOMAX, D2RIVELV, D2DWNELV, D2FLDHGT, D2FLDSTOMAX, D2FLDGRD, &
                         & PTH_UPST, PTH_DOWN, PTH_DST, PTH_ELV, PTH_WTH, PTH_MAN
 USE CMF_CTRL_LEVEE_MOD,  ONLY: D2LEVHGT, D2LEVFRC, D2BASHGT, D2LEVDST, D2LEVBASSTO, D2LEVTOPSTO, D2LEVFILSTO
-USE CMF_CTRL_DAMOUT_MOD, ONLY: NDAM, DAMSEQ, DAMSTAT, CONVOL, EMEVOL, ADJVOL, QA, QN, QF, I1DAM, LDAMH22, LDAMYBY
+USE CMF_CTRL_DAMOUT_MOD, ONLY: NDAM, DAMSEQ, DAMSTAT, CONVOL, EMEVOL, ADJVOL, QA, QN, QF, I1DAM, LDAMH22, &
+                        & NORVOL, FLDVOL, UPREAL
 IMPLICIT NONE
 TYPE(C_PTR), INTENT(INOUT) :: H
 LOGICAL, INTENT(IN)        :: LWITHDAM
@@ -227,16 +237,13 @@ TYPE(CMF_MAP_T)     :: M
 CHARACTER(LEN=16)   :: CPREC
 LOGICAL             :: LDAM
 !================================================
-IF( .NOT. LSPAMAT .OR. LSLPMIX .OR. LSTG_ES .OR. LGDWDLY .OR. LROSPLIT .OR. LWEVAP &
+IF( .NOT. LSPAMAT .OR. LSTG_ES .OR. LGDWDLY .OR. LROSPLIT .OR. LWEVAP &
   & .OR. LOUTINS .OR. (LKINE .AND. LSLOPEMOUTH) ) CALL STOPMSG('option not supported by the HIP physics')
 !! the tracer (drv_advance:89-90, :110) and sediment (drv_advance:105) schemes read
 !! P2RIVSTO/P2FLDSTO and the substep averages on the host every step, which this
 !! module downloads only when the host's output or restart is due
 IF( LTRACE .OR. LSEDOUT ) CALL STOPMSG('LTRACE/LSEDOUT are not supported by the HIP physics')
 LDAM = LDAMOUT .AND. LWITHDAM
-IF( LDAM )THEN
-  IF( LDAMH22 .OR. LDAMYBY ) CALL STOPMSG('LDAMH22/LDAMYBY not supported by the HIP physics')
-ENDIF
 D%NSEQALL = NSEQALL;  D%NSEQRIV = NSEQRIV;  D%NLFP = NLFP
 D%NPTHOUT = 0;  D%NPTHLEV = 0;  D%NDAM = 0
 IF( LPTHOUT ) THEN; D%NPTHOUT = NPTHOUT; D%NPTHLEV = NPTHLEV; ENDIF
@@ -252,9 +259,13 @@ IF( TRIM(CPREC)=='fp64' ) O%PRECISION = 0
 CALL GET_ENVIRONMENT_VARIABLE('CMF_HIP_DOWNLOAD', CPREC)
 IF( TRIM(CPREC)=='every' ) LEVERY = .TRUE.
 O%PGRV = PGRV;  O%PMANFLD = PMANFLD;  O%PDSTMTH = PDSTMTH;  O%PCADP = PCADP;  O%PMINSLP = PMINSLP
+O%LDAMH22 = 0
+IF( LDAM ) O%LDAMH22 = L2I(LDAMH22)
 CALL CHECK(cmf_hip_create(D, O, H), 'create')
 !! LKINE: kinematic-wave outflow (physics:66-72)
 IF( LKINE ) CALL CHECK(cmf_hip_set_kinematic(H, 1_C_INT32_T), 'set_kinematic')
+!! LSLPMIX: slope-mixed outflow on the I2MASK the host set up (physics:68-70)
+IF( LSLPMIX ) CALL CHECK(cmf_hip_set_slopemix(H, 1_C_INT32_T), 'set_slopemix')
 
 M%I1NEXT = PTR_I(I1NEXT)
 M%GRAREA = PTR_D2(D2GRAREA);  M%ELEVTN = PTR_D2(D2ELEVTN);  M%NXTDST = PTR_D2(D2NXTDST)
@@ -277,11 +288,18 @@ IF( LLEVEE )THEN
 ENDIF
 M%I1DAM = C_NULL_PTR; M%DAMSEQ = C_NULL_PTR; M%DAMSTAT = C_NULL_PTR; M%DAM_CONVOL = C_NULL_PTR
 M%DAM_EMEVOL = C_NULL_PTR; M%DAM_ADJVOL = C_NULL_PTR; M%DAM_QA = C_NULL_PTR; M%DAM_QN = C_NULL_PTR
-M%DAM_QF = C_NULL_PTR
+M%DAM_QF = C_NULL_PTR;  M%DAM_NORVOL = C_NULL_PTR;  M%DAM_R_VOLUPA = C_NULL_PTR
 IF( LDAM )THEN
   M%I1DAM = PTR_I(I1DAM);  M%DAMSEQ = PTR_I(DAMSEQ);  M%DAMSTAT = PTR_I(DAMSTAT)
   M%DAM_CONVOL = PTR_D1(CONVOL);  M%DAM_EMEVOL = PTR_D1(EMEVOL);  M%DAM_ADJVOL = PTR_D1(ADJVOL)
   M%DAM_QA = PTR_D1(QA);  M%DAM_QN = PTR_D1(QN);  M%DAM_QF = PTR_D1(QF)
+  IF( LDAMH22 )THEN
+    !! damout:204 with the dam's own index; a dam not yet built (LDAMYBY, FldVol
+    !! set to 0 after :204) is never operated, so its ratio is not read
+    IF( .NOT. ALLOCATED(RVOLUPA) ) ALLOCATE(RVOLUPA(NDAM))
+    RVOLUPA(:) = FLDVOL(:) * 1.E-6 / UPREAL(:)
+    M%DAM_NORVOL = PTR_D1(NORVOL);  M%DAM_R_VOLUPA = PTR_D1(RVOLUPA)
+  ENDIF
 ENDIF
 CALL CHECK(cmf_hip_set_map(H, M), 'set_map')
 END SUBROUTINE OPEN_HANDLE

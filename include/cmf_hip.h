@@ -51,8 +51,8 @@ typedef struct cmf_dims {
 
 /* Namelist switches and parameters read by the physics
  * (src/cmf_ctrl_nmlist_mod.F90:77-115, :227-232). LSPAMAT is implied (gather
- * inflow). LKINE and LUPSINF are switched on by their own entry points
- * (cmf_hip_set_kinematic, cmf_hip_set_upsinf); LSTG_ES, LGDWDLY, LROSPLIT,
+ * inflow). LKINE, LSLPMIX and LUPSINF are switched on by their own entry points
+ * (cmf_hip_set_kinematic, cmf_hip_set_slopemix, cmf_hip_set_upsinf); LSTG_ES, LGDWDLY, LROSPLIT,
  * LWEVAP and LOUTINS must be false. */
 typedef struct cmf_options {
     int32_t ladpstp, lfplain, lfldout, lpthout, ldamout, llevee;
@@ -68,6 +68,9 @@ typedef struct cmf_options {
      * factors stay double. The device arithmetic is chosen by `precision`; the
      * float map tables are checked against the FP64 recurrence to 1e-5. */
     int32_t jprb;
+    /* LDAMH22 (src/cmf_ctrl_damout_mod.F90:354-378): the Hanazaki-2022 reservoir
+     * rule instead of Yamazaki & Funato; needs dam_norvol and dam_r_volupa */
+    int32_t ldamh22;
 } cmf_options;
 
 /* Static map (YOS_CMF_MAP, levee and dam module tables). Required: i1next,
@@ -89,6 +92,10 @@ typedef struct cmf_map {
     /* dams (damout:123-308): per dam, 1-based DamSeq, DamStat */
     const int32_t *i1dam, *damseq, *damstat;
     const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
+    /* with ldamh22 (damout:202-205): NorVol, and R_VolUpa = FldVol*1.E-6/upreal
+     * of each dam at its own index (the reference's loop stores every dam's value
+     * at index NDAM, so its other dams read 0); dam_adjvol and dam_qa may be NULL */
+    const double *dam_norvol, *dam_r_volupa;
 } cmf_map;
 
 /* Prognostic and diagnostic arrays (YOS_CMF_PROG, YOS_CMF_DIAG). Any pointer may
@@ -216,6 +223,13 @@ int cmf_hip_update_boundary(cmf_handle h, const double* meansl, const double* se
  * storage / DT. 0 (the default) restores the local inertial scheme. Dams, paths,
  * the limiter of the inflow and the storage update are unchanged. */
 int cmf_hip_set_kinematic(cmf_handle h, int32_t lkine);
+/* LSLPMIX (namelist NRUNVER): 1 selects CMF_CALC_OUTFLW_KINEMIX
+ * (src/cmf_opt_outflw_mod.F90:118-300, selected at src/cmf_ctrl_physics_mod.F90:68-70
+ * when LKINE is off): the kinematic wave of cmf_hip_set_kinematic on the cells
+ * with I2MASK != 0 (the slope mask of SET_SLOPEMIX, src/cmf_ctrl_maps_mod.F90:1005-1040,
+ * passed in cmf_map.i2mask, plus the dam codes), that routine's local inertial
+ * variant elsewhere. LKINE, when also on, takes precedence as in the reference. */
+int cmf_hip_set_slopemix(cmf_handle h, int32_t lslpmix);
 
 /* One output record of variable `var` appended to `file`, as CMF_OUTPUT_WRITE
  * writes it in binary mode (src/cmf_ctrl_output_mod.F90:474-664): float32,

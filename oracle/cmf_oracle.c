@@ -19,6 +19,8 @@ static const double F2O3   = (double)(2.f / 3.f);
 static const double F0P1   = (double)0.1f;
 static const double P7O3   = -7.0 / 3.0;       /* -7._JPRB/3._JPRB */
 static const double P4O3   = -4.0 / 3.0;       /* -4._JPRB/3._JPRB */
+static const double P7O3F  = (double)(-7.f / 3.f);   /* default-kind -7./3. */
+static const double P4O3F  = (double)(-4.f / 3.f);   /* default-kind -4./3. */
 
 enum { G_STOPRE, G_STONXT, G_STONEW, G_RIVINF, G_RIVOUT,
        G_STOPRE2, G_STONEW2, G_RIVSTO, G_FLDSTO, G_FLDARE, G_LEVSTO };
@@ -820,6 +822,111 @@ void cmfo_calc_outflw_kine(cmfo_model* m, double dt)
 }
 
 /* ------------------------------------------------------------------------
+ * CMF_CALC_OUTFLW_KINEMIX  src/cmf_opt_outflw_mod.F90:118-300 (LSLPMIX): cells
+ * with I2MASK = 0 take the local inertial equations, in the variant written
+ * there; the others (I2MASK = 1: steep cells of the slope mask, dam-upstream
+ * cells; 2: dams) the kinematic wave of CMF_CALC_OUTFLW_KINE. */
+static void kinemix_fld(cmfo_model* m, int i, double dt, double dflw_f, double dflw_pre_f, double dslope_f,
+                        double flddph_pre)
+{
+    const double gdt = m->pgrv * dt;
+    double dare_f = m->fldsto[i] * (1.0 / m->rivlen[i]);                    /* :181-182, :258-259 */
+    dare_f = DMAX(dare_f - m->flddph[i] * m->rivwth[i], 0.0);
+    const double t = dflw_f * dflw_pre_f;
+    double dflw_imp_f = sqrt(DMAX(t, 0.0));                                 /* :185, :262 */
+    dflw_imp_f = DMAX(dflw_imp_f, 1.e-6);
+    double dare_pre_f = m->fldsto_pre[i] * (1.0 / m->rivlen[i]);           /* :187-188, :264-265 */
+    dare_pre_f = DMAX(dare_pre_f - flddph_pre * m->rivwth[i], 1.e-6);
+    double dare_imp_f = sqrt(dare_f * dare_pre_f);                          /* :189, :266 */
+    dare_imp_f = DMAX(dare_imp_f, 1.e-6);
+    double q = 0.0;
+    if (dflw_imp_f > F1EM5 && dare_imp_f > F1EM5) {                        /* :191-197, :268-274 */
+        const double dout_pre_f = m->fldout_pre[i];
+        q = (dout_pre_f + (gdt * dare_imp_f) * dslope_f) *
+            (1.0 / (1.0 + (((gdt * (m->pmanfld * m->pmanfld)) * fabs(dout_pre_f)) * pow(dflw_imp_f, P4O3F)) *
+                              (1.0 / dare_imp_f)));
+    }
+    if (q * m->rivout[i] < 0.0) q = 0.0;                                    /* :199, :276 */
+    m->fldout[i] = q;
+}
+
+static void kinemix_riv(cmfo_model* m, int i, double dt, double dflw, double dflw_pre, double dslope)
+{
+    const double gdt = m->pgrv * dt;
+    const double darea = m->rivwth[i] * dflw;                               /* :159, :236 */
+    double dflw_imp = sqrt(dflw * dflw_pre);                                /* :162, :239 */
+    dflw_imp = DMAX(dflw_imp, 1.e-6);
+    if (dflw_imp > F1EM5 && darea > F1EM5) {                                /* :164-172, :241-249 */
+        const double dout_pre = m->rivout_pre[i] * (1.0 / m->rivwth[i]);
+        const double man2 = m->rivman[i] * m->rivman[i];
+        m->rivout[i] = (m->rivwth[i] * (dout_pre + (gdt * dflw_imp) * dslope)) *
+                       (1.0 / (1.0 + ((gdt * man2) * fabs(dout_pre)) * pow(dflw_imp, P7O3F)));
+        m->rivvel[i] = m->rivout[i] * (1.0 / darea);
+    } else {
+        m->rivout[i] = 0.0;
+        m->rivvel[i] = 0.0;
+    }
+}
+
+static void kinemix_kine(cmfo_model* m, int i, double dt, double dslope)
+{
+    const double dvel = ((1.0 / m->rivman[i]) * sqrt(dslope)) * pow(m->rivdph[i], F2O3);   /* :206-207 */
+    const double darea = m->rivwth[i] * m->rivdph[i];
+    m->rivvel[i] = dvel;
+    const double q = darea * dvel;
+    m->rivout[i] = DMIN(q, m->rivsto[i] / dt);
+    if (m->lfldout) {                                                       /* :214-222 */
+        const double dslope_f = DMIN(0.005, dslope);
+        const double dvel_f = ((1.0 / m->pmanfld) * sqrt(dslope_f)) * pow(m->flddph[i], F2O3);
+        double dare_f = m->fldsto[i] * (1.0 / m->rivlen[i]);
+        dare_f = DMAX(dare_f - m->flddph[i] * m->rivwth[i], 0.0);
+        const double qf = dare_f * dvel_f;
+        m->fldout[i] = DMIN(qf, m->fldsto[i] / dt);
+    }
+}
+
+void cmfo_calc_outflw_kinemix(cmfo_model* m, double dt)
+{
+    const int n = m->n, nr = m->nseqriv;
+    double* sp = m->w_sfcelv_pre;
+    double* fdp = m->w_flddph_pre;
+    for (int i = 0; i < n; ++i) {                                           /* :141-146 */
+        m->sfcelv[i] = m->rivelv[i] + m->rivdph[i];
+        sp[i] = m->rivelv[i] + m->rivdph_pre[i];
+        const double d = m->rivdph_pre[i] - m->rivhgt[i];
+        fdp[i] = DMAX(d, 0.0);
+    }
+    for (int i = 0; i < nr; ++i) {                                          /* :149-226 */
+        const int j = m->next[i];
+        if (m->i2mask[i] == 0) {
+            const double dsfcmax = DMAX(m->sfcelv[i], m->sfcelv[j]);
+            const double dsfcmax_pre = DMAX(sp[i], sp[j]);
+            const double dslope = (m->sfcelv[i] - m->sfcelv[j]) * (1.0 / m->nxtdst[i]);
+            const double dslope_f = DMAX(-0.005, DMIN(0.005, dslope));
+            kinemix_riv(m, i, dt, dsfcmax - m->rivelv[i], dsfcmax_pre - m->rivelv[i], dslope);
+            if (m->lfldout)
+                kinemix_fld(m, i, dt, DMAX(dsfcmax - m->elevtn[i], 0.0), dsfcmax_pre - m->elevtn[i], dslope_f,
+                            fdp[i]);
+        } else {
+            double dslope = (m->elevtn[i] - m->elevtn[j]) * (1.0 / m->nxtdst[i]);   /* :203-204 */
+            dslope = DMAX(dslope, m->pminslp);
+            kinemix_kine(m, i, dt, dslope);
+        }
+    }
+    for (int i = nr; i < n; ++i) {                                          /* :229-297 */
+        if (m->i2mask[i] == 0) {
+            const double dslope = (m->sfcelv[i] - m->dwnelv[i]) * (1.0 / m->pdstmth);
+            const double dslope_f = DMAX(-0.005, DMIN(0.005, dslope));
+            kinemix_riv(m, i, dt, m->rivdph[i], m->rivdph_pre[i], dslope);
+            if (m->lfldout)
+                kinemix_fld(m, i, dt, m->sfcelv[i] - m->elevtn[i], sp[i] - m->elevtn[i], dslope_f, fdp[i]);
+        } else {
+            kinemix_kine(m, i, dt, m->pminslp);                            /* :280-281 */
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------
  * CMF_CALC_OUTPRE  src/cmf_opt_outflw_mod.F90:308-421: previous-step flows for a
  * storage-only restart (called after CMF_PHYSICS_FLDSTG, drv_control:273-278):
  * the diffusive-wave (Manning) discharge of the current water surface, signed by
@@ -897,6 +1004,7 @@ int32_t cmfo_physics_advance(cmfo_model* m, double* dt, int32_t nt_cap)
     const int32_t nrun = (nt_cap > 0 && nt_cap < nt) ? nt_cap : nt;
     for (int32_t it = 0; it < nrun; ++it) {                                 /* :64-144 */
         if (m->lkine) cmfo_calc_outflw_kine(m, *dt);                       /* :66-72 */
+        else if (m->lslpmix) cmfo_calc_outflw_kinemix(m, *dt);
         else cmfo_calc_outflw(m, *dt);
         if (!m->lfldout)
             for (int i = 0; i < m->n; ++i) { m->fldout[i] = 0.0; m->fldout_pre[i] = 0.0; }

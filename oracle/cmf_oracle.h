@@ -93,6 +93,8 @@ typedef struct cmfo_model {
      * over the upstream area per dam (damout:202-205, 354-378); 0 = Yamazaki-Funato */
     int32_t ldamh22;
     const double *dam_norvol, *dam_r_volupa;
+    /* LSLPMIX: CMF_CALC_OUTFLW_KINEMIX, kinematic wave where I2MASK != 0 */
+    int32_t lslpmix;
 } cmfo_model;
 
 /* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */
@@ -114,6 +116,7 @@ void cmfo_calc_outpre(cmfo_model* m);
 void cmfo_calc_pthout(cmfo_model* m, double dt);
 void cmfo_levee_opt_pthout(cmfo_model* m, double dt);
 void cmfo_damout_calc(cmfo_model* m, double dt);
+void cmfo_calc_outflw_kinemix(cmfo_model* m, double dt);
 /* the Hanazaki-2022 operation rule of one dam before the limiter (damout:354-378) */
 double cmfo_dam_rule_h22(double vol, double inflow, double norvol, double convol, double emevol, double qn,
                          double qf, double r_volupa);

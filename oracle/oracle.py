@@ -37,7 +37,7 @@ _WORK = "w_sfcelv_pre w_dwnelv_pre w_flddph_pre w_stoout w_rate w_rivinf w_fldin
 class _Model(C.Structure):
     _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK] + [
         ("upsinf", _P), ("lkine", C.c_int32), ("glb_hist", _P), ("glb_cap", C.c_int32),
-        ("ldamh22", C.c_int32), ("dam_norvol", _P), ("dam_r_volupa", _P)]
+        ("ldamh22", C.c_int32), ("dam_norvol", _P), ("dam_r_volupa", _P), ("lslpmix", C.c_int32)]
 
 
 MAP_KEYS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax "
@@ -146,6 +146,7 @@ class OracleModel:
         s.ladpstp, s.lfldout, s.lpthout = int(o.ladpstp), int(o.lfldout), int(o.lpthout)
         s.ldamout, s.llevee = int(o.ldamout), int(o.llevee)
         s.lkine = int(getattr(o, "lkine", False))
+        s.lslpmix = int(getattr(o, "lslpmix", False))
         s.pgrv, s.pmanfld, s.pdstmth, s.pcadp, s.pminslp = o.pgrv, o.pmanfld, o.pdstmth, o.pcadp, o.pminslp
         s.dfrcinc = m.dfrcinc
         self._keep = {}

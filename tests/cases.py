@@ -18,7 +18,14 @@ CASES = {
     "all":    (90, 45, 6, True, True, True, 1.5, 0.2),
     "levwet": (90, 45, 7, False, True, False, 60.0, 0.5),
     "allwet": (90, 45, 8, True, True, True, 60.0, 0.5),
+    "damh22": (120, 60, 9, False, False, True, 4.0, 0.0),
+    "slpmix": (120, 60, 10, True, True, True, 20.0, 0.3),
 }
+# cases run with the Hanazaki-2022 reservoir rule (LDAMH22) and one dam per 25 cells
+H22_CASES = ("damh22",)
+# cases run with LSLPMIX (CMF_CALC_OUTFLW_KINEMIX) and this fraction of the steepest
+# river cells in the slope mask
+SLPMIX_CASES = {"slpmix": 0.3}
 
 
 def make_network(name):
@@ -32,8 +39,11 @@ def make_network(name):
         else:
             synth.add_levees(net, levfrac, seed, hmin=0.5, hmax=3.0, fmin=0.05, fmax=0.5)
     if ldam:
-        synth.add_dams(net, max(3, net.ncells // 100), seed)
-    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam)
+        synth.add_dams(net, max(3, net.ncells // (25 if name in H22_CASES else 100)), seed)
+    if name in SLPMIX_CASES:
+        synth.add_slope_mask(net, SLPMIX_CASES[name])
+    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam, ldamh22=name in H22_CASES,
+                        lslpmix=name in SLPMIX_CASES)
     return net, opts, mean
 
 
@@ -48,6 +58,8 @@ def initial_state(m, net, stage_fn):
     st = stage_fn(st)
     if m.opts.ldamout:
         maps.init_dams(m, net, st)
+    if m.opts.lslpmix:
+        maps.apply_slope_mask(m, net.slpmix_mask)
     return st
 
 
@@ -65,6 +77,8 @@ def prepare(name, ndays):
     st = {k: v.copy() for k, v in om.st.items()}
     if opts.ldamout:
         maps.init_dams(m, net, st)
+    if opts.lslpmix:
+        maps.apply_slope_mask(m, net.slpmix_mask)
     rvec = [m.vec(ro[d]) for d in range(ndays)]
     return m, net, st, rvec
 

@@ -44,13 +44,23 @@ GOLDEN = {
     "c1_1deg":  (360, 180, 2026, 30, 86400.0, False, False, False, 1.5, 0.0),
     "g_kine":   (90, 45, 17, 4, 86400.0, False, False, True, 20.0, 0.0),
     "g_dam_yby": (90, 45, 18, 4, 86400.0, False, False, True, 20.0, 0.0),
+    "g_dam_h22": (90, 45, 19, 4, 86400.0, False, False, True, 4.0, 0.0),
+    "g_slpmix": (90, 45, 20, 4, 86400.0, True, True, True, 20.0, 0.3),
 }
+# cases run with LSLPMIX (CMF_CALC_OUTFLW_KINEMIX, src/cmf_opt_outflw_mod.F90:118-300)
+# and this fraction of the steepest river cells in the slope mask
+SLPMIX_CASES = {"g_slpmix": 0.3}
 # cases whose dams carry construction years (LDAMYBY, damout:216-225): cycled over
 # the dams, against the start year 2001 of the reference inputs: 0 (no year, active),
 # 1990 (active), 2001 (new this year), 2005 (not yet built)
 YBY_YEARS = {"g_dam_yby": (0, 1990, 2001, 2005)}
 # cases run with LKINE = .TRUE. (kinematic-wave outflow, src/cmf_opt_outflw_mod.F90:27-127)
 KINE_CASES = ("g_kine",)
+# cases run with LDAMH22 = .TRUE. (Hanazaki-2022 rule, damout:354-378). Their dams
+# have no flood-control volume (FldVol = 0, so EmeVol = ConVol): the flood-control
+# zone, the only one that reads R_VolUpa, is empty, and the reference's R_VolUpa,
+# written only at index NDAM (damout:204) and otherwise uninitialised, is never read.
+H22_CASES = ("g_dam_h22",)
 FINAL_KEYS = ["rivsto", "fldsto", "rivout", "fldout", "outflw", "rivvel"]
 
 
@@ -72,6 +82,11 @@ def network_for(name):
         synth.add_levees(net, levfrac, seed, hmin=0.5, hmax=3.0, fmin=0.05, fmax=0.5)
     if ldam:
         synth.add_dams(net, max(3, net.ncells // 100), seed, years=YBY_YEARS.get(name))
+        if name in H22_CASES:
+            for r in net.dams:
+                r["fldvol"] = 0.0
+    if name in SLPMIX_CASES:
+        synth.add_slope_mask(net, SLPMIX_CASES[name])
     nrec = min(days, 5) if dt >= 86400 else 4   # records are cycled by the driver
     ro = synth.make_runoff(net, nrec, seed, mean_mm_day=mean)
     return net, ro
@@ -83,6 +98,8 @@ def save_network(path, net, ro):
     if net.levhgt is not None:
         arrays["levhgt"] = net.levhgt
         arrays["levfrc"] = net.levfrc
+    if net.slpmix_mask is not None:
+        arrays["slpmix_mask"] = net.slpmix_mask
     arrays["runoff"] = ro
     meta = dict(nx=net.nx, ny=net.ny, west=net.west, east=net.east, north=net.north, south=net.south,
                 bifprm=net.bifprm, npthlev=net.npthlev, dams=net.dams)
@@ -103,6 +120,8 @@ def load_network(path):
         net.levhgt = z["levhgt"]
         net.levfrc = z["levfrc"]
     net.dams = meta["dams"]
+    if "slpmix_mask" in z:
+        net.slpmix_mask = z["slpmix_mask"]
     return net, z["runoff"]
 
 
@@ -115,7 +134,8 @@ def make(name):
     with tempfile.TemporaryDirectory() as d:
         ifrq = int(dt // 3600)
         synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=ifrq, ndays=days,
-                                     lpthout=lpth, llevee=llev, ldamout=ldam, lkine=name in KINE_CASES)
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, lkine=name in KINE_CASES,
+                                     ldamh22=name in H22_CASES, slpmix_mask=net.slpmix_mask)
         r = subprocess.run([REF, "runoff.bin", "dump.bin", "1"], cwd=d, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
@@ -128,7 +148,8 @@ def make(name):
         keys_map += list(refdump.LEV_D)
     if ldam:
         keys_map += ["i1dam"]
-    out = dict(case=name, spec=GOLDEN[name], lkine=name in KINE_CASES, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
+    out = dict(case=name, spec=GOLDEN[name], lkine=name in KINE_CASES, ldamh22=name in H22_CASES,
+               lslpmix=name in SLPMIX_CASES, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
                map_sha={k: sha(mp[k]) for k in keys_map}, steps=[])
     skeys = refdump.STATE_P + refdump.STATE_D + (["levsto", "levdph"] if llev else []) + \
         (["damsto", "daminf"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])

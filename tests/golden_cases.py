@@ -13,7 +13,8 @@ from cmf import maps
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GDIR = os.path.join(HERE, "golden")
-NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg", "g_kine", "g_dam_yby"]
+NAMES = ["g_plain", "g_pth", "g_levwet", "g_dam", "g_allwet", "g_hourly", "c1_1deg", "g_kine", "g_dam_yby",
+         "g_dam_h22", "g_slpmix"]
 
 
 def sha(a):
@@ -41,7 +42,8 @@ def setup(name):
     from oracle.oracle import OracleModel
     net, ro, ref, fin = load(name)
     nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
-    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)))
+    opts = maps.Options(lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)),
+                        ldamh22=bool(ref.get("ldamh22", False)), lslpmix=bool(ref.get("lslpmix", False)))
     m = maps.build_map(net, opts)
     st = maps.cold_start(m)
     om = OracleModel(m, st)
@@ -49,6 +51,8 @@ def setup(name):
     st = {k: v.copy() for k, v in om.st.items()}
     if ldam:
         maps.init_dams(m, net, st)
+    if opts.lslpmix:
+        maps.apply_slope_mask(m, net.slpmix_mask)
     rvec = [m.vec(ro[r]) for r in range(ro.shape[0])]
     nsteps = int(round(days * 86400.0 / dt))
     spr = max(1, int(round(max(dt, 3600.0 * int(dt // 3600)) / dt)))   # steps per forcing record

@@ -33,7 +33,8 @@ OUTPUT_VARS = ("rivout,fldout,outflw,rivvel,pthout,runoff,maxflw,maxdph,maxsto,r
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["g_allwet", "g_hourly", "c1_1deg", "g_kine"])
+@pytest.mark.parametrize("name", ["g_allwet", "g_hourly", "c1_1deg", "g_kine", "g_dam_yby", "g_dam_h22",
+                                  "g_slpmix"])
 def test_dropin_matches_reference(name, tmp_path):
     if not os.path.exists(EXE):
         pytest.skip("drop-in binary not built")
@@ -44,7 +45,8 @@ def test_dropin_matches_reference(name, tmp_path):
     nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
     d = str(tmp_path)
     synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
-                                 lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)))
+                                 lpthout=lpth, llevee=llev, ldamout=ldam, lkine=bool(ref.get("lkine", False)),
+                                 ldamh22=bool(ref.get("ldamh22", False)), slpmix_mask=net.slpmix_mask)
     # the test driver reads the state after arbitrary steps (its dumps): whole
     # state downloaded after every step
     env = dict(os.environ, CMF_HIP_DOWNLOAD="every")
