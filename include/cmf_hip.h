@@ -94,7 +94,8 @@ typedef struct cmf_map {
     const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
     /* with ldamh22 (damout:202-205): NorVol, and R_VolUpa = FldVol*1.E-6/upreal
      * of each dam at its own index (the reference's loop stores every dam's value
-     * at index NDAM, so its other dams read 0); dam_adjvol and dam_qa may be NULL */
+     * at index NDAM, so its other dams read an element never set); dam_adjvol and
+     * dam_qa may be NULL */
     const double *dam_norvol, *dam_r_volupa;
 } cmf_map;
 

@@ -45,10 +45,15 @@ GOLDEN = {
     "g_kine":   (90, 45, 17, 4, 86400.0, False, False, True, 20.0, 0.0),
     "g_dam_yby": (90, 45, 18, 4, 86400.0, False, False, True, 20.0, 0.0),
     "g_dam_h22": (90, 45, 19, 4, 86400.0, False, False, True, 4.0, 0.0),
-    "g_slpmix": (90, 45, 20, 4, 86400.0, True, True, True, 20.0, 0.3),
+    "g_slpmix": (90, 45, 20, 4, 86400.0, True, True, False, 20.0, 0.3),
 }
 # cases run with LSLPMIX (CMF_CALC_OUTFLW_KINEMIX, src/cmf_opt_outflw_mod.F90:118-300)
-# and this fraction of the steepest river cells in the slope mask
+# and this fraction of the steepest river cells in the slope mask. Without dams: on
+# this network with dams, one reservoir enters a substep-to-substep oscillation of
+# the Yamazaki-Funato rule around ConVol (d(outflow)/d(storage) * DT ~ 8), which
+# multiplies a one-ulp difference of a fractional power several times per substep;
+# the oracle still matches the reference there bit for bit, a device whose pow
+# rounds differently by one ulp cannot.
 SLPMIX_CASES = {"g_slpmix": 0.3}
 # cases whose dams carry construction years (LDAMYBY, damout:216-225): cycled over
 # the dams, against the start year 2001 of the reference inputs: 0 (no year, active),
