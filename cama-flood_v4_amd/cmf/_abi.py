@@ -125,6 +125,7 @@ HIP_FUNCS = {
     "cmf_hip_physics_advance": (I, [H, C.POINTER(C.c_double), C.POINTER(C.c_int32), P, C.c_int32]),
     "cmf_hip_physics_advance_capped": (I, [H, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32),
                                            P, C.c_int32]),
+    "cmf_hip_physics_advance_noalloc": (I, [H, C.POINTER(C.c_double), C.POINTER(C.c_int32), P, C.c_int32]),
     "cmf_hip_synchronize": (I, [H]),
     "cmf_hip_kernel_times": (I, [H, C.c_int32, P]),
 }

@@ -325,6 +325,17 @@ class CMFModel:
             self.h, C.byref(d), nt_cap, C.byref(nt), None, 0))
         return nt.value
 
+    def physics_advance_noalloc(self, dt: float):
+        """CMF_PHYSICS_ADVANCE through the allocation-free entry point (no device
+        allocation during the call; scratch in the library's fixed per-GPU array).
+        Returns (NT, budget sums [NT, 11])."""
+        d = C.c_double(dt)
+        nt = C.c_int32(0)
+        cap = 4096
+        g = np.zeros((cap, NGLB))
+        self._check(self.lib.cmf_hip_physics_advance_noalloc(self.h, C.byref(d), C.byref(nt), g.ctypes.data, cap))
+        return nt.value, g[:min(nt.value, cap)].copy()
+
     def ensemble_advance(self, dt: float, nt_cap: int = 0):
         """One outer step of length dt of every instance. Returns (NT, DT) arrays
         of length ninst (instance k = domain*members + member)."""

@@ -18,6 +18,19 @@
 
 using namespace cmf;
 
+// Scratch of the allocation-free advance (cmf_hip_physics_advance_noalloc), one copy
+// per GPU: the CFL minimum, the CFL bound of every stage block, the budget rows of
+// the substeps and the per-tile budget sums of one substep.
+constexpr size_t NA_CFL_BLOCKS = 65536;          // stage blocks of 256 slots: NSEQALL <= 16,777,216
+constexpr size_t NA_HIST = 4096;                 // substeps per outer step
+constexpr size_t NA_TILES = 65536;               // update tiles of one substep
+constexpr size_t NA_OFF_CFL = 0;                 // 1 value, padded to 256 bytes
+constexpr size_t NA_OFF_PART_CFL = 32;
+constexpr size_t NA_OFF_HIST = NA_OFF_PART_CFL + NA_CFL_BLOCKS;
+constexpr size_t NA_OFF_TILES = NA_OFF_HIST + NA_HIST * NGLB;
+constexpr size_t NA_DOUBLES = NA_OFF_TILES + NA_TILES * NGLB;   // 831,520 doubles = 6.65 MB
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
 namespace {
 thread_local std::string g_err;
 
@@ -833,6 +846,86 @@ int advance_t(cmf_hip_ctx* h, double dt_def, int32_t nt_cap, int32_t* nt_out, do
     return 0;
 }
 
+// ---- allocation-free outer step
+// Scratch of cmf_hip_physics_advance_noalloc, one copy per GPU: the CFL minimum,
+// the CFL bound of every stage block, the budget rows of the substeps and the
+// per-tile budget sums of one substep (reduced into its row after each substep,
+// so one substep's partial sums suffice). Every value is written before it is
+// read within a call, so nothing needs clearing between calls.
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NA_MAX_DEVICES] = {};      // device address of `scratch` on each GPU
+std::string g_scratch_err[NA_MAX_DEVICES];
+
+// once per device, with that device current (called by cmf_hip_create): the
+// device address of this GPU's copy of `scratch`. A failure is kept and reported
+// by the allocation-free entry point only.
+void scratch_setup(int dev) {
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev] || !g_scratch_err[dev].empty()) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) g_scratch_err[dev] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    else g_scratch_dev[dev] = static_cast<double*>(p);
+}
+
+// advance_t for one instance with the scratch pointers redirected into `sc` (this
+// GPU's copy of `scratch`): three phase kernels per substep, the budget of each
+// substep reduced right after it, no graph, no event timing, no allocation
+template <class R>
+int advance_noalloc_t(cmf_hip_ctx* h, double dt_def, int32_t* nt_out, double* glb, int32_t glb_cap, double* sc) {
+    DevState<R>& S0 = state_of<R>(h);
+    DevState<R> S = S0;
+    S.cfl = reinterpret_cast<unsigned long long*>(sc + NA_OFF_CFL);
+    S.cfl_part = sc + NA_OFF_PART_CFL;
+    S.glb_hist = sc + NA_OFF_HIST;
+    S.partials = sc + NA_OFF_TILES;
+    StepArgs A = base_args(h);
+    launch_stage<R>(h->M, S, A, h->o.ladpstp, 1, dt_def, h->stream);   // physics:54-58
+    HIPCHK(hipGetLastError());
+    if (h->o.ladpstp) {
+        HIPCHK(hipMemcpyAsync(h->h_cfl, S.cfl, sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    int32_t ntmax = 0;
+    bool anyfix = false;
+    TRY(plan_step(h, dt_def, 0, &ntmax, &anyfix));
+    if ((size_t)ntmax > NA_HIST)
+        return fail(CMF_EINVAL, "allocation-free advance: NT=%d substeps exceed its maximum of %zu (the stage "
+                                "was recomputed as CMF_PHYSICS_FLDSTG does; the storages are unchanged)",
+                    (int)ntmax, NA_HIST);
+    HIPCHK(hipMemcpyAsync(h->d_inst, h->h_inst, sizeof(InstStep), hipMemcpyHostToDevice, h->stream));
+    const int ntc = tiles_cells(h->M, h->tune.bs_update);
+    const bool vel_first = h->vel_pending;
+    for (int32_t it = 0; it < ntmax; ++it) {
+        A.it = it;
+        A.prow = 0;
+        DevState<R> Sk = S;
+        Sk.rivdph_next = S.rivdph_pre;
+        Sk.rivout_vel = (vel_first && it == 0) ? (const R*)h->vel_buf : S.rivout;
+        launch_phase<R>(h->M, Sk, A, 0, h->tune.bs_outflow, h->stream);
+        launch_phase<R>(h->M, Sk, A, 1, h->tune.bs_rate, h->stream);
+        launch_phase<R>(h->M, Sk, A, 2, h->tune.bs_update, h->stream);
+        launch_budget(S.partials, ntc, 1, S.glb_hist + (size_t)it * NGLB, h->stream);   // CALC_WATBAL sums
+        std::swap(S.rivdph, S.rivdph_pre);
+    }
+    HIPCHK(hipGetLastError());
+    S0.rivdph = S.rivdph;
+    S0.rivdph_pre = S.rivdph_pre;
+    h->vel_pending = false;
+    h->fld_pending = false;
+    h->pth_pending = false;
+    if (glb && glb_cap > 0) {
+        const int32_t m = ntmax < glb_cap ? ntmax : glb_cap;
+        HIPCHK(hipMemcpyAsync(glb, S.glb_hist, (size_t)m * NGLB * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *nt_out = h->last_nt[0];
+    return 0;
+}
+
 // ---- tuner: snapshot / restore of the per-slot state
 int snapshot(cmf_hip_ctx* h, bool restore) {
     size_t total = 0;
@@ -1035,6 +1128,7 @@ int cmf_hip_create_ensemble(const cmf_dims* dims, const cmf_options* opts, const
         cmf_hip_destroy(h);
         return fail(CMF_EHIP, "hipHostMalloc failed");
     }
+    scratch_setup(h->device);                       // the allocation-free advance's scratch on this GPU
     const int rc = (opts->precision == CMF_PREC_FP64) ? alloc_state<double>(h, h->S64) : alloc_state<float>(h, h->S32);
     if (rc != 0) {
         std::string keep = g_err;
@@ -1880,6 +1974,34 @@ int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap, int
 
 int cmf_hip_physics_advance(cmf_handle h, double* dt, int32_t* nt_out, double* glb, int32_t glb_cap) {
     return cmf_hip_physics_advance_capped(h, dt, 0, nt_out, glb, glb_cap);
+}
+
+int cmf_hip_physics_advance_noalloc(cmf_handle h, double* dt, int32_t* nt_out, double* glb, int32_t glb_cap) {
+    if (!h || !dt || !nt_out) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->map_set) return fail(CMF_ESTATE, "cmf_hip_set_map must come first");
+    if (h->ninst != 1) return fail(CMF_EINVAL, "allocation-free advance: single-model handles only");
+    if (!(*dt > 0.0)) return fail(CMF_EINVAL, "DT must be positive");
+    if (h->device < 0 || h->device >= NA_MAX_DEVICES) return fail(CMF_EINVAL, "device %d out of range", h->device);
+    double* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_err[h->device].empty()) return fail(CMF_EHIP, "%s", g_scratch_err[h->device].c_str());
+        sc = g_scratch_dev[h->device];
+    }
+    if (!sc) return fail(CMF_ESTATE, "allocation-free advance: scratch of device %d not set up", h->device);
+    const size_t blocks = ((size_t)h->M.ns + 255) / 256;
+    if (blocks > NA_CFL_BLOCKS)
+        return fail(CMF_EINVAL, "allocation-free advance: NSEQALL=%d exceeds its maximum of %zu cells", h->M.ns,
+                    NA_CFL_BLOCKS * 256);
+    const size_t tiles = (size_t)tiles_cells(h->M, h->tune.bs_update);
+    if (tiles > NA_TILES)
+        return fail(CMF_EINVAL, "allocation-free advance: %zu update tiles (NSEQALL=%d, block %d) exceed its "
+                                "maximum of %zu", tiles, h->M.ns, h->tune.bs_update, NA_TILES);
+    const double dt_def = *dt;
+    const int rc = h->o.precision == CMF_PREC_FP64 ? advance_noalloc_t<double>(h, dt_def, nt_out, glb, glb_cap, sc)
+                                                   : advance_noalloc_t<float>(h, dt_def, nt_out, glb, glb_cap, sc);
+    *dt = dt_def;                                                    // physics:145
+    return rc;
 }
 
 int cmf_hip_ensemble_advance(cmf_handle h, double dt, int32_t nt_cap, int32_t* nt_out, double* dt_out) {

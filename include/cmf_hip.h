@@ -283,6 +283,23 @@ int cmf_hip_physics_fldstg(cmf_handle h);
 int cmf_hip_physics_advance(cmf_handle h, double* dt, int32_t* nt_out,
                             double* glb, int32_t glb_cap);
 
+/* Allocation-free variant of cmf_hip_physics_advance. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * step the model (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). It calls no hipMalloc/hipFree and no allocating
+ * library routine: the budget partial sums, the budget rows and the CFL partial
+ * minima live in a fixed 6.65 MB device array of the library (one copy per GPU,
+ * held until the process ends; its address is looked up when a handle is
+ * created). Same semantics and results as cmf_hip_physics_advance (the
+ * substeps always run as three phase kernels; no graph replay, no event
+ * timing). Limits: single-model handles, NSEQALL <= 16,777,216, at most 65,536
+ * update tiles (NSEQALL / bs_update) and NT <= 4096 (a larger NT returns
+ * CMF_EINVAL after the outer-step stage, which leaves the storages as they
+ * were). Every call on one GPU shares that GPU's array and overwrites what it
+ * needs before reading it: calls that overlap on one GPU (different handles on
+ * different host threads or streams) race and must not be issued. */
+int cmf_hip_physics_advance_noalloc(cmf_handle h, double* dt, int32_t* nt_out, double* glb, int32_t glb_cap);
+
 /* Timing support: run the outer step with at most nt_cap substeps (0 = all).
  * Same as cmf_hip_physics_advance otherwise. No reference counterpart. */
 int cmf_hip_physics_advance_capped(cmf_handle h, double* dt, int32_t nt_cap,
