@@ -554,7 +554,10 @@ def spinup_section(a):
 
 C5_NX, C5_NY = 7200, 3600        # glb_03min-shaped: 0.05-degree global grid
 C5_MAX_LAT = 70.0                # no land poleward of +-70 degrees (cells there are metres wide)
-C5_TRAIN = (2, 100)              # 200 basins: the 100 largest of two 1440x720 (glb_15min) networks
+# training set: 2000 basins of twenty 1440x720 (glb_15min-shaped) networks, labelled by the
+# FP64 device spin-up (tests/golden/make_surrogate_set.py); 200 basins are too few (the
+# network then fits its basins and is worse than the prior on unseen ones)
+C5_TRAIN_SET = os.path.join(ROOT, "tests", "golden", "surrogate_train.npz")
 
 
 def c5_network(seed):
@@ -567,36 +570,37 @@ def c5_network(seed):
 
 
 def spinup_full_section(a):
-    """The learned warm start at its specified size, in phases that exchange files
-    through a.spin_dir (each phase fits one bounded run; a spin-up phase resumes
-    from its saved state when run again):
-      train  labels from a batched FP64 spin-up of 200 basins cut from two
-             glb_15min-shaped 1440x720 networks, training on the device
+    """The learned warm start at its specified size, in phases that exchange small
+    files: each phase writes to a.spin_dir and reads what earlier phases wrote from
+    a.spin_in (default a.spin_dir); a.spinup_full may list several phases, run in
+    order. With --spin-keep-state a spin-up phase also saves its full state
+    (hundreds of MB at this size) and resumes from it when run again.
+      train  training on the device from the stored FP64 equilibria of 2000 basins
+             cut from twenty glb_15min-shaped 1440x720 networks
       cold   FP64 spin-up of the held-out glb_03min-shaped 7200x3600 network
              (~6e6 cells) from the cold start (STORAGE_SEA_SURFACE)
       warm   the same from the physical prior and from the learned warm start
              (two members of one handle)
       report years and days to an annual storage change below 0.1% of every
              start, against the equilibrium of the start that changes least"""
+    phases = a.spinup_full.split(",")
+    if len(phases) > 1:
+        return [spinup_full_section(argparse.Namespace(**{**vars(a), "spinup_full": p_})) for p_ in phases]
     from cmf import surrogate as sg
     d = a.spin_dir
+    src = a.spin_in or d
     os.makedirs(d, exist_ok=True)
     ph = a.spinup_full
     t0 = time.time()
     if ph == "train":
-        nets, per = C5_TRAIN
-        basins = sg.basins_from_networks(nets, per, seed=a.seed + 7, nx=1440, ny=720, max_lat=C5_MAX_LAT)
-        X, Y, res = sg.equilibrium_labels(basins, tol=1e-5, max_years=40, log=lambda s_: log("labels " + s_))
-        t_labels = time.time() - t0
-        Xs, Ys, ws = sg.training_set(X, Y, res.state["rivsto"] + res.state["fldsto"])
+        Xs, Ys, ws, bid = sg.load_training_set(C5_TRAIN_SET)
         t1 = time.time()
         sur, hist = sg.train(Xs, Ys, epochs=a.spin_epochs, batch=1024, lr=2e-3, seed=0, weights=ws,
                              log=lambda s_: log("train " + s_))
         sur.save(os.path.join(d, "surrogate.npz"))
-        info = {"basins": len(basins), "networks": f"{nets} x 1440x720 (glb_15min-shaped, land within +-70 deg)",
-                "cells": int(X.shape[0]), "corrected_cells": int(Xs.shape[0]), "epochs": a.spin_epochs,
-                "label_years_median": int(np.median(res.years)), "label_years_max": int(res.years.max()),
-                "loss_first": hist[0], "loss_last": hist[-1], "label_seconds": round(t_labels, 1),
+        info = {"basins": int(bid.max()) + 1, "networks": "20 x 1440x720 (glb_15min-shaped, land within +-70 deg)",
+                "training_set": os.path.relpath(C5_TRAIN_SET, ROOT), "corrected_cells": int(Xs.shape[0]),
+                "epochs": a.spin_epochs, "loss_first": hist[0], "loss_last": hist[-1],
                 "train_seconds": round(time.time() - t1, 1)}
         with open(os.path.join(d, "train.json"), "w") as f:
             json.dump(info, f)
@@ -604,17 +608,17 @@ def spinup_full_section(a):
     if ph in ("cold", "warm"):
         net, m, clim = c5_network(a.seed)
         log(f"glb_03min-shaped network: {m.n} cells ({time.time() - t0:.1f}s)")
-        state_file = os.path.join(d, f"{ph}_state.npz")
+        state_in = os.path.join(src, f"{ph}_state.npz")
         resume = None
-        if os.path.exists(state_file):
-            z = np.load(state_file, allow_pickle=False)
+        if a.spin_keep_state and os.path.exists(state_in):
+            z = np.load(state_in, allow_pickle=False)
             resume = {k: z[k] for k in z.files}
             starts = None
             log(f"{ph}: resuming after {resume['hist'].shape[0] - 1} years")
         elif ph == "cold":
             starts = {"cold": sg.cold_state(m)}
         else:
-            sur = sg.Surrogate.load(os.path.join(d, "surrogate.npz"))
+            sur = sg.Surrogate.load(os.path.join(src, "surrogate.npz"))
             tp = time.time()
             pr = sg.prior(m, clim)
             t_prior = time.time() - tp
@@ -626,8 +630,11 @@ def spinup_full_section(a):
                 json.dump({"prior_seconds": round(t_prior, 1), "features_and_inference_seconds": round(t_inf, 1)}, f)
         names = ["cold"] if ph == "cold" else ["prior", "warm"]
         r = sg.spinup_from_starts(m, clim, starts, years=a.spin_years, names=names, resume=resume,
-                                  time_budget=a.spin_seconds, log=lambda s_: log(f"{ph} " + s_))
-        np.savez(state_file, **r["resume"])
+                                  time_budget=a.spin_seconds, log=lambda s_: log(f"{ph} " + s_), progress_days=30)
+        if a.spin_keep_state:
+            np.savez(os.path.join(d, f"{ph}_state.npz"), **r["resume"])
+        small = {k: r["resume"][k] for k in ("hist", "daily", "conv", "seconds", "member_substeps", "cells")}
+        np.savez(os.path.join(d, f"{ph}_summary.npz"), **small)
         done = all(r["conv"])
         log(f"{ph}: years run {r['hist'].shape[0] - 1}, converged {r['conv']}, {r['seconds']} s this run")
         return {"phase": ph, "cells": m.n, "years_run": int(r["hist"].shape[0] - 1), "converged": done,
@@ -635,14 +642,15 @@ def spinup_full_section(a):
     if ph == "report":
         runs = []
         for nm in ("cold", "warm"):
-            z = np.load(os.path.join(d, f"{nm}_state.npz"), allow_pickle=False)
+            z = np.load(os.path.join(src, f"{nm}_summary.npz"), allow_pickle=False)
             runs.append(dict(names=["cold"] if nm == "cold" else ["prior", "warm"], hist=z["hist"], daily=z["daily"],
                              conv=[int(c) for c in z["conv"]], seconds=float(z["seconds"]),
                              member_substeps=int(z["member_substeps"]), cells=int(z["cells"])))
         res = sg.summarize_spinups(runs)
-        with open(os.path.join(d, "train.json")) as f:
+        with open(os.path.join(src, "train.json")) as f:
             train = json.load(f)
-        setup = json.load(open(os.path.join(d, "warm_setup.json")))
+        with open(os.path.join(src, "warm_setup.json")) as f:
+            setup = json.load(f)
         c, p_, w = res["cold"], res["prior"], res["warm"]
         ys = lambda v: v["years"] or f">{v['years_run']}"
         member_years = sum((r["hist"].shape[0] - 1) * len(r["names"]) for r in runs)
@@ -761,10 +769,15 @@ def main():
     ap.add_argument("--spinup-only", action="store_true")
     ap.add_argument("--spin-epochs", type=int, default=150)
     ap.add_argument("--spin-years", type=int, default=20)
-    ap.add_argument("--spinup-full", choices=("train", "cold", "warm", "report"),
-                    help="one phase of the glb_03min-sized learned warm start (JSON of the phase on stdout)")
+    ap.add_argument("--spinup-full", default=None,
+                    help="phases of the glb_03min-sized learned warm start, comma-separated, from train, cold, "
+                         "warm, report (JSON of the phases on stdout)")
     ap.add_argument("--spin-dir", default=os.path.join(ROOT, "gpurun_out", "spinup_full"),
-                    help="directory the --spinup-full phases exchange files through")
+                    help="directory the --spinup-full phases write to")
+    ap.add_argument("--spin-in", default=None,
+                    help="directory the --spinup-full phases read earlier phases' files from (default: --spin-dir)")
+    ap.add_argument("--spin-keep-state", action="store_true",
+                    help="save (and resume from) the full spin-up state of the cold/warm phases")
     ap.add_argument("--spin-seconds", type=float, default=900.0,
                     help="a spin-up phase stops after the simulated year that passes this many seconds")
     ap.add_argument("--precision-report", action="store_true",
@@ -779,6 +792,9 @@ def main():
                     help="also time the FP64 path with cells in NSEQ order")
     a = ap.parse_args()
     if a.spinup_full:
+        bad = set(a.spinup_full.split(",")) - {"train", "cold", "warm", "report"}
+        if bad:
+            ap.error(f"unknown --spinup-full phase(s): {sorted(bad)}")
         print(json.dumps({"spinup_full": spinup_full_section(a)}), flush=True)
         return
     if a.spinup_only:

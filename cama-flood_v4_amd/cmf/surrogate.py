@@ -349,6 +349,37 @@ def train(X: np.ndarray, Y: np.ndarray, epochs: int = 40, batch: int = 1024, lr:
     return s, hist
 
 
+def load_training_set(path: str):
+    """(features, targets [n, 2], sampling weights, basin index) of a stored
+    training set (tests/golden/make_surrogate_set.py): features and the storage
+    target as stored, the auxiliary flow target replaced by the storage target (the
+    set does not keep it), weights from the equilibrium storage as in
+    training_set."""
+    z = np.load(path, allow_pickle=False)
+    X = z["X"].astype(np.float32)
+    y0 = z["Y0"].astype(np.float32)
+    s = np.maximum(z["sto"].astype(np.float64), STO_FLOOR)
+    return X, np.stack([y0, y0], axis=1), np.clip(s / s.mean(), 0.05, 50.0), z["bid"].astype(np.int64)
+
+
+def correction_error(pred: np.ndarray, X: np.ndarray, sto_eq: np.ndarray, prior_sto: np.ndarray) -> dict:
+    """How far the prior and the warm start (prior x the predicted correction on
+    the fast cells) lie from the equilibrium on the cells the network corrects:
+    mean |log(start / equilibrium)| weighted by equilibrium storage, and the
+    storage-weighted L1 share of the whole network."""
+    fast = fast_cells(X)
+    eq = np.maximum(sto_eq, STO_FLOOR)
+    truth = np.log(eq / prior_sto)
+    y = pred[:, 0].astype(np.float64)
+    w = eq[fast] / max(eq[fast].sum(), STO_FLOOR)
+    warm = prior_sto * np.exp(np.where(fast, y, 0.0))
+    return {"prior_log_err": float((np.abs(truth[fast]) * w).sum()),
+            "warm_log_err": float((np.abs(truth[fast] - y[fast]) * w).sum()),
+            "prior_l1": float(np.abs(prior_sto - eq).sum() / eq.sum()),
+            "warm_l1": float(np.abs(warm - eq).sum() / eq.sum()),
+            "fast_storage_share": float(eq[fast].sum() / eq.sum())}
+
+
 # ---------------------------------------------------------------------------
 # experiment helpers: synthetic basins, equilibrium labels, cold vs warm spin-up
 
@@ -484,13 +515,14 @@ RESUME_KEYS = ("rivsto", "fldsto", "rivout_pre", "fldout_pre", "rivdph_pre", "fl
 
 def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], years: int = 20, tol: float = 1e-3,
                        log=None, names=None, resume: Optional[dict] = None,
-                       time_budget: Optional[float] = None) -> dict:
+                       time_budget: Optional[float] = None, progress_days: int = 0) -> dict:
     """FP64 spin-up of one network from several starts, run as the members of one
     handle (same map and forcing, own adaptive steps; the member count is padded
     to a power of two by repeating the last start), until every start's annual
     storage change is below tol, `years` pass, or (time_budget) the year that
     passes time_budget seconds ends. The total storage of every member is summed
-    on the device at the end of every day. The result's "resume" dict (the
+    on the device at the end of every day (progress_days > 0: a log line every
+    that many days). The result's "resume" dict (the
     restart state of every member plus the history) continues the run exactly
     when passed back as `resume`."""
     import time
@@ -520,6 +552,8 @@ def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], yea
             nt, _ = model.ensemble_advance(86400.0)
             substeps += int(nt[:len(names)].sum())
             daily.append(model.instance_storage()[:len(names)])
+            if log and progress_days and (d + 1) % progress_days == 0 and d + 1 < DAYS:
+                log(f"year {y} day {d + 1} ({time.time() - t0:.1f}s)")
         now = daily[-1]
         ch = np.abs(now - hist[-1]) / now
         hist.append(now)

@@ -1,0 +1,45 @@
+"""What the trained network adds to the physical prior it corrects.
+
+The network is trained on the device (the project's BF16 MFMA forward and
+backward kernels, Adam) on tests/golden/surrogate_train.npz: 2000 river basins of
+twenty glb_15min-shaped 1440x720 networks, each with its own runoff climatology,
+labelled by the FP64 device spin-up (tests/golden/make_surrogate_set.py). It is
+applied to the 200 basins of two other 1440x720 networks (climatologies drawn
+the same way, none of them seen in training), whose equilibria the FP64 device
+spin-up computes here from the prior to an annual storage change below 1e-5.
+
+Checked on the cells the network corrects (memory < 60 days): the warm start's
+storage-weighted mean |log(start / equilibrium)| is at least 5% below the prior's,
+and its storage-weighted L1 distance over the whole networks is no larger.
+Measured (float32 training of the same network on the CPU, same sets): 0.0133
+against 0.0153. With only 200 training basins the network fits its basins
+(training error 0.0016) and is worse than the prior on unseen ones (0.0145
+against 0.0114): the residual it learns varies from basin to basin with the
+climatology, so it needs many basins, not many cells."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_network_improves_on_the_prior():
+    from cmf import surrogate as sg
+    X, Y, w, bid = sg.load_training_set(os.path.join(HERE, "golden", "surrogate_train.npz"))
+    print(f"training set: {X.shape[0]} cells of {int(bid.max()) + 1} basins", flush=True)
+    sur, hist = sg.train(X, Y, epochs=40, batch=1024, lr=2e-3, seed=0, weights=w)
+    print(f"loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
+    basins = sg.basins_from_networks(2, 100, seed=14, nx=1440, ny=720, max_lat=70.0)
+    Xh, Yh, res = sg.equilibrium_labels(basins, tol=1e-5, max_years=40,
+                                        log=lambda s: print("labels:", s, flush=True))
+    eq = res.state["rivsto"] + res.state["fldsto"]
+    prior = np.concatenate([sg.prior(b.m, b.clim).storage for b in basins])
+    r = sg.correction_error(sur.predict(Xh), Xh, eq, prior)
+    print(f"held-out: {Xh.shape[0]} cells; weighted |log| error prior {r['prior_log_err']:.5f} "
+          f"warm {r['warm_log_err']:.5f}; L1 prior {r['prior_l1']:.5f} warm {r['warm_l1']:.5f}; "
+          f"corrected cells hold {r['fast_storage_share']:.2%} of the storage", flush=True)
+    assert r["warm_log_err"] <= 0.95 * r["prior_log_err"]
+    assert r["warm_l1"] <= r["prior_l1"]
