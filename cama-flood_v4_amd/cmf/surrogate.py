@@ -530,7 +530,8 @@ def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], yea
     from .spinup import DAYS
     names = list(names if names is not None else starts)
     M = 1 << max(len(names) - 1, 0).bit_length()
-    model = CMFModel(m, "fp64", members=M, dom_cells=[m.n])
+    single = M == 1                     # one start: the single-model kernels
+    model = CMFModel(m, "fp64") if single else CMFModel(m, "fp64", members=M, dom_cells=[m.n])
     if resume is None:
         sts = [starts[nm] for nm in names] + [starts[names[-1]]] * (M - len(names))
         model.set_state({k: np.stack([st[k] for st in sts]) for k in ("rivsto", "fldsto", "rivdph_pre")})
@@ -548,9 +549,13 @@ def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], yea
     y0 = len(hist) - 1
     for y in range(y0 + 1, years + 1):
         for d in range(DAYS):
-            model.set_runoff_scaled(clim.daily(d), one)
-            nt, _ = model.ensemble_advance(86400.0)
-            substeps += int(nt[:len(names)].sum())
+            if single:
+                model.set_runoff(clim.daily(d))
+                substeps += int(model.physics_advance(86400.0))
+            else:
+                model.set_runoff_scaled(clim.daily(d), one)
+                nt, _ = model.ensemble_advance(86400.0)
+                substeps += int(nt[:len(names)].sum())
             daily.append(model.instance_storage()[:len(names)])
             if log and progress_days and (d + 1) % progress_days == 0 and d + 1 < DAYS:
                 log(f"year {y} day {d + 1} ({time.time() - t0:.1f}s)")
@@ -566,7 +571,7 @@ def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], yea
         if all(conv) or (time_budget is not None and time.time() - t0 > time_budget):
             break
     seconds = spent + (time.time() - t0)
-    st = model.get_state(list(RESUME_KEYS))
+    st = {k: np.reshape(v, (M, -1)) for k, v in model.get_state(list(RESUME_KEYS)).items()}
     model.close()
     hist_a, daily_a = np.asarray(hist), np.asarray(daily)
     res = dict(names=names, hist=hist_a, daily=daily_a, conv=conv, seconds=round(seconds, 1),
