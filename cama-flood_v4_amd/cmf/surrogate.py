@@ -457,15 +457,21 @@ def cold_state(m: CMFMap, precision: str = "fp64") -> dict:
 
 
 def warm_state(m: CMFMap, rivsto: np.ndarray, fldsto: np.ndarray, precision: str = "fp64") -> dict:
-    """Predicted storages, repartitioned by CMF_PHYSICS_FLDSTG on the device."""
+    """Predicted storages entered as the reference enters a storage-only restart
+    (LRESTART + LSTOONLY, src/cmf_drv_control_mod.F90:273-278): flows zero,
+    D2FLDSTO_PRE = P2FLDSTO (src/cmf_ctrl_restart_mod.F90:128-131), then on the
+    device CMF_PHYSICS_FLDSTG and CMF_CALC_OUTPRE, which gives the previous-step
+    flows (RIVOUT_PRE, FLDOUT_PRE) and RIVDPH_PRE of the predicted surface."""
     from .model import CMFModel
     st = maps.cold_start(m)
     st["rivsto"], st["fldsto"] = rivsto.copy(), fldsto.copy()
     g = CMFModel(m, precision)
     g.set_state(st)
     g.physics_fldstg()
-    st.update(g.get_state(["rivsto", "fldsto", "rivdph"]))
-    st["rivdph_pre"] = st["rivdph"].copy()
+    g.calc_outpre()
+    st.update(g.get_state(["rivsto", "fldsto", "rivdph", "rivout", "fldout", "rivout_pre", "fldout_pre",
+                           "rivdph_pre"]))
+    st["fldsto_pre"] = st["fldsto"].copy()
     g.close()
     return st
 
@@ -496,7 +502,7 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None,
         sts = [cold_state(b.m) for b in basins]
     else:
         raise ValueError(f"start must be 'prior' or 'cold', not {start!r}")
-    for k in ("rivsto", "fldsto", "rivdph_pre"):
+    for k in START_KEYS:
         init[k] = np.concatenate([s[k] for s in sts])
     model = CMFModel(cm, "fp64", members=1, dom_cells=dom_cells)
     model.set_state(init)
@@ -511,6 +517,9 @@ def equilibrium_labels(basins, tol: float = 1e-6, max_years: int = 10, log=None,
 
 
 RESUME_KEYS = ("rivsto", "fldsto", "rivout_pre", "fldout_pre", "rivdph_pre", "fldsto_pre", "rivout")
+# what a start hands to the spin-up: storages, the previous-step flows and stage,
+# and the current flows (zero for every start, as after a restart)
+START_KEYS = ("rivsto", "fldsto", "rivdph_pre", "rivout", "fldout", "rivout_pre", "fldout_pre", "fldsto_pre")
 
 
 def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], years: int = 20, tol: float = 1e-3,
@@ -534,7 +543,7 @@ def spinup_from_starts(m: CMFMap, clim: Climatology, starts: Optional[dict], yea
     model = CMFModel(m, "fp64") if single else CMFModel(m, "fp64", members=M, dom_cells=[m.n])
     if resume is None:
         sts = [starts[nm] for nm in names] + [starts[names[-1]]] * (M - len(names))
-        model.set_state({k: np.stack([st[k] for st in sts]) for k in ("rivsto", "fldsto", "rivdph_pre")})
+        model.set_state({k: np.stack([st[k] for st in sts]) for k in START_KEYS})
         model.physics_fldstg()
         hist, daily = [model.instance_storage()[:len(names)]], []
         conv = [0] * len(names)
