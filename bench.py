@@ -666,12 +666,24 @@ def spinup_full_section(a):
         net, m, clim = c5_network(a.seed)
         log(f"glb_03min-shaped network: {m.n} cells ({time.time() - t0:.1f}s)")
         state_in = os.path.join(src, f"{ph}_state.npz")
+        carry_in = os.path.join(src, f"{ph}_carry.npz")
         resume = None
         if a.spin_keep_state and os.path.exists(state_in):
             z = np.load(state_in, allow_pickle=False)
             resume = {k: z[k] for k in z.files}
             starts = None
             log(f"{ph}: resuming after {resume['hist'].shape[0] - 1} years")
+        elif a.spin_carry and os.path.exists(carry_in):
+            # continue from the float32 storages of an earlier run, entered as a
+            # storage-only restart (stage, then CMF_CALC_OUTPRE flows): the float32
+            # rounding (6e-8) is far below the 0.1% annual-change criterion
+            z = np.load(carry_in, allow_pickle=False)
+            sts = [sg.warm_state(m, z["rivsto"][k].astype(np.float64), z["fldsto"][k].astype(np.float64))
+                   for k in range(z["rivsto"].shape[0])]
+            resume = {k: np.stack([s_[k] for s_ in sts]) for k in sg.RESUME_KEYS}
+            resume.update({k: z[k] for k in ("hist", "daily", "conv", "seconds", "member_substeps", "cells")})
+            starts = None
+            log(f"{ph}: continuing after {resume['hist'].shape[0] - 1} years from float32 storages")
         elif ph == "cold":
             starts = {"cold": sg.cold_state(m)}
         else:
@@ -693,6 +705,10 @@ def spinup_full_section(a):
         small = {k: r["resume"][k] for k in ("hist", "daily", "conv", "seconds", "member_substeps", "cells")}
         np.savez(os.path.join(d, f"{ph}_summary.npz"), **small)
         done = all(r["conv"])
+        if a.spin_carry and not done:
+            np.savez_compressed(os.path.join(d, f"{ph}_carry.npz"), **small,
+                                rivsto=r["resume"]["rivsto"].astype(np.float32),
+                                fldsto=r["resume"]["fldsto"].astype(np.float32))
         log(f"{ph}: years run {r['hist'].shape[0] - 1}, converged {r['conv']}, {r['seconds']} s this run")
         return {"phase": ph, "cells": m.n, "years_run": int(r["hist"].shape[0] - 1), "converged": done,
                 "conv": r["conv"], "seconds": r["seconds"]}
@@ -845,6 +861,9 @@ def main():
                     help="directory the --spinup-full phases read earlier phases' files from (default: --spin-dir)")
     ap.add_argument("--spin-keep-state", action="store_true",
                     help="save (and resume from) the full spin-up state of the cold/warm phases")
+    ap.add_argument("--spin-carry", action="store_true",
+                    help="an unfinished cold/warm phase saves its storages in float32 (<phase>_carry.npz); when "
+                         "--spin-in holds one, the phase continues from it as a storage-only restart")
     ap.add_argument("--spin-seconds", type=float, default=900.0,
                     help="a spin-up phase stops after the simulated year that passes this many seconds")
     ap.add_argument("--mixed-drift", type=int, default=0, metavar="DAYS",

@@ -28,6 +28,10 @@ MODULE CMF_CTRL_PHYSICS_MOD
 !  under the FP64 host and to the mixed-precision path (FP32 flux and stage
 !  arithmetic, FP64 storage) under the single-precision host;
 !  CMF_HIP_PRECISION=fp64 or =mixed in the environment chooses it explicitly.
+!  Options carried to the device: LKINE, LSLPMIX (with the host's I2MASK),
+!  LDAMOUT with LDAMH22 (each dam's own FldVol*1.E-6/upreal) and LDAMYBY, LLEVEE,
+!  LPTHOUT, LSEALEV, LUPSINF. The module stops with a message under LTRACE,
+!  LSEDOUT and the options the device path does not carry (OPEN_HANDLE).
 !==========================================================
 USE ISO_C_BINDING
 USE PARKIND1,              ONLY: JPIM, JPRB, JPRD
