@@ -1000,6 +1000,9 @@ def main():
         "ensemble": ens,
         "forcing_interpolation": forcing,
         "spinup": spin,
+        # the glb_03min-sized spin-up (6e6 cells, ~90 GPU-minutes) is not re-run here;
+        # its result from the --spinup-full phases is kept in this file
+        "spinup_glb03min_profile": "profiles/spinup_glb03min_7200x3600.json",
         "dropin": dropin,
     }
     print(json.dumps(out), flush=True)
