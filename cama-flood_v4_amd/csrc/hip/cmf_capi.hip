@@ -75,6 +75,8 @@ struct cmf_hip_ctx {
     void* pth_buf = nullptr;          // D1PTHFLW when it differs from D1PTHFLW_PRE (allocated when needed)
     bool pth_pending = false;
     double* h_ctl = nullptr;          // pinned scratch for small device->host reads
+    int32_t* d_flag = nullptr;        // device flag of k_daminf_nonzero
+    int32_t daminf_sparse = 1;        // DAMINF == 0 in every cell with I1DAM == 0
     // ensemble: ndom domains concatenated in NSEQ, nmem = 2^lgm members each
     int ndom = 1, nmem = 1, lgm = 0, ninst = 1;
     std::vector<int32_t> dom_off;     // NSEQ offset of each domain (+ total)
@@ -487,6 +489,26 @@ DevState<double>& state_of<double>(cmf_hip_ctx* h) { return h->S64; }
 template <>
 DevState<float>& state_of<float>(cmf_hip_ctx* h) { return h->S32; }
 
+void drop_graphs(cmf_hip_ctx* h);
+
+// After an upload of DAMINF: whether every cell with I1DAM == 0 holds zero (the
+// reference's initial value, CMF_PROG_INIT), which lets the update skip those
+// cells' DAMINF and its average
+int check_daminf(cmf_hip_ctx* h, const double* daminf) {
+    if (!h->d_flag) TRY(dalloc(h, &h->d_flag, 1));
+    HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int32_t), h->stream));
+    launch_daminf_nonzero(h->M.dam_of_cell, daminf, h->M.ns, h->M.lgm, h->d_flag, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->h_ctl, h->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int32_t sparse = *reinterpret_cast<int32_t*>(h->h_ctl) == 0;
+    if (sparse != h->daminf_sparse) {
+        h->daminf_sparse = sparse;
+        drop_graphs(h);                                  // captured graphs hold the old arguments
+    }
+    return 0;
+}
+
 template <class R>
 int set_state_t(cmf_hip_ctx* h, const cmf_state* st, bool shared = false) {
     DevState<R>& S = state_of<R>(h);
@@ -496,6 +518,7 @@ int set_state_t(cmf_hip_ctx* h, const cmf_state* st, bool shared = false) {
     TRY(putd(S.fldsto, st->fldsto));
     if (h->o.llevee) TRY(putd(S.levsto, st->levsto));
     if (h->o.ldamout) { TRY(putd(S.damsto, st->damsto)); TRY(putd(S.daminf, st->daminf)); }
+    if (h->o.ldamout && st->daminf) TRY(check_daminf(h, S.daminf));
     // D2RIVOUT_PRE lives in the RIVOUT array; a different D2RIVOUT is kept for the
     // RIVVEL of the next substep only (outflw:75 reads the pre-update RIVOUT).
     const double* rpre = st->rivout_pre ? st->rivout_pre : st->rivout;
@@ -610,6 +633,7 @@ StepArgs base_args(const cmf_hip_ctx* h) {
     A.lupsinf = h->upsinf_on;
     A.lkine = h->lkine;
     A.lslpmix = h->lslpmix && !h->lkine;                 // physics:66-72: LKINE first
+    A.daminf_sparse = h->daminf_sparse;
     return A;
 }
 
@@ -1374,6 +1398,9 @@ int cmf_hip_set_map(cmf_handle h, const cmf_map* mp_in) {
             doc[(size_t)IP[(size_t)s]] = d;
         }
         std::vector<int32_t> i1d = to_dev(h, mp->i1dam, (size_t)n);
+        // the other cells whose DAMINF the dam module sets (I1DAM > 0, damout:444-465)
+        for (int p = 0; p < n; ++p)
+            if (i1d[(size_t)p] > 0 && doc[(size_t)p] == -1) doc[(size_t)p] = -2;
         TRY(put_raw(h, &p32, i1d.data(), (size_t)n)); M.i1dam = p32;
         TRY(put_raw(h, &p32, doc.data(), (size_t)n)); M.dam_of_cell = p32;
         TRY(put_raw(h, &p32, mp->damstat, (size_t)nd)); M.damstat = p32;

@@ -85,7 +85,7 @@ struct DevMap {
     const double* lev_tab;                     // [7][nlevcell]
     // dams
     const int32_t* i1dam;                      // per cell (0,1,10,11)
-    const int32_t* dam_of_cell;                // per cell: dam row or -1
+    const int32_t* dam_of_cell;                // per cell: dam row; -2 other cells with I1DAM > 0; -1
     const int32_t* damstat;
     const double *dam_convol, *dam_emevol, *dam_adjvol, *dam_qa, *dam_qn, *dam_qf;
     // Hanazaki-2022 rule (LDAMH22): NorVol and R_VolUpa of every dam
@@ -155,6 +155,7 @@ struct StepArgs {
     int32_t lupsinf;                           // add S.upsinf to the river in STONXT
     int32_t lkine;                             // kinematic-wave outflow (LKINE)
     int32_t lslpmix;                           // slope-mixed outflow (LSLPMIX, without LKINE)
+    int32_t daminf_sparse;                     // DAMINF is zero in every cell with I1DAM == 0
 };
 
 // Cell / path slot -> static index, member and instance. MB = false is the
@@ -299,9 +300,8 @@ __device__ __forceinline__ void stage_def(const DevMap& M, int i, const Geom& ge
 struct Lev {
     double levhgt, levfrc, bashgt, levdst, levbassto, levtopsto, levfilsto;
 };
-__device__ __forceinline__ Lev load_lev(const DevMap& M, int i) {
+__device__ __forceinline__ Lev load_lev_row(const DevMap& M, int li) {
     Lev L{0.0, 1.0, K_LEVSENT, K_LEVSENT, K_LEVSENT, K_LEVSENT, K_LEVSENT};
-    const int li = M.lev_idx[i];
     const int nlc = M.nlevcell;
     if (li >= 0) {
         L.levhgt = M.lev_tab[li]; L.levfrc = M.lev_tab[nlc + li]; L.bashgt = M.lev_tab[2 * nlc + li];
@@ -310,6 +310,7 @@ __device__ __forceinline__ Lev load_lev(const DevMap& M, int i) {
     }
     return L;
 }
+__device__ __forceinline__ Lev load_lev(const DevMap& M, int i) { return load_lev_row(M, M.lev_idx[i]); }
 
 // CMF_LEVEE_FLDSTG per cell, src/cmf_ctrl_levee_mod.F90:302-473. Every case
 // computes the same eight local values and the result is written once, so the

@@ -649,11 +649,16 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const R upsinf = A.lupsinf ? S.upsinf[s] : R(0);
     const double rs0 = S.rivsto[s], fs0 = S.fldsto[s];
     const R fldfrc_old = S.fldfrc[s];
-    const double levsto = A.llevee ? S.levsto[s] : 0.0;
     const Geom geo = load_geom(M, i);
     const float f_elevtn = M.elevtn[i];
+    // LEVSTO and LEVDPH of a cell without a levee are zero after every stage
+    // inversion (the one at the outer-step start included: it moves any LEVSTO of
+    // such a cell into the river and floodplain), so only levee cells read and
+    // write them here
+    const int li = A.llevee ? M.lev_idx[i] : -1;
+    const double levsto = li >= 0 ? S.levsto[s] : 0.0;
     Lev lev{};
-    if (A.llevee) lev = load_lev(M, i);
+    if (A.llevee) lev = load_lev_row(M, li);
     const int dam = A.ldamout ? M.dam_of_cell[i] : -1;
     // the mixed path forms OUTFLW's average from the RIVOUT and FLDOUT averages on
     // the last substep instead of accumulating it (it differs by FP32 rounding)
@@ -661,8 +666,12 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     // The running averages and maxima are read where they are used, not up front:
     // fewer values live across the stage inversion, 5 waves per SIMD instead of 4
     // for the FP64 update (measured 5% faster than hoisting them).
+    // DAMINF changes only in cells with I1DAM > 0 (damout:442-465); elsewhere it
+    // keeps its initial value, zero unless the host set another (daminf_sparse
+    // says it did not), so such cells neither read it nor accumulate its average
+    const bool dam_inf = A.ldamout && (dam != -1 || !A.daminf_sparse);
     double daminf = 0.0, damsto = 0.0;
-    if (A.ldamout) daminf = S.daminf[s];
+    if (dam_inf) daminf = S.daminf[s];
     if (!I.active) return;
     const double dt = I.dt;
     const bool f0 = I.first;
@@ -761,8 +770,10 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     }
     double stg = o.rivsto + o.fldsto;
     if (A.llevee) {
-        S.levsto[s] = o.levsto;
-        S.levdph[s] = R(o.levdph);
+        if (li >= 0) {
+            S.levsto[s] = o.levsto;
+            S.levdph[s] = R(o.levdph);
+        }
         stg = stg + o.levsto;
     }
     const R storge = R(stg);
@@ -778,7 +789,7 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     const R p_oavg = kOutSum ? zero : S.outflw_avg[s];
     const R p_omax = S.outflw_max[s], p_dmax = S.rivdph_max[s], p_smax = S.storge_max[s];
     const R p_pavg = A.lpthout ? S.pthout_avg[s] : zero;
-    const R p_davg = A.ldamout ? S.daminf_avg[s] : zero;
+    const R p_davg = dam_inf ? S.daminf_avg[s] : zero;
     const R rdt = R(dt);
     R a1 = (f0 ? zero : p_ravg) + qrl * rdt;
     R a2 = (f0 ? zero : p_favg) + qfl * rdt;
@@ -804,10 +815,12 @@ __device__ __forceinline__ void update_cell(const DevMap& M, const DevState<R>& 
     S.outflw_max[s] = vmax(f0 ? zero : p_omax, vabs(outflw));
     S.rivdph_max[s] = vmax(f0 ? zero : p_dmax, rivdph);
     S.storge_max[s] = vmax(f0 ? zero : p_smax, storge);
-    if (A.ldamout) {
+    if (dam_inf) {
         R da = (f0 ? zero : p_davg) + R(daminf * dt);
         if (I.last) da = da / R(I.nadd);
         S.daminf_avg[s] = da;
+    } else if (A.ldamout && f0) {
+        S.daminf_avg[s] = zero;                                     // 0 + 0*DT, unchanged after
     }
 }
 
@@ -1545,6 +1558,19 @@ void launch_out_reset(const OutState& O, int ns, int npth_slots, hipStream_t st)
 void launch_boundary(int n, const float* elevtn, const double* meansl, const double* sealev, double* dwnelv,
                      hipStream_t st) {
     hipLaunchKernelGGL(k_boundary, dim3(cdiv(n, 256)), dim3(256), 0, st, n, elevtn, meansl, sealev, dwnelv);
+}
+
+// 1 in *flag when a cell with I1DAM == 0 (dam_of_cell == -1) holds a nonzero
+// DAMINF (slots member-minor, cell = slot >> lgm)
+__global__ void k_daminf_nonzero(const int32_t* dam_of_cell, const double* daminf, int ns, int lgm, int32_t* flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns && dam_of_cell[s >> lgm] == -1 && daminf[s] != 0.0) *flag = 1;
+}
+
+void launch_daminf_nonzero(const int32_t* dam_of_cell, const double* daminf, int ns, int lgm, int32_t* flag,
+                           hipStream_t st) {
+    if (ns > 0) hipLaunchKernelGGL(k_daminf_nonzero, dim3(cdiv(ns, 256)), dim3(256), 0, st, dam_of_cell, daminf, ns,
+                                   lgm, flag);
 }
 
 void launch_inst_storage(const double* rivsto, const double* fldsto, const double* levsto, const int32_t* dom_off,

@@ -52,6 +52,8 @@ void launch_boundary(int n, const float* elevtn, const double* meansl, const dou
                      hipStream_t st);
 // total storage of every instance into out[ninst] (part: ninst * nch doubles;
 // dom_off: first cell of each domain in device order, and the end)
+void launch_daminf_nonzero(const int32_t* dam_of_cell, const double* daminf, int ns, int lgm, int32_t* flag,
+                           hipStream_t st);
 void launch_inst_storage(const double* rivsto, const double* fldsto, const double* levsto, const int32_t* dom_off,
                          int lgm, int ninst, int nch, double* part, double* out, hipStream_t st);
 // reduce the per-tile budget sums of nrun substeps into glb_hist[nrun][NGLB]

@@ -745,6 +745,8 @@ void cmfo_diag_reset_adpstp(cmfo_model* m)
         for (size_t k = 0; k < (size_t)m->npth * m->npthlev; ++k) m->pthflw_avg[k] = 0.0;
         for (int p = 0; p < m->npth; ++p) m->pthflwsum_avg[p] = 0.0;
     }
+    if (m->ldamout && m->daminf_avg)
+        for (int i = 0; i < m->n; ++i) m->daminf_avg[i] = 0.0;
 }
 
 void cmfo_diag_avemax_adpstp(cmfo_model* m, double dt)
@@ -764,6 +766,8 @@ void cmfo_diag_avemax_adpstp(cmfo_model* m, double dt)
     if (m->lpthout)
         for (size_t k = 0; k < (size_t)m->npth * m->npthlev; ++k)
             m->pthflw_avg[k] = m->pthflw_avg[k] + m->pthflw[k] * dt;
+    if (m->ldamout && m->daminf_avg)                   /* diag:130-135, every cell */
+        for (int i = 0; i < m->n; ++i) m->daminf_avg[i] = m->daminf_avg[i] + m->daminf[i] * dt;
 }
 
 void cmfo_diag_getave_adpstp(cmfo_model* m)
@@ -777,6 +781,8 @@ void cmfo_diag_getave_adpstp(cmfo_model* m)
         m->pthout_avg[i] = m->pthout_avg[i] / a;
         m->runoff_avg[i] = m->runoff_avg[i] / a;
     }
+    if (m->ldamout && m->daminf_avg)                   /* diag:188-194 */
+        for (int i = 0; i < m->n; ++i) m->daminf_avg[i] = m->daminf_avg[i] / a;
     if (m->lpthout) {
         const int np = m->npth;
         for (int l = 0; l < m->npthlev; ++l)

@@ -95,6 +95,9 @@ typedef struct cmfo_model {
     const double *dam_norvol, *dam_r_volupa;
     /* LSLPMIX: CMF_CALC_OUTFLW_KINEMIX, kinematic wave where I2MASK != 0 */
     int32_t lslpmix;
+    /* D2DAMINF_aAVG of CMF_DIAG_*_ADPSTP (diag:130-135, :188-194) with LDAMOUT;
+     * NULL = not kept */
+    double* daminf_avg;
 } cmfo_model;
 
 /* CMF_PHYSICS_FLDSTG (src/cmf_ctrl_physics_mod.F90:288-305) */

@@ -37,7 +37,8 @@ _WORK = "w_sfcelv_pre w_dwnelv_pre w_flddph_pre w_stoout w_rate w_rivinf w_fldin
 class _Model(C.Structure):
     _fields_ = _FIELDS + [(p, _P) for p in _PTRS] + _TAIL + [(w, _P) for w in _WORK] + [
         ("upsinf", _P), ("lkine", C.c_int32), ("glb_hist", _P), ("glb_cap", C.c_int32),
-        ("ldamh22", C.c_int32), ("dam_norvol", _P), ("dam_r_volupa", _P), ("lslpmix", C.c_int32)]
+        ("ldamh22", C.c_int32), ("dam_norvol", _P), ("dam_r_volupa", _P), ("lslpmix", C.c_int32),
+        ("daminf_avg", _P)]
 
 
 MAP_KEYS = ("next up_ptr up_idx grarea elevtn nxtdst rivlen rivwth rivhgt rivman rivstomax "
@@ -175,6 +176,7 @@ class OracleModel:
         if o.ldamout:
             self.st["damsto"] = np.array(state["damsto"], np.float64)
             self.st["daminf"] = np.array(state.get("daminf", np.zeros(n)), np.float64)
+            self.st["daminf_avg"] = np.array(state.get("daminf_avg", np.zeros(n)), np.float64)
         if o.lpthout:
             shp = (m.npthlev, m.npth)
             for k in ("pthflw", "pthflw_pre", "pthflw_avg"):

@@ -76,6 +76,7 @@ def read_dump(path: str, lpthout=False, llevee=False, ldamout=False):
         if ldamout:
             s["damsto"] = c.take(np.float64, n)
             s["daminf"] = c.take(np.float64, n)
+            s["daminf_avg"] = c.take(np.float64, n)
         states.append(s)
     return mp, states
 

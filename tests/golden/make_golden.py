@@ -69,6 +69,11 @@ H22_CASES = ("g_dam_h22",)
 FINAL_KEYS = ["rivsto", "fldsto", "rivout", "fldout", "outflw", "rivvel"]
 
 
+def final_keys(ldam):
+    """Fields of the final-state fixture: D2DAMINF_aAVG too where dams operate."""
+    return FINAL_KEYS + (["daminf_avg"] if ldam else [])
+
+
 def sha(a):
     a = np.ascontiguousarray(a)
     if a.dtype.kind == "f":
@@ -157,14 +162,14 @@ def make(name):
                lslpmix=name in SLPMIX_CASES, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
                map_sha={k: sha(mp[k]) for k in keys_map}, steps=[])
     skeys = refdump.STATE_P + refdump.STATE_D + (["levsto", "levdph"] if llev else []) + \
-        (["damsto", "daminf"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])
+        (["damsto", "daminf", "daminf_avg"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])
     for s in states:
         out["steps"].append(dict(step=s["step"], sha={k: sha(s[k]) for k in skeys},
                                  glb={k: float(v) for k, v in s["glb"].items()}))
     with open(os.path.join(HERE, f"{name}.json"), "w") as f:
         json.dump(out, f, indent=0)
     fin = states[-1]
-    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"), **{k: fin[k] for k in FINAL_KEYS})
+    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"), **{k: fin[k] for k in final_keys(ldam)})
     print(f"{name}: n={mp['n']} steps={len(states) - 1} sum NT={sum(nts)}")
 
 
@@ -242,7 +247,7 @@ def make_restart_in(name):
     out = dict(case=name, base=base, lstoonly=lstoonly, spec=spec, n=mp["n"], nseqriv=mp["nseqriv"], nt=nts,
                steps=[])
     skeys = refdump.STATE_P + refdump.STATE_D + (["levsto", "levdph"] if llev else []) + \
-        (["damsto", "daminf"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])
+        (["damsto", "daminf", "daminf_avg"] if ldam else []) + (["pthflw", "pthflwsum"] if lpth else [])
     for s_ in states:
         out["steps"].append(dict(step=s_["step"], sha={k: sha(s_[k]) for k in skeys},
                                  glb={k: float(v) for k, v in s_["glb"].items()}))
@@ -253,7 +258,8 @@ def make_restart_in(name):
           "rivdph"] + (["pthflw", "pthflw_pre"] if lpth else []) + (["damsto"] if ldam else []) + \
         (["levsto"] if llev else [])
     np.savez_compressed(os.path.join(HERE, f"{name}_step0.npz"), **{k: s0[k] for k in k0})
-    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"), **{k: states[-1][k] for k in FINAL_KEYS})
+    np.savez_compressed(os.path.join(HERE, f"{name}_final.npz"),
+                        **{k: states[-1][k] for k in final_keys(ldam)})
     print(f"{name}: n={mp['n']} steps={len(states) - 1} sum NT={sum(nts)} (from {base}, lstoonly={lstoonly})")
 
 

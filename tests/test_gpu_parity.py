@@ -36,7 +36,8 @@ def _run(name, ndays=NDAYS, precision="fp64"):
             e["levsto"] = rel_err(g["levsto"], om.st["levsto"])
             e["levdph"] = rel_err(g["levdph"], om.st["levdph"])
         if m.opts.ldamout:
-            e["damsto"] = rel_err(g["damsto"], om.st["damsto"])
+            for k in ("damsto", "daminf", "daminf_avg"):
+                e[k] = rel_err(g[k], om.st[k])
         if m.opts.lpthout:
             e["pthflw"] = rel_err(g["pthflw"], om.st["pthflw"])
             e["pthflwsum"] = rel_err(g["pthflwsum"], om.st["pthflwsum"])
@@ -86,4 +87,42 @@ def test_dt_min_closed_form(name):
         print(f"  day {d + 1}: DT_MIN device {got!r} closed form {want!r}, NT {nt_g}", flush=True)
         assert got == want
         assert nt_g == nt_o == int(86400.0 * (1.0 / got) - float(np.float32(0.01))) + 1
+    gm.close()
+
+
+@pytest.mark.gpu
+def test_daminf_set_outside_dams():
+    """DAMINF changes only in cells with I1DAM > 0 (damout:442-465); every other
+    cell keeps the value the host gave it, and its D2DAMINF_aAVG (diag:130-135)
+    accumulates that value every substep. The device skips those cells while all
+    of them hold zero (the reference's initial value); a host state with nonzero
+    values there takes the full path. Both against the oracle (1e-10, identical
+    NT), and back to zero on the same handle."""
+    from cmf.model import CMFModel
+    from oracle.oracle import OracleModel
+    m, net, st, rvec = prepare("dam", 2)
+    rng = np.random.default_rng(5)
+    dense = dict(st)
+    dense["daminf"] = np.where(m.i1dam == 0, rng.uniform(0.5, 50.0, m.n), 0.0)
+    gm = CMFModel(m, "fp64")
+    for label, s0 in (("nonzero outside dams", dense), ("zero outside dams", st)):
+        om = OracleModel(m, s0)
+        gm.set_state(upload_keys(s0))
+        for d in range(2):
+            om.st["runoff"][:] = rvec[d]
+            gm.set_runoff(rvec[d])
+            nt_o = om.physics_advance(86400.0)
+            nt_g = gm.physics_advance(86400.0)
+            g = gm.get_state(["damsto", "daminf", "daminf_avg", "rivsto", "fldsto"])
+            e = {k: rel_err(g[k], om.st[k]) for k in g}
+            print(f"  {label}, day {d + 1}: NT {nt_o}/{nt_g}, {e}", flush=True)
+            assert nt_o == nt_g
+            for k, v in e.items():
+                assert v <= TOL_FP64, f"{label} day {d + 1}: {k} rel err {v:.3e}"
+        avg = gm.get_state(["daminf_avg"])["daminf_avg"]
+        outside = m.i1dam == 0
+        if s0 is dense:
+            assert np.all(avg[outside] > 0.0)
+        else:
+            assert np.all(avg[outside] == 0.0)
     gm.close()
