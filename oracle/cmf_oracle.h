@@ -14,7 +14,8 @@
  * Pinning: the physics (every function called by cmfo_physics_advance and
  * cmfo_physics_fldstg, the kinematic wave included) is bit-identical to the
  * reference's own run on the golden fixtures (every state array of every outer
- * step, NT, budget sums; tests/test_oracle_golden.py); cmfo_roff_interp and the
+ * step, the substep averages and maxima with D2DAMINF_aAVG included, NT, budget
+ * sums; tests/test_oracle_golden.py); cmfo_roff_interp and the
  * output diagnostics are bit-identical to the reference routines compiled from
  * their sources (tests/test_forcing.py, tests/test_output_diag.py); the upstream
  * inflow term is pinned by the one-substep storage difference it must produce
