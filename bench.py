@@ -113,13 +113,14 @@ def run_gpu(net, m, rvec, precision, steps, warmup, dt, tune="auto"):
     return sum_nt, t_total, kt, tune_used
 
 
-def _reference_case(d, net, ro, dt, steps, llevee=True, ldamout=True):
+def _reference_case(d, net, ro, dt, steps, llevee=True, ldamout=True, inpmat=None):
     """Reference inputs of the workload (levees, dams unless switched off) in
-    directory d, the run shortened to `steps` outer steps."""
+    directory d, the run shortened to `steps` outer steps; with inpmat, LINTERP
+    forcing on an input grid (ro then holds its records)."""
     from cmf import synth
     days = max(1, int(np.ceil(steps * dt / 86400.0)))
     synth.write_reference_inputs(net, d, ro, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
-                                 llevee=llevee, ldamout=ldamout)
+                                 llevee=llevee, ldamout=ldamout, inpmat=inpmat)
     # end time = start + steps*dt
     nml = open(os.path.join(d, "input_cmf.nam")).read()
     import datetime as _dt
@@ -243,7 +244,7 @@ def spinup_cpu_baseline(net, m, clim, days=30):
                        f"{sum(times):.1f} s, scaled to 365 days")
 
 
-def dropin_section(net, ro, n_cells, dt, steps):
+def dropin_section(net, ro, n_cells, dt, steps, m=None):
     """The Fortran drop-in end to end: the reference's host (serial flang build:
     time loop, forcing, diagnostics, log) with the device physics and the
     device-side output diagnostics (cmf_dropin_devdiag), on the same inputs as the
@@ -251,7 +252,11 @@ def dropin_section(net, ro, n_cells, dt, steps):
     host program (fortran/cmf_dropin_main.F90). Timed per outer step inside
     CMF_DRV_ADVANCE; the first step also uploads the map and the state. Once as
     built (downloads only when the host reads the state) and once with
-    CMF_HIP_DOWNLOAD=every (the whole state after every step over PCIe)."""
+    CMF_HIP_DOWNLOAD=every (the whole state after every step over PCIe). With the
+    map m, also with LINTERP forcing from a 1440x720 (0.25 deg) input grid, the
+    forcing put inside the timed step: the reference's CMF_FORCING_PUT on the host
+    (then D2RUNOFF goes over PCIe) against CMF_HIP_FORCING_PUT (the input field
+    goes over PCIe, the interpolation runs on the device)."""
     exe = os.path.join(ROOT, "cama-flood_v4_amd", "fortran", "_build", "cmf_dropin_devdiag")
     if not os.path.exists(exe):
         return None
@@ -275,6 +280,27 @@ def dropin_section(net, ro, n_cells, dt, steps):
                              substeps=int(sum(nts[1:])))
             log(f"drop-in ({mode}): first step {st[0]:.2f} s, then {1e3 * np.median(rest):.1f} ms per "
                 f"{dt:.0f} s step, {res[mode]['cell_substeps_per_s']:.3g} cell-substeps/s")
+    if m is None:
+        return res
+    from cmf import synth
+    inpx, inpy, inpa, nxin, nyin = synth.make_inpmat(net, (m.seqx, m.seqy), 0.25)
+    grids = synth.make_runoff_grid(nxin, nyin, 4, 2026, mean_mm_day=3.0, missing=0.0)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        _reference_case(d, net, grids, dt, steps, inpmat=(inpx, inpy, inpa, nxin, nyin, m.seqx, m.seqy))
+        for put in ("host", "device"):
+            env = dict(os.environ, CMF_HIP_FORCING=put)
+            log(f"drop-in run, LINTERP forcing put on the {put}")
+            r = _run_host(exe, d, env)
+            if r is None or len(r[1]) < 3:
+                return res
+            _out, st, nts = r
+            rest = st[1:]
+            key = f"forcing_{put}"
+            res[key] = dict(input_grid=f"{nxin}x{nyin}", inpn=int(inpx.shape[0]),
+                            step_ms_median=round(1e3 * float(np.median(rest)), 2),
+                            cell_substeps_per_s=n_cells * sum(nts[1:]) / sum(rest), substeps=int(sum(nts[1:])))
+            log(f"drop-in (LINTERP, put on the {put}): {1e3 * np.median(rest):.1f} ms per {dt:.0f} s step "
+                f"with the put, {res[key]['cell_substeps_per_s']:.3g} cell-substeps/s")
     return res
 
 
@@ -974,7 +1000,7 @@ def main():
     dropin = None
     if not a.no_dropin:
         try:
-            dropin = dropin_section(net, ro, n, dt, a.dropin_steps)
+            dropin = dropin_section(net, ro, n, dt, a.dropin_steps, m)
         except Exception as e:
             log(f"drop-in section failed: {e!r}")
             dropin = {"error": repr(e)}

@@ -382,7 +382,8 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
                            restart: bool = False, outputs: Optional[str] = None,
                            lkine: bool = False, restart_in: Optional[str] = None,
                            lstoonly: bool = False, ldamh22: bool = False,
-                           slpmix_mask: Optional[np.ndarray] = None) -> None:
+                           slpmix_mask: Optional[np.ndarray] = None,
+                           inpmat: Optional[tuple] = None) -> None:
     """Write the map files, tables, runoff records and input_cmf.nam that the
     reference needs (formats of src/cmf_ctrl_maps_mod.F90 READ_MAP_BIN,
     READ_TOPO_BIN, READ_BIFPARAM; src/cmf_ctrl_damout_mod.F90:141-179;
@@ -392,7 +393,12 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
     Hanazaki-2022 dam rule (LDAMH22). slpmix_mask ([ny, nx], 1 = kinematic wave)
     switches LSLPMIX on and is written to slpmix_mask.bin, which the test driver
     oracle/ref_driver.F90 applies to I2MASK (the reference reads it from its
-    netCDF river map only)."""
+    netCDF river map only). inpmat = (inpx, inpy, inpa, nxin, nyin, seqx, seqy),
+    the input matrix of make_inpmat with the NSEQ cell positions, switches LINTERP
+    on: inpmat.bin in the layout CMF_INPMAT_INIT_BIN reads (3*INPN records of NX*NY
+    4-byte values: INPX, INPY, INPA as float32; src/cmf_ctrl_forcing_mod.F90:418-458),
+    NXIN/NYIN/INPN in diminfo.txt, and runoff records on the input grid in mm/day
+    (DROFUNIT 86400*1000)."""
     os.makedirs(d, exist_ok=True)
     nx, ny = net.nx, net.ny
 
@@ -409,8 +415,21 @@ def write_reference_inputs(net: SynthNetwork, d: str, runoff: np.ndarray, *, dt:
     wf("rivhgt.bin", net.rivhgt)
     wf("rivman.bin", net.rivman)
     wf("fldhgt.bin", net.fldhgt)
+    nxin, nyin, inpn = nx, ny, 1
+    if inpmat is not None:
+        ix, iy, ia, nxin, nyin, sx, sy = inpmat
+        inpn = int(np.asarray(ix).shape[0])
+        cell = (np.asarray(sy, np.int64) - 1) * nx + (np.asarray(sx, np.int64) - 1)
+        recs = []
+        for tab, dt_ in ((ix, np.int32), (iy, np.int32), (ia, np.float32)):
+            for k in range(inpn):
+                mp_ = np.zeros(nx * ny, dt_)
+                mp_[cell] = np.asarray(tab[k]).astype(dt_)
+                recs.append(mp_.tobytes())
+        with open(os.path.join(d, "inpmat.bin"), "wb") as f:
+            f.write(b"".join(recs))
     with open(os.path.join(d, "diminfo.txt"), "w") as f:
-        f.write(f"{nx}\n{ny}\n{NLFP}\n{nx}\n{ny}\n1\nNONE\n{net.west}\n{net.east}\n"
+        f.write(f"{nx}\n{ny}\n{NLFP}\n{nxin}\n{nyin}\n{inpn}\nNONE\n{net.west}\n{net.east}\n"
                 f"{net.north}\n{net.south}\n")
     if lpthout:
         with open(os.path.join(d, "bifprm.txt"), "w") as f:
@@ -493,7 +512,8 @@ CPTHOUT = "bifprm.txt"
 /
 &NFORCE
 LINPCDF = .FALSE.
-LINTERP = .FALSE.
+LINTERP = {L(inpmat is not None)}
+CINPMAT = "inpmat.bin"
 /
 &NRESTART
 {restart_nml}

@@ -38,6 +38,9 @@ USE PARKIND1,              ONLY: JPIM, JPRB, JPRD
 IMPLICIT NONE
 PRIVATE
 PUBLIC :: CMF_PHYSICS_ADVANCE, CMF_PHYSICS_FLDSTG
+!! CMF_FORCING_PUT with the runoff interpolation on the device, for a main program
+!! or coupler that calls it in place of the reference's (see CMF_HIP_FORCING_PUT)
+PUBLIC :: CMF_HIP_FORCING_PUT
 !! for the device-side CMF_CALC_DIAG_MOD (cmf_calc_diag_mod.F90 of this package):
 !! the open handle, the error check, and LHIP_OUTDIAG, which that module sets so
 !! that the substep averages (D2*_aAVG, D2*_aMAX) are no longer downloaded after
@@ -45,6 +48,8 @@ PUBLIC :: CMF_PHYSICS_ADVANCE, CMF_PHYSICS_FLDSTG
 PUBLIC :: CMF_HIP_HANDLE, CMF_HIP_CHECK, LHIP_OUTDIAG
 LOGICAL, SAVE :: LHIP_OUTDIAG = .FALSE.
 LOGICAL, SAVE :: LEVERY = .FALSE.          ! CMF_HIP_DOWNLOAD=every
+LOGICAL, SAVE :: LDEVROF = .FALSE.         ! D2RUNOFF formed on the device (CMF_HIP_FORCING_PUT)
+LOGICAL, SAVE :: LINPMAT = .FALSE.         ! input matrix uploaded
 
 TYPE, BIND(C) :: CMF_DIMS_T
   INTEGER(C_INT32_T) :: NSEQALL, NSEQRIV, NLFP, NPTHOUT, NPTHLEV, NDAM
@@ -107,6 +112,17 @@ INTERFACE
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h, r
   END FUNCTION
+  INTEGER(C_INT) FUNCTION cmf_hip_set_inpmat(h, inpn, nxin, nyin, inpx, inpy, inpa, drofunit, rmis) &
+      & BIND(C, NAME='cmf_hip_set_inpmat')
+    IMPORT :: C_INT, C_PTR, C_INT32_T, C_DOUBLE
+    TYPE(C_PTR), VALUE :: h, inpx, inpy, inpa
+    INTEGER(C_INT32_T), VALUE :: inpn, nxin, nyin
+    REAL(C_DOUBLE), VALUE :: drofunit, rmis
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cmf_hip_set_runoff_grid(h, grid) BIND(C, NAME='cmf_hip_set_runoff_grid')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: h, grid
+  END FUNCTION
   INTEGER(C_INT) FUNCTION cmf_hip_set_dwnelv(h, d) BIND(C, NAME='cmf_hip_set_dwnelv')
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h, d
@@ -166,7 +182,8 @@ IF( .NOT. C_ASSOCIATED(HMAIN) )THEN
   CALL OPEN_HANDLE(HMAIN, .TRUE.)
   CALL CHECK(cmf_hip_set_state(HMAIN, STATE_PTRS(.TRUE.)), 'set_state')
 ENDIF
-CALL CHECK(cmf_hip_set_runoff(HMAIN, PTR_D(D2RUNOFF)), 'set_runoff')
+!! D2RUNOFF from the host, unless CMF_HIP_FORCING_PUT formed it on the device
+IF( .NOT. LDEVROF ) CALL CHECK(cmf_hip_set_runoff(HMAIN, PTR_D(D2RUNOFF)), 'set_runoff')
 !! CMF_BOUNDARY_UPDATE (drv_advance:84) moves D2DWNELV every step with LSEALEV
 IF( LSEALEV ) CALL CHECK(cmf_hip_set_dwnelv(HMAIN, PTR_D2(D2DWNELV)), 'set_dwnelv')
 !! CMF_UPSINF_UPDATE (drv_advance:87) sets D2UPSINF; stonxt:81 adds it
@@ -188,6 +205,49 @@ ELSEIF( .NOT. LHIP_OUTDIAG )THEN
   CALL CHECK(cmf_hip_get_state(HMAIN, STATE_PTRS(.FALSE., .TRUE.)), 'get_state')
 ENDIF
 END SUBROUTINE CMF_PHYSICS_ADVANCE
+!####################################################################
+SUBROUTINE CMF_HIP_FORCING_PUT(PBUFF)
+!! CMF_FORCING_PUT (src/cmf_ctrl_forcing_mod.F90:716-750) for a main program or
+!! coupler that calls it in place of the reference's: with LINTERP, the input field
+!! PBUFF(:,:,1) [NXIN,NYIN] goes to HBM and ROFF_INTERP (:757-790) runs there with
+!! the input matrix of YOS_CMF_MAP (uploaded on the first call), so D2RUNOFF is
+!! formed on the device, bit-identical to the host routine's, and never crosses
+!! PCIe; D2RUNOFF in host memory is then not updated (the device holds it). The
+!! host's CMF_FORCING_PUT runs instead (and the next advance uploads D2RUNOFF)
+!! without LINTERP (nearest-point CONV_RESOL), with LROSPLIT or LWEVAP, and on a
+!! single-precision host, whose interpolation rounds in float.
+USE YOS_CMF_INPUT,        ONLY: NXIN, NYIN, INPN, RMIS, LROSPLIT, LWEVAP
+USE YOS_CMF_MAP,          ONLY: INPX, INPY, INPA
+USE YOS_CMF_PROG,         ONLY: D2ROFSUB
+USE CMF_CTRL_FORCING_MOD, ONLY: CMF_FORCING_PUT, LINTERP, DROFUNIT
+IMPLICIT NONE
+REAL(KIND=JPRB), INTENT(IN), TARGET :: PBUFF(:,:,:)
+REAL(KIND=JPRB), ALLOCATABLE, TARGET, SAVE :: ZGRID(:,:)
+!================================================
+IF( .NOT. LINTERP .OR. LROSPLIT .OR. LWEVAP .OR. STORAGE_SIZE(0._JPRB) /= 64 )THEN
+  CALL CMF_FORCING_PUT(PBUFF)
+  LDEVROF = .FALSE.
+  RETURN
+ENDIF
+IF( .NOT. C_ASSOCIATED(HMAIN) )THEN
+  CALL OPEN_HANDLE(HMAIN, .TRUE.)
+  CALL CHECK(cmf_hip_set_state(HMAIN, STATE_PTRS(.TRUE.)), 'set_state')
+ENDIF
+IF( .NOT. LINPMAT )THEN
+  CALL CHECK(cmf_hip_set_inpmat(HMAIN, INT(INPN, C_INT32_T), INT(NXIN, C_INT32_T), INT(NYIN, C_INT32_T), &
+    & PTR_I2(INPX), PTR_I2(INPY), PTR_D2(INPA), REAL(DROFUNIT, C_DOUBLE), REAL(RMIS, C_DOUBLE)), 'set_inpmat')
+  D2ROFSUB(:,:) = 0._JPRB                    !! forcing_put:730-731 without LROSPLIT
+  LINPMAT = .TRUE.
+ENDIF
+IF( IS_CONTIGUOUS(PBUFF) .AND. SIZE(PBUFF,1) == NXIN .AND. SIZE(PBUFF,2) == NYIN )THEN
+  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D3(PBUFF)), 'set_runoff_grid')
+ELSE
+  IF( .NOT. ALLOCATED(ZGRID) ) ALLOCATE(ZGRID(NXIN, NYIN))
+  ZGRID(:,:) = PBUFF(1:NXIN, 1:NYIN, 1)
+  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D2(ZGRID)), 'set_runoff_grid')
+ENDIF
+LDEVROF = .TRUE.
+END SUBROUTINE CMF_HIP_FORCING_PUT
 !####################################################################
 LOGICAL FUNCTION STATE_DUE()
 !! the host reads the state after this step: output time (drv_advance:121) or

@@ -205,3 +205,46 @@ def test_single_precision_dropin_matches_reference(name, tmp_path):
     e_out = rel_err(last["outflw"], fin["outflw"])
     print(f"  storage cell {e_cell:.3e} total {e_tot:.3e} outflow {e_out:.3e}", flush=True)
     assert e_cell <= 2e-5 and e_tot <= 1e-6 and e_out <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["g_allwet", "g_hourly"])
+def test_dropin_device_forcing_put(name, tmp_path):
+    """LINTERP forcing (an input matrix onto a 5-degree input grid, runoff fields
+    with missing values) in cmf_dropin_devdiag: CMF_HIP_FORCING_PUT, which runs
+    ROFF_INTERP on the device, against the reference's CMF_FORCING_PUT on the host
+    (CMF_HIP_FORCING=host). The interpolation is bit-identical and the physics the
+    same, so every output file of the two runs is byte-identical; the runoff
+    output shows that the forcing arrived."""
+    path = os.path.join(BUILD, "cmf_dropin_devdiag")
+    if not os.path.exists(path):
+        pytest.skip("cmf_dropin_devdiag not built")
+    from cmf import synth
+    from golden_cases import setup
+    print(f"drop-in device forcing put, {name}", flush=True)
+    net, ro, ref, fin = load(name)
+    nx, ny, seed, days, dt, lpth, llev, ldam, mean, levfrac = ref["spec"]
+    m = setup(name)[0]
+    inpx, inpy, inpa, nxin, nyin = synth.make_inpmat(net, (m.seqx, m.seqy), 5.0)
+    grids = synth.make_runoff_grid(nxin, nyin, 4, seed, mean_mm_day=3.0 * mean, missing=0.05)
+    variables = OUTPUT_VARS if lpth else OUTPUT_VARS.replace(",pthflw", "")
+    got = {}
+    for put in ("host", "device"):
+        d = os.path.join(str(tmp_path), put)
+        synth.write_reference_inputs(net, d, grids, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
+                                     lpthout=lpth, llevee=llev, ldamout=ldam, outputs=variables,
+                                     inpmat=(inpx, inpy, inpa, nxin, nyin, m.seqx, m.seqy))
+        env = dict(os.environ, CMF_HIP_FORCING=put)
+        r = subprocess.run([path, "runoff.bin"], cwd=d, capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        got[put] = {}
+        for v in variables.split(","):
+            files = [f for f in os.listdir(d) if f.startswith(v + "_cmf")]
+            assert len(files) == 1, (put, v, files)
+            got[put][v] = np.fromfile(os.path.join(d, files[0]), np.uint8)
+    for v in variables.split(","):
+        same = np.array_equal(got["host"][v], got["device"][v])
+        print(f"  {v}: {got['host'][v].size} bytes, identical {same}", flush=True)
+        assert same, v
+    roff = got["device"]["runoff"].view(np.float32)
+    assert np.isfinite(roff).all() and roff.max() > 0.0
