@@ -9,6 +9,9 @@ MODULE CMF_CTRL_PHYSICS_MOD
 !  (src/cmf_drv_control_mod.F90:171,243,276) compile unchanged against it:
 !    CMF_PHYSICS_ADVANCE  -- one outer step (physics:21-155)
 !    CMF_PHYSICS_FLDSTG   -- storage -> stage (physics:288-305)
+!  and, for a main program or coupler that calls it in place of CMF_FORCING_PUT
+!  (src/cmf_ctrl_forcing_mod.F90:716-750), CMF_HIP_FORCING_PUT: the LINTERP runoff
+!  interpolation on the device, so that D2RUNOFF is formed in HBM.
 !
 !  The model state lives in HBM. The first CMF_PHYSICS_ADVANCE (after the whole
 !  initialisation, dams and restart included) uploads the map and the state;
