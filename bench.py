@@ -256,7 +256,8 @@ def dropin_section(net, ro, n_cells, dt, steps, m=None):
     map m, also with LINTERP forcing from a 1440x720 (0.25 deg) input grid, the
     forcing put inside the timed step: the reference's CMF_FORCING_PUT on the host
     (then D2RUNOFF goes over PCIe) against CMF_HIP_FORCING_PUT (the input field
-    goes over PCIe, the interpolation runs on the device)."""
+    goes over PCIe, the interpolation runs on the device) and against the fields
+    kept resident in HBM (CMF_HIP_FORCING=resident: nothing crosses PCIe per step)."""
     exe = os.path.join(ROOT, "cama-flood_v4_amd", "fortran", "_build", "cmf_dropin_devdiag")
     if not os.path.exists(exe):
         return None
@@ -287,7 +288,7 @@ def dropin_section(net, ro, n_cells, dt, steps, m=None):
     grids = synth.make_runoff_grid(nxin, nyin, 4, 2026, mean_mm_day=3.0, missing=0.0)
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
         _reference_case(d, net, grids, dt, steps, inpmat=(inpx, inpy, inpa, nxin, nyin, m.seqx, m.seqy))
-        for put in ("host", "device"):
+        for put in ("host", "device", "resident"):
             env = dict(os.environ, CMF_HIP_FORCING=put)
             log(f"drop-in run, LINTERP forcing put on the {put}")
             r = _run_host(exe, d, env)

@@ -93,6 +93,8 @@ HIP_FUNCS = {
     "cmf_hip_set_inpmat": (I, [H, C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_double, C.c_double]),
     "cmf_hip_set_runoff_grid": (I, [H, P]),
     "cmf_hip_interp_runoff": (I, [H, P]),
+    "cmf_hip_set_runoff_records": (I, [H, C.c_int32, P]),
+    "cmf_hip_use_runoff_record": (I, [H, C.c_int32]),
     "cmf_hip_set_dwnelv": (I, [H, P]),
     "cmf_hip_set_upsinf": (I, [H, P]),
     "cmf_hip_set_kinematic": (I, [H, C.c_int32]),

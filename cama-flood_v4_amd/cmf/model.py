@@ -214,6 +214,18 @@ class CMFModel:
             raise CMFError("set_runoff_grid: grid must hold NXIN*NYIN values")
         self._check(self.lib.cmf_hip_set_runoff_grid(self.h, g.ctypes.data))
 
+    def set_runoff_records(self, grids: np.ndarray) -> None:
+        """Keep a run's input-grid runoff fields [nrec, nyin, nxin] resident in
+        HBM (one copy); use_runoff_record(k) then forms D2RUNOFF from field k."""
+        g = _f64(grids)
+        if g.size == 0 or g.size % (self.nxin * self.nyin):
+            raise CMFError("set_runoff_records: grids must hold nrec*NXIN*NYIN values")
+        self._check(self.lib.cmf_hip_set_runoff_records(self.h, g.size // (self.nxin * self.nyin), g.ctypes.data))
+
+    def use_runoff_record(self, k: int) -> None:
+        """Interpolate resident field k (0-based) into D2RUNOFF on the device."""
+        self._check(self.lib.cmf_hip_use_runoff_record(self.h, int(k)))
+
     def interp_runoff(self, device_ptr: int) -> None:
         """Interpolate a runoff field that is already in device memory."""
         self._check(self.lib.cmf_hip_interp_runoff(self.h, C.c_void_p(device_ptr)))

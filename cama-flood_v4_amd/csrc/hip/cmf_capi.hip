@@ -100,6 +100,8 @@ struct cmf_hip_ctx {
     double nadd_out = 0.0;
     double* d_grid = nullptr;                        // staging of cmf_hip_set_runoff_grid
     size_t grid_cap = 0;
+    double* d_records = nullptr;                     // forcing fields resident in HBM
+    int32_t nrecords = 0;
     size_t partial_cap = 0;           // doubles in S.partials
     int hist_cap = 0;
     // device cell order: perm[p] = NSEQ index of device position p (identity when
@@ -1566,6 +1568,35 @@ int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid_in) {
     }
     HIPCHK(hipMemcpyAsync(h->d_grid, grid, ng * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return cmf_hip_interp_runoff(h, h->d_grid);
+}
+
+int cmf_hip_set_runoff_records(cmf_handle h, int32_t nrec, const double* grids_in) {
+    if (!h || !grids_in) return fail(CMF_EINVAL, "NULL argument");
+    if (!h->roff.inpx) return fail(CMF_ESTATE, "cmf_hip_set_inpmat must come first");
+    if (nrec <= 0) return fail(CMF_EINVAL, "nrec=%d must be positive", (int)nrec);
+    const size_t ng = (size_t)h->roff.nxin * (size_t)h->roff.nyin, nt = ng * (size_t)nrec;
+    Widen w(h);
+    const double* grids = w(grids_in, nt);
+    if (h->d_records) {
+        (void)hipFree(h->d_records);
+        for (auto& a : h->allocs) if (a == (void*)h->d_records) a = nullptr;
+        h->d_records = nullptr;
+        h->nrecords = 0;
+    }
+    TRY(dalloc(h, &h->d_records, nt));
+    HIPCHK(hipMemcpyAsync(h->d_records, grids, nt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));        // the widened copy lives until here
+    h->nrecords = nrec;
+    return 0;
+}
+
+int cmf_hip_use_runoff_record(cmf_handle h, int32_t k) {
+    if (!h) return fail(CMF_EINVAL, "NULL handle");
+    if (!h->d_records) return fail(CMF_ESTATE, "cmf_hip_set_runoff_records must come first");
+    if (k < 0 || k >= h->nrecords)
+        return fail(CMF_EINVAL, "record %d outside 0..%d", (int)k, (int)h->nrecords - 1);
+    const size_t ng = (size_t)h->roff.nxin * (size_t)h->roff.nyin;
+    return cmf_hip_interp_runoff(h, h->d_records + (size_t)k * ng);
 }
 
 namespace {

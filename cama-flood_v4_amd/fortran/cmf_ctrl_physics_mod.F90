@@ -44,6 +44,8 @@ PUBLIC :: CMF_PHYSICS_ADVANCE, CMF_PHYSICS_FLDSTG
 !! CMF_FORCING_PUT with the runoff interpolation on the device, for a main program
 !! or coupler that calls it in place of the reference's (see CMF_HIP_FORCING_PUT)
 PUBLIC :: CMF_HIP_FORCING_PUT
+!! a run's forcing fields resident in HBM: copied once, one selected per forcing step
+PUBLIC :: CMF_HIP_FORCING_RECORDS, CMF_HIP_FORCING_PUT_RECORD
 !! for the device-side CMF_CALC_DIAG_MOD (cmf_calc_diag_mod.F90 of this package):
 !! the open handle, the error check, and LHIP_OUTDIAG, which that module sets so
 !! that the substep averages (D2*_aAVG, D2*_aMAX) are no longer downloaded after
@@ -125,6 +127,16 @@ INTERFACE
   INTEGER(C_INT) FUNCTION cmf_hip_set_runoff_grid(h, grid) BIND(C, NAME='cmf_hip_set_runoff_grid')
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h, grid
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cmf_hip_set_runoff_records(h, nrec, grids) BIND(C, NAME='cmf_hip_set_runoff_records')
+    IMPORT :: C_INT, C_PTR, C_INT32_T
+    TYPE(C_PTR), VALUE :: h, grids
+    INTEGER(C_INT32_T), VALUE :: nrec
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cmf_hip_use_runoff_record(h, k) BIND(C, NAME='cmf_hip_use_runoff_record')
+    IMPORT :: C_INT, C_PTR, C_INT32_T
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT32_T), VALUE :: k
   END FUNCTION
   INTEGER(C_INT) FUNCTION cmf_hip_set_dwnelv(h, d) BIND(C, NAME='cmf_hip_set_dwnelv')
     IMPORT :: C_INT, C_PTR
@@ -219,19 +231,71 @@ SUBROUTINE CMF_HIP_FORCING_PUT(PBUFF)
 !! host's CMF_FORCING_PUT runs instead (and the next advance uploads D2RUNOFF)
 !! without LINTERP (nearest-point CONV_RESOL), with LROSPLIT or LWEVAP, and on a
 !! single-precision host, whose interpolation rounds in float.
-USE YOS_CMF_INPUT,        ONLY: NXIN, NYIN, INPN, RMIS, LROSPLIT, LWEVAP
-USE YOS_CMF_MAP,          ONLY: INPX, INPY, INPA
-USE YOS_CMF_PROG,         ONLY: D2ROFSUB
-USE CMF_CTRL_FORCING_MOD, ONLY: CMF_FORCING_PUT, LINTERP, DROFUNIT
+USE YOS_CMF_INPUT,        ONLY: NXIN, NYIN
+USE CMF_CTRL_FORCING_MOD, ONLY: CMF_FORCING_PUT
 IMPLICIT NONE
 REAL(KIND=JPRB), INTENT(IN), TARGET :: PBUFF(:,:,:)
 REAL(KIND=JPRB), ALLOCATABLE, TARGET, SAVE :: ZGRID(:,:)
 !================================================
-IF( .NOT. LINTERP .OR. LROSPLIT .OR. LWEVAP .OR. STORAGE_SIZE(0._JPRB) /= 64 )THEN
+IF( .NOT. DEVICE_FORCING() )THEN
   CALL CMF_FORCING_PUT(PBUFF)
   LDEVROF = .FALSE.
   RETURN
 ENDIF
+CALL DEVICE_FORCING_INIT
+IF( IS_CONTIGUOUS(PBUFF) .AND. SIZE(PBUFF,1) == NXIN .AND. SIZE(PBUFF,2) == NYIN )THEN
+  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D3(PBUFF)), 'set_runoff_grid')
+ELSE
+  IF( .NOT. ALLOCATED(ZGRID) ) ALLOCATE(ZGRID(NXIN, NYIN))
+  ZGRID(:,:) = PBUFF(1:NXIN, 1:NYIN, 1)
+  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D2(ZGRID)), 'set_runoff_grid')
+ENDIF
+LDEVROF = .TRUE.
+END SUBROUTINE CMF_HIP_FORCING_PUT
+!####################################################################
+SUBROUTINE CMF_HIP_FORCING_RECORDS(PBUFFS)
+!! A run's input fields PBUFFS(NXIN,NYIN,NREC), in the order the main program will
+!! select them, copied to HBM once (LINTERP on an FP64 host; the module stops
+!! otherwise); CMF_HIP_FORCING_PUT_RECORD(IREC) then does CMF_HIP_FORCING_PUT's
+!! work for field IREC with nothing crossing PCIe.
+USE YOS_CMF_INPUT,        ONLY: NXIN, NYIN
+IMPLICIT NONE
+REAL(KIND=JPRB), INTENT(IN), TARGET :: PBUFFS(:,:,:)
+!================================================
+IF( .NOT. DEVICE_FORCING() ) CALL STOPMSG('resident forcing needs LINTERP on an FP64 host')
+IF( SIZE(PBUFFS,1) /= NXIN .OR. SIZE(PBUFFS,2) /= NYIN .OR. .NOT. IS_CONTIGUOUS(PBUFFS) ) &
+  & CALL STOPMSG('resident forcing: PBUFFS must be a contiguous (NXIN,NYIN,NREC) array')
+CALL DEVICE_FORCING_INIT
+CALL CHECK(cmf_hip_set_runoff_records(HMAIN, INT(SIZE(PBUFFS,3), C_INT32_T), PTR_D3(PBUFFS)), 'set_runoff_records')
+END SUBROUTINE CMF_HIP_FORCING_RECORDS
+!####################################################################
+SUBROUTINE CMF_HIP_FORCING_PUT_RECORD(IREC)
+!! D2RUNOFF from resident field IREC (1-based) of CMF_HIP_FORCING_RECORDS
+IMPLICIT NONE
+INTEGER(KIND=JPIM), INTENT(IN) :: IREC
+!================================================
+CALL CHECK(cmf_hip_use_runoff_record(HMAIN, INT(IREC-1, C_INT32_T)), 'use_runoff_record')
+LDEVROF = .TRUE.
+END SUBROUTINE CMF_HIP_FORCING_PUT_RECORD
+!####################################################################
+LOGICAL FUNCTION DEVICE_FORCING()
+!! the options under which ROFF_INTERP runs on the device: LINTERP, FP64 host, no
+!! second or third forcing field (LROSPLIT, LWEVAP)
+USE YOS_CMF_INPUT,        ONLY: LROSPLIT, LWEVAP
+USE CMF_CTRL_FORCING_MOD, ONLY: LINTERP
+IMPLICIT NONE
+DEVICE_FORCING = LINTERP .AND. .NOT. LROSPLIT .AND. .NOT. LWEVAP .AND. STORAGE_SIZE(0._JPRB) == 64
+END FUNCTION DEVICE_FORCING
+!####################################################################
+SUBROUTINE DEVICE_FORCING_INIT
+!! the handle (with the state after the host's initialisation) and, once, the
+!! input matrix of YOS_CMF_MAP
+USE YOS_CMF_INPUT,        ONLY: NXIN, NYIN, INPN, RMIS
+USE YOS_CMF_MAP,          ONLY: INPX, INPY, INPA
+USE YOS_CMF_PROG,         ONLY: D2ROFSUB
+USE CMF_CTRL_FORCING_MOD, ONLY: DROFUNIT
+IMPLICIT NONE
+!================================================
 IF( .NOT. C_ASSOCIATED(HMAIN) )THEN
   CALL OPEN_HANDLE(HMAIN, .TRUE.)
   CALL CHECK(cmf_hip_set_state(HMAIN, STATE_PTRS(.TRUE.)), 'set_state')
@@ -242,15 +306,7 @@ IF( .NOT. LINPMAT )THEN
   D2ROFSUB(:,:) = 0._JPRB                    !! forcing_put:730-731 without LROSPLIT
   LINPMAT = .TRUE.
 ENDIF
-IF( IS_CONTIGUOUS(PBUFF) .AND. SIZE(PBUFF,1) == NXIN .AND. SIZE(PBUFF,2) == NYIN )THEN
-  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D3(PBUFF)), 'set_runoff_grid')
-ELSE
-  IF( .NOT. ALLOCATED(ZGRID) ) ALLOCATE(ZGRID(NXIN, NYIN))
-  ZGRID(:,:) = PBUFF(1:NXIN, 1:NYIN, 1)
-  CALL CHECK(cmf_hip_set_runoff_grid(HMAIN, PTR_D2(ZGRID)), 'set_runoff_grid')
-ENDIF
-LDEVROF = .TRUE.
-END SUBROUTINE CMF_HIP_FORCING_PUT
+END SUBROUTINE DEVICE_FORCING_INIT
 !####################################################################
 LOGICAL FUNCTION STATE_DUE()
 !! the host reads the state after this step: output time (drv_advance:121) or

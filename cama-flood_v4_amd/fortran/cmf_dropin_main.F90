@@ -13,7 +13,10 @@
 !                (DROFUNIT), passed as the first field of PBUFF to the forcing put
 !                of the main loop of src/MAIN_cmf.F90: CMF_HIP_FORCING_PUT
 !                (interpolation on the device), or the reference's CMF_FORCING_PUT
-!                on the host when CMF_HIP_FORCING=host is set
+!                on the host when CMF_HIP_FORCING=host is set; with
+!                CMF_HIP_FORCING=resident every record is copied to HBM before the
+!                time loop (CMF_HIP_FORCING_RECORDS) and each forcing step selects
+!                its record there (CMF_HIP_FORCING_PUT_RECORD)
 !   stdout     : "STEPTIME <step> <seconds>" per outer step (time inside
 !                CMF_DRV_ADVANCE, and with LINTERP inside the forcing put too),
 !                then "RUNTIME <seconds> <nsteps> <nseqall>"
@@ -29,14 +32,14 @@ USE YOS_CMF_PROG,            ONLY: D2RUNOFF
 USE CMF_DRV_CONTROL_MOD,     ONLY: CMF_DRV_INPUT, CMF_DRV_INIT, CMF_DRV_END
 USE CMF_DRV_ADVANCE_MOD,     ONLY: CMF_DRV_ADVANCE
 USE CMF_CTRL_FORCING_MOD,    ONLY: CMF_FORCING_PUT, LINTERP
-USE CMF_CTRL_PHYSICS_MOD,    ONLY: CMF_HIP_FORCING_PUT
+USE CMF_CTRL_PHYSICS_MOD,    ONLY: CMF_HIP_FORCING_PUT, CMF_HIP_FORCING_RECORDS, CMF_HIP_FORCING_PUT_RECORD
 IMPLICIT NONE
 CHARACTER(LEN=512)           :: CROF
 CHARACTER(LEN=16)            :: CPUT
 INTEGER(KIND=JPIM)           :: ISTEP, NPERREC, IREC, NREC, ISEQ, IU, MX, MY
 REAL(KIND=JPRD), ALLOCATABLE :: ROF(:,:)
-REAL(KIND=JPRB), ALLOCATABLE :: ZBUFF(:,:,:)
-LOGICAL                      :: LNEW, LHOSTPUT
+REAL(KIND=JPRB), ALLOCATABLE :: ZBUFF(:,:,:), ZALL(:,:,:)
+LOGICAL                      :: LNEW, LHOSTPUT, LRESIDENT
 INTEGER(KIND=8)              :: T0, T1, RATE, TSUM, NBYTES
 
 CALL GET_COMMAND_ARGUMENT(1, CROF)
@@ -53,12 +56,22 @@ IF( LINTERP )THEN
 ENDIF
 CALL GET_ENVIRONMENT_VARIABLE('CMF_HIP_FORCING', CPUT)
 LHOSTPUT = TRIM(CPUT) == 'host'
+LRESIDENT = LINTERP .AND. TRIM(CPUT) == 'resident'
 ALLOCATE(ROF(MX,MY))
 OPEN(NEWUNIT=IU, FILE=TRIM(CROF), FORM='UNFORMATTED', ACCESS='STREAM', STATUS='OLD')
 INQUIRE(UNIT=IU, SIZE=NBYTES)
 NREC = INT(NBYTES / (8_8*MX*MY), JPIM)
 IF( NREC < 1 ) STOP 'cmf_dropin_main: runoff file holds no float64 record of the forcing grid'
 NPERREC = MAX(1, INT(DTIN/DT, JPIM))
+IF( LRESIDENT )THEN
+  ALLOCATE(ZALL(NXIN,NYIN,NREC))
+  DO IREC=1, NREC
+    READ(IU, POS=1_8 + INT(IREC-1,8)*8_8*MX*MY) ROF
+    ZALL(:,:,IREC) = REAL(ROF, JPRB)
+  END DO
+  CALL CMF_HIP_FORCING_RECORDS(ZALL)
+  DEALLOCATE(ZALL)
+ENDIF
 
 TSUM = 0
 IREC = 0
@@ -66,6 +79,8 @@ DO ISTEP=1, NSTEPS
   LNEW = MOD(ISTEP-1, NPERREC) == 0
   IF( LNEW )THEN
     IREC = MOD(IREC, NREC) + 1
+  ENDIF
+  IF( LNEW .AND. .NOT. LRESIDENT )THEN
     READ(IU, POS=1_8 + INT(IREC-1,8)*8_8*MX*MY) ROF
     IF( LINTERP )THEN
       ZBUFF(:,:,1) = REAL(ROF, JPRB)
@@ -77,7 +92,9 @@ DO ISTEP=1, NSTEPS
   ENDIF
   CALL SYSTEM_CLOCK(T0, RATE)
   IF( LINTERP .AND. LNEW )THEN
-    IF( LHOSTPUT )THEN
+    IF( LRESIDENT )THEN
+      CALL CMF_HIP_FORCING_PUT_RECORD(IREC)
+    ELSEIF( LHOSTPUT )THEN
       CALL CMF_FORCING_PUT(ZBUFF)
     ELSE
       CALL CMF_HIP_FORCING_PUT(ZBUFF)

@@ -184,6 +184,16 @@ int cmf_hip_set_inpmat(cmf_handle h, int32_t inpn, int32_t nxin, int32_t nyin, c
 int cmf_hip_set_runoff_grid(cmf_handle h, const double* grid);
 int cmf_hip_interp_runoff(cmf_handle h, const double* d_grid);
 
+/* A run's forcing resident in HBM. No reference counterpart (the reference reads
+ * one field per forcing interval, CMF_FORCING_GET): cmf_hip_set_runoff_records
+ * copies nrec input fields PBUFFIN(NXIN,NYIN) (x fastest, one after another, JPRB
+ * per the handle) to the device once, replacing any earlier set;
+ * cmf_hip_use_runoff_record interpolates record k (0-based) into D2RUNOFF as
+ * cmf_hip_set_runoff_grid does, with no host copy. Needs cmf_hip_set_inpmat first.
+ * A year of hourly 0.25-degree fields is 73 GB in FP64. */
+int cmf_hip_set_runoff_records(cmf_handle h, int32_t nrec, const double* grids);
+int cmf_hip_use_runoff_record(cmf_handle h, int32_t k);
+
 /* Output-period diagnostics on the device: CMF_DIAG_RESET_OUTPUT,
  * CMF_DIAG_AVEMAX_OUTPUT and CMF_DIAG_GETAVE_OUTPUT (src/cmf_calc_diag_mod.F90:
  * 222-356), called where the reference's CMF_DRV_ADVANCE calls them

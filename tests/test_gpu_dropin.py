@@ -212,9 +212,10 @@ def test_single_precision_dropin_matches_reference(name, tmp_path):
 def test_dropin_device_forcing_put(name, tmp_path):
     """LINTERP forcing (an input matrix onto a 5-degree input grid, runoff fields
     with missing values) in cmf_dropin_devdiag: CMF_HIP_FORCING_PUT, which runs
-    ROFF_INTERP on the device, against the reference's CMF_FORCING_PUT on the host
+    ROFF_INTERP on the device, and the same from fields kept resident in HBM
+    (CMF_HIP_FORCING=resident), against the reference's CMF_FORCING_PUT on the host
     (CMF_HIP_FORCING=host). The interpolation is bit-identical and the physics the
-    same, so every output file of the two runs is byte-identical; the runoff
+    same, so every output file of the three runs is byte-identical; the runoff
     output shows that the forcing arrived."""
     path = os.path.join(BUILD, "cmf_dropin_devdiag")
     if not os.path.exists(path):
@@ -229,7 +230,7 @@ def test_dropin_device_forcing_put(name, tmp_path):
     grids = synth.make_runoff_grid(nxin, nyin, 4, seed, mean_mm_day=3.0 * mean, missing=0.05)
     variables = OUTPUT_VARS if lpth else OUTPUT_VARS.replace(",pthflw", "")
     got = {}
-    for put in ("host", "device"):
+    for put in ("host", "device", "resident"):
         d = os.path.join(str(tmp_path), put)
         synth.write_reference_inputs(net, d, grids, dt=dt, ifrq_inp=int(dt // 3600), ndays=days,
                                      lpthout=lpth, llevee=llev, ldamout=ldam, outputs=variables,
@@ -243,8 +244,8 @@ def test_dropin_device_forcing_put(name, tmp_path):
             assert len(files) == 1, (put, v, files)
             got[put][v] = np.fromfile(os.path.join(d, files[0]), np.uint8)
     for v in variables.split(","):
-        same = np.array_equal(got["host"][v], got["device"][v])
-        print(f"  {v}: {got['host'][v].size} bytes, identical {same}", flush=True)
-        assert same, v
+        same = [np.array_equal(got["host"][v], got[put][v]) for put in ("device", "resident")]
+        print(f"  {v}: {got['host'][v].size} bytes, device / resident identical {same}", flush=True)
+        assert all(same), v
     roff = got["device"]["runoff"].view(np.float32)
     assert np.isfinite(roff).all() and roff.max() > 0.0
