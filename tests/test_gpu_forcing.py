@@ -74,3 +74,35 @@ def test_inpmat_errors():
     with pytest.raises(CMFError):
         g.set_inpmat(*mat[:5])                               # only once
     g.close()
+
+
+@pytest.mark.gpu
+def test_resident_records_match_single_fields():
+    """Fields kept resident (cmf_hip_set_runoff_records, used out of order and
+    more than once through cmf_hip_use_runoff_record) give the state that one
+    cmf_hip_set_runoff_grid per step gives, bit for bit; a record index outside
+    the set is an error, and so is a record before any set."""
+    from cmf.model import CMFError, CMFModel
+    m, net, st, _ = prepare("plain", 1)
+    mat, grids, ref = _inputs(m, net, 9)
+    order = [1, 0, 1]
+    a = CMFModel(m, "fp64")
+    b = CMFModel(m, "fp64")
+    for g in (a, b):
+        g.set_state(upload_keys(st))
+        g.set_inpmat(*mat[:5], drofunit=mat[5])
+    with pytest.raises(CMFError):
+        a.use_runoff_record(0)                               # no records yet
+    a.set_runoff_records(np.stack(grids))
+    for k in order:
+        print(f"record {k}", flush=True)
+        a.use_runoff_record(k)
+        b.set_runoff_grid(grids[k])
+        assert a.physics_advance(86400.0) == b.physics_advance(86400.0)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    with pytest.raises(CMFError):
+        a.use_runoff_record(len(grids))
+    a.close()
+    b.close()
