@@ -8,6 +8,8 @@ src/cmf_ctrl_forcing_mod.F90:757-790), CPU side.
 * Mass conservation: the interpolated total equals the area-weighted sum of the
   field over the entries in the grid that are not RMIS.
 * The synthetic input matrix covers every cell with its own area.
+* The input matrix file written for the reference has the layout its binary
+  reader expects, and the reference's initialisation reads it.
 """
 import os
 import subprocess
@@ -90,3 +92,45 @@ def test_oracle_matches_reference_forcing_put():
     assert ref.shape == (n,)
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
     assert "XXX" in r.stdout                                  # the reference logged the skipped entries
+
+
+def test_inpmat_file_in_the_reference_layout():
+    """write_reference_inputs with an input matrix: inpmat.bin holds 3*INPN records
+    of NX*NY 4-byte values in the order CMF_INPMAT_INIT_BIN reads them (INPX, INPY,
+    then INPA as float32, one record per entry, zero off the river network;
+    src/cmf_ctrl_forcing_mod.F90:418-458), diminfo.txt carries NXIN/NYIN/INPN and the
+    namelist switches LINTERP on. Where the reference build is present, its own
+    initialisation (CMF_DRV_INIT -> CMF_INPMAT_INIT_BIN) reads the file."""
+    net, m, inpx, inpy, inpa, nxin, nyin, grid = _case(seed=6)
+    inpn = inpx.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        synth.write_reference_inputs(net, d, grid[None], dt=86400.0, ifrq_inp=24, ndays=1,
+                                     inpmat=(inpx, inpy, inpa, nxin, nyin, m.seqx, m.seqy))
+        raw = np.fromfile(os.path.join(d, "inpmat.bin"), np.uint8)
+        assert raw.size == 3 * inpn * net.nx * net.ny * 4
+        recs = raw.reshape(3 * inpn, net.ny * net.nx, 4)
+        cell = (m.seqy.astype(np.int64) - 1) * net.nx + (m.seqx.astype(np.int64) - 1)
+        off = np.ones(net.nx * net.ny, bool)
+        off[cell] = False
+        for k in range(inpn):
+            x = recs[k].copy().view(np.int32).ravel()
+            y = recs[inpn + k].copy().view(np.int32).ravel()
+            a = recs[2 * inpn + k].copy().view(np.float32).ravel()
+            assert np.array_equal(x[cell], inpx[k]) and np.array_equal(y[cell], inpy[k])
+            assert np.array_equal(a[cell], inpa[k].astype(np.float32))
+            assert not x[off].any() and not y[off].any() and not a[off].any()
+        dim = open(os.path.join(d, "diminfo.txt")).read().split()
+        assert [int(v) for v in dim[:6]] == [net.nx, net.ny, 10, nxin, nyin, inpn]
+        nml = open(os.path.join(d, "input_cmf.nam")).read()
+        assert "LINTERP = .TRUE." in nml and 'CINPMAT = "inpmat.bin"' in nml
+        exe = os.path.join(os.path.dirname(REF), "cmf_ref")
+        if os.path.exists(exe):
+            # the test driver assigns D2RUNOFF from NX*NY records of its own; the
+            # initialisation reads the input matrix all the same
+            np.zeros(net.nx * net.ny).tofile(os.path.join(d, "runoff_map.bin"))
+            r = subprocess.run([exe, "runoff_map.bin", "NONE", "0"], cwd=d, capture_output=True, text=True,
+                               timeout=120)
+            log = open(os.path.join(d, "log_CaMa.txt"), errors="replace").read()
+            assert r.returncode == 0, r.stdout[-1500:] + r.stderr[-1500:]
+            assert "INPUT MATRIX done" in log
+            assert [int(v) for v in log.split("NXIN,NYIN,INPN")[1].split()[:3]] == [nxin, nyin, inpn]
