@@ -28,8 +28,9 @@ MODULE CMF_CTRL_PHYSICS_MOD
 !  Links into either build of the reference: FP64 (JPRB = 8 bytes) or the default
 !  single precision (-DSinglePrec_CMF, JPRB = 4: the JPRB arrays cross the C-ABI
 !  as c_float, the P2* storages as c_double). The device path defaults to FP64
-!  under the FP64 host and to the mixed-precision path (FP32 flux and stage
-!  arithmetic, FP64 storage) under the single-precision host;
+!  under the FP64 host and to the mixed-precision path (FP32 flux arithmetic and
+!  flux/stage arrays; FP64 storage, storage update and stage inversion) under the
+!  single-precision host;
 !  CMF_HIP_PRECISION=fp64 or =mixed in the environment chooses it explicitly.
 !  Options carried to the device: LKINE, LSLPMIX (with the host's I2MASK),
 !  LDAMOUT with LDAMH22 (each dam's own FldVol*1.E-6/upreal) and LDAMYBY, LLEVEE,

@@ -39,7 +39,7 @@ extern "C" {
 /* Device arithmetic of the flux/stage variables (JPRB on the device). Storages,
  * the storage update, the stage inversion and the budget sums are FP64 in both. */
 #define CMF_PREC_FP64   0   /* reference FP64 build semantics */
-#define CMF_PREC_MIXED  1   /* FP32 flux/stage arithmetic, FP64 storage */
+#define CMF_PREC_MIXED  1   /* FP32 flux arithmetic and flux/stage arrays, FP64 storage */
 
 typedef struct cmf_hip_ctx* cmf_handle;
 

@@ -161,8 +161,8 @@ def test_dropin_restart_matches_reference(name, exe, tmp_path):
 def test_single_precision_dropin_matches_reference(name, tmp_path):
     """The reference built in its default single precision (-DSinglePrec_CMF,
     JPRB = 4) linked with this package's module: the JPRB arrays cross the C-ABI
-    as float, the device runs its mixed path (FP32 flux and stage arithmetic,
-    FP64 storage). Against the reference's own single-precision run of the same
+    as float, the device runs its mixed path (FP32 flux arithmetic and flux/stage
+    arrays, FP64 storage and stage inversion). Against the reference's own single-precision run of the same
     inputs (tests/golden/*_sp: paths, levees, dams; daily and hourly DT):
     identical NT at every step, final storage within 2e-5 of the largest cell
     storage and 1e-6 in total, outflow within 1e-3 of the largest. The two single-
