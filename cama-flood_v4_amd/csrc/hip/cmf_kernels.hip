@@ -899,11 +899,12 @@ __device__ __forceinline__ void update_tile(const DevMap& M, const DevState<R>& 
     }
 }
 
-// The mixed-precision ensemble outflow is held to 64 VGPRs (8 waves per SIMD;
-// it needs 66 without the bound, 7 waves) and spills nothing; the single-model
-// forms would spill and keep the compiler's choice.
+// The mixed-precision ensemble outflow of the default scheme is held to 64 VGPRs
+// (8 waves per SIMD) and spills nothing; the single-model forms and the
+// kinematic / slope-mixed ensemble form would spill under that bound and keep the
+// compiler's choice.
 template <class R, int BS, bool MB, bool KINE>
-__global__ __launch_bounds__(BS, (MB && sizeof(R) == 4) ? 8 : 1) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
+__global__ __launch_bounds__(BS, (MB && sizeof(R) == 4 && !KINE) ? 8 : 1) void k_outflow(DevMap M, DevState<R> S, StepArgs A, int ntc, int nt_all) {
     const int t = tile_of(blockIdx.x, gridDim.x, nt_all, A.xcd);
     if (t < nt_all) outflow_tile<R, BS, MB, KINE>(M, S, A, t, ntc);
 }
