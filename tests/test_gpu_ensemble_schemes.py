@@ -3,10 +3,7 @@ schemes: every instance (domain, member) of one ensemble handle must reproduce,
 bit for bit, a single-model handle run on that domain with that member's runoff,
 as tests/test_gpu_ensemble.py checks for the default scheme. The single-model
 kinematic and slope-mixed paths are pinned against the reference by the golden
-tests (g_kine, g_slpmix).
-
-Not yet collected (no test_ prefix): it has not had its first GPU run. Run it by
-naming the file: python -m pytest tests/wip_ensemble_schemes.py -m gpu"""
+tests (g_kine, g_slpmix). Paths, levees and dams on both domains; FP64 and mixed."""
 import numpy as np
 import pytest
 
